@@ -1,0 +1,217 @@
+/*
+ * opticks_amd.h : C-ABI of the MI355X optical-photon propagation path.
+ *
+ * This is the drop-in boundary for the reference's simulate call:
+ * gensteps in, photons / per-photon history (seq) / hits out.
+ * It replaces, one for one:
+ *
+ *   reference                                          here
+ *   ---------------------------------------------------------------------------
+ *   SSimulator (sysrap/SSimulator.h) implemented by     mi_sim_* (a handle that
+ *     CSGOptiX (CSGOptiX/CSGOptiX.cc:349,755,1218)        owns geometry, physics
+ *                                                         tables and device buffers)
+ *   CSGOptiX::Create(CSGFoundry*) +                     mi_sim_create
+ *     QSim::UploadComponents (qudarap/QSim.cc:119)
+ *   NP* QSim::simulate(const NP* gs, int eventID)       mi_sim_simulate (host arrays)
+ *     (qudarap/QSim.cc:720-755)
+ *   QSim::simulate(eventID, reset) per-slice launches   mi_sim_simulate_device
+ *     (qudarap/QSim.cc:436-536, CSGOptiX.cc:1091-1189)    (device-resident arrays)
+ *   SSimulator::reset(eventID) (G4CXOpticks.cc:491-511)  mi_sim_reset
+ *   CSGOptiX simtrace (CSGOptiX7.cu:563-604)            mi_sim_trace_device
+ *   (no counterpart: batched events in one launch)      mi_sim_simulate_batch_device
+ *
+ * Array layouts are the reference's, so arrays exported by the reference
+ * (CSGFoundry / SSim folders) can be passed unchanged:
+ *   genstep  (G,6,4)  f32/i32  quad6: storch (sysrap/storch.h:42-74) or
+ *                                     sscint (sysrap/sscint.h:34-62)
+ *   photon   (N,4,4)  f32/u32  sphoton (sysrap/sphoton.h:171-246)
+ *   seq      (N,2,2)  u64      [seqhis[2], seqbnd[2]] (sysrap/sseq.h:54-66,
+ *                              sysrap/SEvt.cc:3754-3756)
+ *   hit      (M,4,4)  f32/u32  photons with flagmask & hit_mask, photon order
+ *                              (qudarap/QEvt.cc:1113-1132, sysrap/SU.cu:145-160)
+ *   node     (n,4,4)  CSGNode  (CSG/CSGNode.h:196)
+ *   prim     (n,4,4)  CSGPrim  (CSG/CSGPrim.h:75)
+ *   solid    (n) 48 B CSGSolid (CSG/CSGSolid.h:37)
+ *   itra     (n,4,4)  qat4 inverse node transforms (sysrap/sqat4.h)
+ *   inst     (n,4,4)  qat4 instance transforms, identity in the w column
+ *                     (sysrap/sqat4.h:345-390)
+ *   bnd      (nbnd,4,2,nwl,4) f32 boundary table (qudarap/QBnd.cc:130-175)
+ *   optical  (nbnd,4,4) i32 (sysrap/sstandard.h make_optical)
+ *   icdf     (nlayer, nicdf) f32 scintillation inverse CDF (qudarap/QScint.cc:25-52)
+ *
+ * Errors: no exceptions cross this boundary. Functions return MI_OK (0),
+ * MI_NOTHING_TO_DO (1, e.g. zero gensteps, cf. QEvt.cc:436 rc=1) or a negative
+ * MI_ERR_* code; mi_last_error() gives a thread-local message.
+ * Threading: a handle is not re-entrant; the caller serializes calls on it.
+ * Several handles may share one device. All device work of a handle runs on
+ * the HIP stream given to the call (NULL: the handle's own stream).
+ */
+#ifndef OPTICKS_AMD_H
+#define OPTICKS_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_OK               0
+#define MI_NOTHING_TO_DO    1
+#define MI_ERR_ARG         -1
+#define MI_ERR_HIP         -2
+#define MI_ERR_UNSUPPORTED -3
+#define MI_ERR_CAPACITY    -4
+#define MI_ERR_GEOMETRY    -5
+
+/* output_mask bits */
+#define MI_OUT_PHOTON 0x1u
+#define MI_OUT_SEQ    0x2u
+#define MI_OUT_HIT    0x4u
+
+/* rng_kind */
+#define MI_RNG_PHILOX    0    /* Philox4x32-10, counter keyed on absolute photon index */
+#define MI_RNG_PRECOOKED 1    /* uniforms read from a caller array (aligned comparison) */
+
+/* kernel: which propagation kernel runs the bounce loop */
+#define MI_KERNEL_PLAIN      0   /* one thread per photon, CSGOptiX7.cu:432-530 shape */
+#define MI_KERNEL_WAVEFRONT  1   /* persistent waves that refill finished lanes */
+
+typedef struct mi_geometry {
+    const float*   node;  int64_t n_node;
+    const float*   prim;  int64_t n_prim;
+    const void*    solid; int64_t n_solid;
+    const float*   itra;  int64_t n_tran;
+    const float*   plan;  int64_t n_plan;
+    const float*   inst;  int64_t n_inst;
+} mi_geometry;
+
+typedef struct mi_physics {
+    const float*   bnd;            /* (n_bnd,4,2,n_wl,4)            */
+    int32_t        n_bnd;
+    int32_t        n_wl;           /* 761 for the 60..820 nm domain  */
+    float          wl_low;         /* 60 nm  (sysrap/sdomain.h:26-31) */
+    float          wl_step;        /* 1 nm                            */
+    const int32_t* optical;        /* (n_bnd,4,4)                     */
+    const float*   icdf;           /* (n_icdf_layer, n_icdf) or NULL  */
+    int32_t        n_icdf_layer;   /* 1 or 3                          */
+    int32_t        n_icdf;         /* 4096                            */
+    int32_t        icdf_hd_factor; /* 0, 10 or 20 (qudarap/qscint.h:160-200) */
+} mi_physics;
+
+/*
+ * Run configuration: names and defaults follow SEventConfig
+ * (sysrap/SEventConfig.cc:41-110); mi_config_default fills them in.
+ */
+typedef struct mi_config {
+    int32_t  max_bounce;       /* OPTICKS_MAX_BOUNCE         31      */
+    float    max_time;         /* OPTICKS_MAX_TIME           1e27 ns */
+    float    tmin;             /* OPTICKS_PROPAGATE_EPSILON  0.05 mm */
+    float    tmin0;            /* OPTICKS_PROPAGATE_EPSILON0 0.05 mm */
+    uint32_t eps0_mask;        /* OPTICKS_PROPAGATE_EPSILON0_MASK TO,CK,SI,SC,RE */
+    float    tmax;             /* 1e6 mm (CSGOptiX/CSGOptiX.cc:662) */
+    uint32_t hit_mask;         /* OPTICKS_HIT_MASK "SD"           */
+    uint64_t seed;             /* QRng__SEED_OFFSET seed   (0)    */
+    uint64_t rng_offset;       /* QRng__SEED_OFFSET offset (0)    */
+    uint64_t event_skipahead;  /* OPTICKS_EVENT_SKIPAHEAD 100000  */
+    int32_t  rng_kind;         /* MI_RNG_*                        */
+    const float* precooked;    /* (precooked_ni, precooked_nk) uniforms, host */
+    int64_t  precooked_ni;
+    int64_t  precooked_nk;
+    uint32_t output_mask;      /* MI_OUT_* bits                   */
+    int64_t  max_slot;         /* photons per launch, 0 = from free VRAM */
+    int32_t  kernel;           /* MI_KERNEL_*                     */
+    int32_t  block_size;       /* threads per block (64..1024, multiple of 64) */
+    int32_t  refill_period;    /* wavefront kernel: bounces between lane refills */
+    int32_t  bvh_leaf_size;    /* instances per TLAS leaf (1..8)  */
+    int32_t  waves_per_cu;     /* wavefront kernel: resident waves per CU (0 = auto) */
+} mi_config;
+
+typedef struct mi_sim mi_sim;
+
+/* Host-side results of mi_sim_simulate, owned by the handle until the next
+   simulate/reset/destroy (the reference keeps the SEvt fold until reset). */
+typedef struct mi_event_out {
+    int64_t         n_photon;
+    int64_t         n_hit;
+    const float*    photon;     /* (n_photon,4,4) or NULL when not in output_mask */
+    const uint64_t* seq;        /* (n_photon,2,2) or NULL */
+    const float*    hit;        /* (n_hit,4,4)    or NULL */
+    double          device_ms;  /* seeding -> hit compaction, HIP events */
+} mi_event_out;
+
+void        mi_config_default(mi_config* cfg);
+const char* mi_last_error(void);
+const char* mi_version(void);
+
+int  mi_sim_create(const mi_geometry* geo, const mi_physics* phys,
+                   const mi_config* cfg, mi_sim** out);
+void mi_sim_destroy(mi_sim* sim);
+int  mi_sim_set_config(mi_sim* sim, const mi_config* cfg);
+int  mi_sim_get_config(const mi_sim* sim, mi_config* cfg);
+
+/* gensteps (n_gs,6,4) in host memory; results copied to handle-owned host buffers. */
+int  mi_sim_simulate(mi_sim* sim, int32_t event_id, const float* gensteps,
+                     int64_t n_gs, mi_event_out* out);
+int  mi_sim_reset(mi_sim* sim, int32_t event_id);
+
+/*
+ * Device-resident form. d_gensteps: (n_gs,6,4) in device memory.
+ * n_photon: total photons of the gensteps when known, or -1 to have the
+ * library sum them on the device (one 8-byte read back).
+ * d_photon (n_photon,4,4), d_seq (n_photon,2,2) u64, d_hit (hit_capacity,4,4):
+ * caller-owned device buffers; any may be NULL to skip that output.
+ * photon_offset: absolute index of the first photon (multi-launch slicing,
+ * CSGOptiX/Params.h photon_slot_offset). *n_hit_out gets the hit count.
+ * stream: hipStream_t or NULL for the handle's stream. Synchronous on return.
+ */
+int  mi_sim_simulate_device(mi_sim* sim, int32_t event_id,
+                            const void* d_gensteps, int64_t n_gs,
+                            int64_t n_photon, int64_t photon_offset,
+                            void* d_photon, void* d_seq,
+                            void* d_hit, int64_t hit_capacity, int64_t* n_hit_out,
+                            void* stream);
+
+/*
+ * Batched form: n_event events, event e owns gensteps
+ * [gs_offsets[e], gs_offsets[e+1]) of d_gensteps and photons
+ * [photon_offsets[e], photon_offsets[e+1]) of d_photon/d_seq, all propagated
+ * by one persistent launch. Each event keeps its own RNG streams
+ * (event_ids[e] skipahead, photon index from 0 within the event), so results
+ * equal per-event mi_sim_simulate_device calls. gs_offsets/event_ids/
+ * photon_offsets are host arrays; photon_offsets is filled by the call.
+ * Hits of all events are compacted in event order into d_hit;
+ * hit_offsets (host, n_event+1) receives the per-event ranges.
+ */
+int  mi_sim_simulate_batch_device(mi_sim* sim, int32_t n_event,
+                                  const int32_t* event_ids,
+                                  const int64_t* gs_offsets,
+                                  const void* d_gensteps,
+                                  int64_t* photon_offsets,
+                                  void* d_photon, void* d_seq,
+                                  void* d_hit, int64_t hit_capacity,
+                                  int64_t* hit_offsets,
+                                  void* stream);
+
+/*
+ * Nearest-hit query over the whole geometry for n rays, the trace() +
+ * closest-hit contract of CSGOptiX7.cu:113-218,776-874:
+ * d_rays  (n,2,4) f32: (pos.xyz, tmin), (dir.xyz, tmax)
+ * d_isect (n,2,4):     (world normal.xyz normalized, t), then u32
+ *                      (iindex<<16|identity, boundary (0xffff = miss),
+ *                       lposcost as f32 bits, instance index)
+ */
+int  mi_sim_trace_device(mi_sim* sim, const void* d_rays, int64_t n,
+                         void* d_isect, void* stream);
+
+/* Replace instance transforms (n_inst,4,4) and refit the TLAS on the GPU. */
+int  mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst);
+
+/* Last launch timings in ms (HIP events): [seed, propagate, compact, total]. */
+int  mi_sim_last_timing(const mi_sim* sim, double* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTICKS_AMD_H */

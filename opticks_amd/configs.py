@@ -1,0 +1,207 @@
+"""
+The benchmark workloads, built in code (the reference ships no data fixtures).
+
+  config0  1M photons, one torch genstep (T_SPHERE, radius 0: isotropic from the
+           origin), 440 nm, seed 42. Water sphere r=1000 mm (n 1.333, absorption
+           20 m, Rayleigh 30 m) in a vacuum box of half-side 2000 mm whose inner
+           surface detects with efficiency 1.
+  config1  10M photons from 1,000 torch gensteps placed uniformly in the r<5 m
+           liquid-scintillator ball, isotropic, wavelength uniform in 200-800 nm.
+           LS sphere r=5000 (n 1.49, absorption 25 m, Rayleigh 40 m, reemission 0.4
+           below 410 nm, Gaussian emission 430+-20 nm) inside a 120 mm acrylic shell
+           (n 1.50) in water (n 1.33) with 2,000 instanced detectors on a Fibonacci
+           lattice at r=5.5 m, local +z towards the centre. Each detector is
+           difference(union(z-cut ellipsoid 254/254/190 mm z>-50, cylinder r127
+           h200), cylinder r120) with a 0.25 detect / 0.75 absorb surface.
+  config2  64 events x Poisson(200) scintillation gensteps x Poisson(5000)
+           photons in the config1 geometry scaled to 20,000 detectors at r=19.4 m
+           around a 17.7 m LS sphere.
+
+Choices the workload statements leave open (documented, fixed here):
+  * world box half-side 6500 mm (config1) / 21000 mm (config2), inner surface absorbs;
+  * water absorption 20 m and Rayleigh 30 m (as config0), acrylic absorption and
+    Rayleigh 1e6 mm; detector solids are vacuum inside;
+  * detector neck z in [-250,-40] mm, inner bore z in [-260,-60] mm (solid frame);
+  * torch subtype T_POINT_ISO=100: T_SPHERE directions about gs.pos, wavelength
+    uniform in [gs.wavelength, gs.wavelength+gs.weight] (sysrap/storch.h has no
+    isotropic point source at gs.pos);
+  * genstep placement and Poisson counts from numpy.random.default_rng(seed).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .foundry import (Foundry, Node, CSG_UNION, CSG_DIFFERENCE, scale_transform,
+                      rotation_z_to, fibonacci_sphere)
+from .physics import BoundaryLib, Material, Surface, gaussian_icdf, EMS_SURFACE
+
+# sysrap/OpticksGenstep.h, sysrap/storchtype.h
+OpticksGenstep_TORCH = 6
+OpticksGenstep_SCINTILLATION = 16
+T_DISC, T_LINE, T_POINT, T_CIRCLE, T_RECTANGLE, T_SPHERE_MARSAGLIA, T_SPHERE = 1, 2, 3, 4, 5, 6, 7
+T_POINT_ISO = 100
+
+
+def torch_genstep(numphoton, type_=T_SPHERE, pos=(0, 0, 0), time=0.0, mom=(0, 0, 1), weight=0.0,
+                  pol=(1, 0, 0), wavelength=440.0, zenith=(0.0, 1.0), azimuth=(0.0, 1.0),
+                  radius=0.0, distance=0.0, mode=0):
+    """quad6 genstep with the storch layout (sysrap/storch.h:42-74)."""
+    g = np.zeros(24, np.float32)
+    u = g.view(np.uint32)
+    u[0] = OpticksGenstep_TORCH
+    u[3] = numphoton
+    g[4:7] = pos
+    g[7] = time
+    g[8:11] = mom
+    g[11] = weight
+    g[12:15] = pol
+    g[15] = wavelength
+    g[16:18] = zenith
+    g[18:20] = azimuth
+    g[20] = radius
+    g[21] = distance
+    u[22] = mode
+    u[23] = type_
+    return g.reshape(6, 4)
+
+
+def scint_genstep(numphoton, pos, time=0.0, delta=(0, 0, 0), step_length=0.0, charge=0.0,
+                  weight=1.0, mean_velocity=1.0, scint_time=0.0):
+    """quad6 genstep with the sscint layout (sysrap/sscint.h:34-62)."""
+    g = np.zeros(24, np.float32)
+    u = g.view(np.uint32)
+    u[0] = OpticksGenstep_SCINTILLATION
+    u[3] = numphoton
+    g[4:7] = pos
+    g[7] = time
+    g[8:11] = delta
+    g[11] = step_length
+    g[13] = charge
+    g[14] = weight
+    g[15] = mean_velocity
+    g[20] = scint_time
+    return g.reshape(6, 4)
+
+
+class Workload:
+    def __init__(self, name, geometry, physics, gensteps, seed=42, events=None, meta=None):
+        self.name = name
+        self.geometry = geometry      # dict of foundry arrays
+        self.physics = physics        # dict bnd/optical/... + icdf
+        self.gensteps = gensteps      # (G,6,4) float32 (single event)
+        self.events = events          # list of (G_e,6,4) for batched workloads
+        self.seed = seed
+        self.meta = meta or {}
+
+    @property
+    def num_photon(self):
+        if self.events is not None:
+            return int(sum(e.view(np.uint32)[:, 0, 3].sum() for e in self.events))
+        return int(self.gensteps.view(np.uint32)[:, 0, 3].sum())
+
+
+def _scint_materials(lib, water_n=1.33):
+    lib.add_material(Material("Vacuum", 1.0, 1e9, 1e9))
+    lib.add_material(Material("Water", water_n, 20000.0, 30000.0))
+    lib.add_material(Material("Acrylic", 1.50, 1e6, 1e6))
+    lib.add_material(Material("LS", 1.49, 25000.0, 40000.0,
+                              reemission=lambda wl: np.where(wl < 410.0, 0.4, 0.0)))
+    lib.add_material(Material("Rock", 1.5, 1.0, 1e9))
+    lib.add_surface(Surface("absSurf", detect=0.0, absorb=1.0))
+    lib.add_surface(Surface("detSurf", detect=0.25, absorb=0.75))
+
+
+def detector_tree():
+    """difference(union(z-cut ellipsoid, neck cylinder), inner bore cylinder)."""
+    ell = Node.zsphere(190.0, -50.0, 190.0)
+    ell.transform = scale_transform(254.0 / 190.0, 254.0 / 190.0, 1.0)
+    neck = Node.cylinder(127.0, -250.0, -40.0)
+    bore = Node.cylinder(120.0, -260.0, -60.0)
+    return Node.op(CSG_DIFFERENCE, Node.op(CSG_UNION, ell, neck), bore)
+
+
+def config0(num_photon=1_000_000, seed=42):
+    lib = BoundaryLib()
+    lib.add_material(Material("Vacuum", 1.0, 1e9, 1e9))
+    lib.add_material(Material("Water", 1.333, 20000.0, 30000.0))
+    lib.add_surface(Surface("detSurf", detect=1.0, absorb=0.0))
+    b_world = lib.boundary("Vacuum/detSurf/detSurf/Vacuum")
+    b_sphere = lib.boundary("Vacuum///Water")
+    fd = Foundry()
+    p0 = fd.addPrim(Node.box3(4000.0, 4000.0, 4000.0), b_world)
+    p1 = fd.addPrim(Node.sphere(1000.0), b_sphere)
+    s0 = fd.addSolid("world", [p0, p1])
+    fd.addInstance(np.eye(4), s0)
+    phys = lib.arrays()
+    phys["icdf"], phys["icdf_hd_factor"] = gaussian_icdf()
+    phys["boundary_names"] = lib.boundaries
+    gs = torch_genstep(num_photon, T_SPHERE, wavelength=440.0, radius=0.0, distance=0.0)[None]
+    return Workload("config0", fd.arrays(), phys, gs, seed=seed,
+                    meta=dict(world_half=2000.0, sphere_r=1000.0))
+
+
+def _scint_world(num_det, det_r, ls_r, shell, world_half):
+    lib = BoundaryLib()
+    _scint_materials(lib)
+    b_world = lib.boundary("Rock//absSurf/Water")
+    b_acr = lib.boundary("Water///Acrylic")
+    b_ls = lib.boundary("Acrylic///LS")
+    b_det = lib.boundary("Water/detSurf//Vacuum")
+    fd = Foundry()
+    p0 = fd.addPrim(Node.box3(2 * world_half, 2 * world_half, 2 * world_half), b_world)
+    p1 = fd.addPrim(Node.sphere(ls_r + shell), b_acr)
+    p2 = fd.addPrim(Node.sphere(ls_r), b_ls)
+    s0 = fd.addSolid("global", [p0, p1, p2])
+    pd = fd.addPrim(detector_tree(), b_det, mesh_idx=1)
+    s1 = fd.addSolid("detector", [pd])
+    fd.addInstance(np.eye(4), s0)
+    dirs = fibonacci_sphere(num_det)
+    for i, d in enumerate(dirs):
+        t = rotation_z_to(-d)
+        t[3, :3] = det_r * d
+        fd.addInstance(t, s1, sensor_identifier=i, sensor_index=i)
+    phys = lib.arrays()
+    phys["icdf"], phys["icdf_hd_factor"] = gaussian_icdf()
+    phys["boundary_names"] = lib.boundaries
+    return fd.arrays(), phys
+
+
+def uniform_ball(rng, n, r):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return d * (r * rng.random(n) ** (1.0 / 3.0))[:, None]
+
+
+def config1(num_gs=1000, per_gs=10_000, seed=42):
+    geo, phys = _scint_world(2000, 5500.0, 5000.0, 120.0, 6500.0)
+    rng = np.random.default_rng(seed)
+    pos = uniform_ball(rng, num_gs, 5000.0)
+    gs = np.stack([torch_genstep(per_gs, T_POINT_ISO, pos=p, wavelength=200.0, weight=600.0)
+                   for p in pos]).astype(np.float32)
+    return Workload("config1", geo, phys, gs, seed=seed,
+                    meta=dict(ls_r=5000.0, det_r=5500.0, num_det=2000, world_half=6500.0))
+
+
+def config2(num_events=64, mean_gs=200, mean_photons=5000, seed=42):
+    geo, phys = _scint_world(20000, 19400.0, 17700.0, 120.0, 21000.0)
+    events = []
+    for e in range(num_events):
+        rng = np.random.default_rng([seed, e])
+        ng = max(1, int(rng.poisson(mean_gs)))
+        counts = rng.poisson(mean_photons, size=ng).astype(np.int64)
+        counts[counts == 0] = 1
+        pos = uniform_ball(rng, ng, 17700.0)
+        dirs = rng.normal(size=(ng, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        gs = np.stack([scint_genstep(int(c), p, time=float(rng.random() * 10.0),
+                                     delta=1.0 * d, step_length=1.0, charge=1.0,
+                                     mean_velocity=0.9 * 299.792458, scint_time=1.5)
+                       for c, p, d in zip(counts, pos, dirs)]).astype(np.float32)
+        events.append(gs)
+    return Workload("config2", geo, phys, None, seed=seed, events=events,
+                    meta=dict(ls_r=17700.0, det_r=19400.0, num_det=20000, world_half=21000.0))
+
+
+def scaled_config1(num_gs, per_gs, seed=42):
+    """config1 geometry with fewer photons (parity-test sizes)."""
+    return config1(num_gs=num_gs, per_gs=per_gs, seed=seed)

@@ -1,0 +1,221 @@
+/*
+ * oa_portable.h : bit-reproducible FP32 transcendentals and the Philox4x32-10
+ * counter-based generator, shared by the HIP kernels and by the CPU checker.
+ *
+ * Everything here is plain C99 arithmetic (no libm calls apart from floorf and
+ * sqrtf, which are correctly rounded on both targets) so that gcc and hipcc,
+ * both invoked with -ffp-contract=off, produce the same bits.
+ *
+ * Transcendentals (written from the mathematics, methods named per function):
+ *   oa_logf    Cody-Waite reduction x = 2^k m, m in [sqrt(1/2), sqrt(2)),
+ *              log(m) = 2 atanh(s), s = (m-1)/(m+1), evaluated in the
+ *              "f - (hfsq - s(hfsq+R))" form of Sun's fdlibm e_logf.c with
+ *              R the truncated Taylor series of 2 atanh(s)/s - 2
+ *              (coefficients 2/(2i+1)); ln2 split into a 12-bit head and a tail.
+ *   oa_sinf / oa_cosf / oa_sincosf
+ *              Cody-Waite reduction by pi/2 (three-part split, 12+12+24 bits)
+ *              and Taylor polynomials on [-pi/4, pi/4] (sin to r^9, cos to r^10).
+ *   Valid domains: oa_logf for positive finite normal x; sin/cos for |x| < 1e4.
+ *   Accuracy is pinned against glibc in tests/test_portable_math.py.
+ *
+ * Random numbers:
+ *   Philox4x32-10 block function from J. Salmon, M. Moraes, R. Dror, D. Shaw,
+ *   "Parallel random numbers: as easy as 1, 2, 3", SC'11 (multipliers
+ *   0xD2511F53/0xCD9E8D57, Weyl key increments 0x9E3779B9/0xBB67AE85).
+ *   Stream convention of this project (documented, not cuRAND's):
+ *     key     = (seed_lo32, seed_hi32)
+ *     counter = (block_lo32, block_hi32, photon_lo32, photon_hi32)
+ *   output word j of a photon's stream is word (j%4) of block j/4.
+ *   The reference keys each photon's stream on the absolute photon index and
+ *   separates events by a skipahead of event_idx*skipahead_event_offset words
+ *   (qudarap/qrng.h:131-137); oa_rng_init reproduces that structure.
+ *   Uniform mapping: u = ((x >> 8) + 1) * 2^-24, exactly representable, in
+ *   (0, 1] like curand_uniform's documented range (logf(u) must be finite).
+ */
+#ifndef OA_PORTABLE_H
+#define OA_PORTABLE_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define OA_FN __host__ __device__ static inline __attribute__((always_inline))
+#else
+#define OA_FN static inline
+#endif
+
+#if !defined(__HIPCC__)
+#include <math.h>
+#endif
+
+OA_FN uint32_t oa_f2u(float f) { union { float f; uint32_t u; } c; c.f = f; return c.u; }
+OA_FN float oa_u2f(uint32_t u) { union { float f; uint32_t u; } c; c.u = u; return c.f; }
+
+/* ------------------------------------------------------------------ log */
+
+#define OA_LN2_HI   0.693115234375f            /* 0x3f317000: ln2 to 12 bits  */
+#define OA_LN2_LO   3.1946183298714459e-05f    /* 0x3805fdf4: ln2 - OA_LN2_HI */
+#define OA_SQRT2_F  1.41421353816986083984f    /* 0x3fb504f3                  */
+
+OA_FN float oa_logf(float x)
+{
+    uint32_t ix = oa_f2u(x);
+    int k = (int)(ix >> 23) - 127;
+    float m = oa_u2f((ix & 0x007fffffu) | 0x3f800000u);   /* m in [1,2) */
+    if (m > OA_SQRT2_F) { m = m * 0.5f; k = k + 1; }      /* exact       */
+    float f = m - 1.0f;                                   /* exact (Sterbenz) */
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float R = z * (0.666666686534881591796875f             /* 2/3  */
+            + z * (0.4000000059604644775390625f            /* 2/5  */
+            + z * (0.2857142984867095947265625f            /* 2/7  */
+            + z * (0.2222222238779067993164062f            /* 2/9  */
+            + z * (0.1818181872367858886718750f)))));      /* 2/11 */
+    float hfsq = 0.5f * f * f;
+    float dk = (float)k;
+    return dk * OA_LN2_HI - ((hfsq - (s * (hfsq + R) + dk * OA_LN2_LO)) - f);
+}
+
+/* -------------------------------------------------------------- sin/cos */
+
+#define OA_TWO_OVER_PI 0.636619746685028076171875f  /* 0x3f22f983 */
+#define OA_PIO2_1 1.5703125f                        /* 0x3fc90000 */
+#define OA_PIO2_2 4.8375129699707031e-04f           /* 0x39fda000 */
+#define OA_PIO2_3 7.5497901264043321e-08f           /* 0x33a22169 */
+
+OA_FN float oa_sin_poly(float r)
+{
+    float r2 = r * r;
+    return r + r * r2 * (-0.16666667163372039794921875f          /* -1/3!  */
+             + r2 * (8.33333376795053482055664062e-03f           /*  1/5!  */
+             + r2 * (-1.98412701138295233249664307e-04f          /* -1/7!  */
+             + r2 * (2.75573188446287531405687332e-06f))));      /*  1/9!  */
+}
+
+OA_FN float oa_cos_poly(float r)
+{
+    float r2 = r * r;
+    return 1.0f - 0.5f * r2 + r2 * r2 * (4.16666679084300994873046875e-02f  /*  1/4!  */
+             + r2 * (-1.38888892251998186111450195e-03f                    /* -1/6!  */
+             + r2 * (2.48015876569843292236328125e-05f                     /*  1/8!  */
+             + r2 * (-2.75573199814971303567290306e-07f))));               /* -1/10! */
+}
+
+OA_FN float oa_reduce_pio2(float x, int* quadrant)
+{
+    float kf = floorf(x * OA_TWO_OVER_PI + 0.5f);
+    *quadrant = ((int)kf) & 3;
+    return ((x - kf * OA_PIO2_1) - kf * OA_PIO2_2) - kf * OA_PIO2_3;
+}
+
+OA_FN void oa_sincosf(float x, float* sn, float* cs)
+{
+    int q;
+    float r = oa_reduce_pio2(x, &q);
+    float s = oa_sin_poly(r);
+    float c = oa_cos_poly(r);
+    float so = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
+    float co = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
+    *sn = so;
+    *cs = co;
+}
+
+OA_FN float oa_sinf(float x) { float s, c; oa_sincosf(x, &s, &c); return s; }
+OA_FN float oa_cosf(float x) { float s, c; oa_sincosf(x, &s, &c); return c; }
+
+/* --------------------------------------------------------------- Philox */
+
+#define OA_PHILOX_M0 0xD2511F53u
+#define OA_PHILOX_M1 0xCD9E8D57u
+#define OA_PHILOX_W0 0x9E3779B9u
+#define OA_PHILOX_W1 0xBB67AE85u
+
+typedef struct { uint32_t v[4]; } oa_u32x4;
+
+OA_FN void oa_mulhilo(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo)
+{
+    uint64_t p = (uint64_t)a * (uint64_t)b;
+    *hi = (uint32_t)(p >> 32);
+    *lo = (uint32_t)p;
+}
+
+/* One Philox4x32-10 block: 10 rounds with 9 key bumps in between. */
+OA_FN oa_u32x4 oa_philox4x32_10(oa_u32x4 ctr, uint32_t k0, uint32_t k1)
+{
+    uint32_t c0 = ctr.v[0], c1 = ctr.v[1], c2 = ctr.v[2], c3 = ctr.v[3];
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0, lo0, hi1, lo1;
+        oa_mulhilo(OA_PHILOX_M0, c0, &hi0, &lo0);
+        oa_mulhilo(OA_PHILOX_M1, c2, &hi1, &lo1);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += OA_PHILOX_W0;
+        k1 += OA_PHILOX_W1;
+    }
+    oa_u32x4 out;
+    out.v[0] = c0; out.v[1] = c1; out.v[2] = c2; out.v[3] = c3;
+    return out;
+}
+
+OA_FN float oa_u32_to_uniform(uint32_t x)
+{
+    return (float)((x >> 8) + 1u) * 5.9604644775390625e-08f;   /* 2^-24 */
+}
+
+/*
+ * Per-photon stream. Registers only: no dynamic indexing of the cached block,
+ * so the HIP compiler keeps it out of scratch memory.
+ */
+typedef struct {
+    uint32_t k0, k1;          /* key = seed                        */
+    uint32_t p0, p1;          /* photon index (counter words 2,3)  */
+    uint64_t pos;             /* absolute output-word position     */
+    uint32_t b0, b1, b2, b3;  /* cached block for pos/4            */
+    uint64_t cached_block;    /* block index held in b0..b3        */
+} oa_philox_stream;
+
+OA_FN void oa_philox_fill(oa_philox_stream* s, uint64_t block)
+{
+    oa_u32x4 c;
+    c.v[0] = (uint32_t)block;
+    c.v[1] = (uint32_t)(block >> 32);
+    c.v[2] = s->p0;
+    c.v[3] = s->p1;
+    oa_u32x4 o = oa_philox4x32_10(c, s->k0, s->k1);
+    s->b0 = o.v[0]; s->b1 = o.v[1]; s->b2 = o.v[2]; s->b3 = o.v[3];
+    s->cached_block = block;
+}
+
+/*
+ * oa_rng_init: stream for absolute photon index `photon`, advanced by
+ * `offset + event_idx*skipahead_event_offset` words, mirroring the
+ * subsequence/skipahead structure of qudarap/qrng.h:131-137.
+ */
+OA_FN void oa_rng_init(oa_philox_stream* s, uint64_t seed, uint64_t offset,
+                       uint64_t skipahead_event_offset, uint64_t event_idx,
+                       uint64_t photon)
+{
+    s->k0 = (uint32_t)seed;
+    s->k1 = (uint32_t)(seed >> 32);
+    s->p0 = (uint32_t)photon;
+    s->p1 = (uint32_t)(photon >> 32);
+    s->pos = offset + skipahead_event_offset * event_idx;
+    oa_philox_fill(s, s->pos >> 2);
+}
+
+OA_FN uint32_t oa_rng_next_u32(oa_philox_stream* s)
+{
+    uint64_t blk = s->pos >> 2;
+    if (blk != s->cached_block) oa_philox_fill(s, blk);
+    uint32_t w = (uint32_t)(s->pos & 3u);
+    uint32_t x = (w == 0u) ? s->b0 : (w == 1u) ? s->b1 : (w == 2u) ? s->b2 : s->b3;
+    s->pos += 1u;
+    return x;
+}
+
+OA_FN float oa_rng_uniform(oa_philox_stream* s)
+{
+    return oa_u32_to_uniform(oa_rng_next_u32(s));
+}
+
+#endif /* OA_PORTABLE_H */

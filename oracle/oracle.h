@@ -1,0 +1,113 @@
+/*
+ * oracle.h : CPU restatement of the reference's optical-photon hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP path is
+ * compared against (tests/, __graft_entry__.smoke, bench.py cpu_baseline).
+ * Nothing in the product (opticks_amd/, include/) links, loads or calls it.
+ *
+ * It restates, in plain C99 + OpenMP, one photon per iteration:
+ *   generation        qudarap/qsim.h:2532-2561, sysrap/storch.h:192-520,
+ *                     qudarap/qscint.h:66-200
+ *   bounce loop       CSGOptiX/CSGOptiX7.cu:432-530
+ *   world trace       optixTrace + __intersection__is/__closesthit__ch/__miss__ms
+ *                     (CSGOptiX7.cu:113-218,682-967): nearest hit over all
+ *                     instances x prims, here by brute force or a plain BVH
+ *   CSG intersect     CSG/csg_intersect_tree.h:277-719,
+ *                     CSG/csg_intersect_leaf.h:174-324 and leaf headers
+ *   physics           qudarap/qsim.h:180-201,433-575,604-866,1001-1208,
+ *                     1680-1758,1988-2093,2229-2338; qudarap/qbnd.h:108-223
+ *   history           sysrap/sseq.h:174-183, sysrap/sctx.h:134-182
+ *
+ * The bnd/icdf "textures" are read with FP32 linear interpolation and wrap
+ * addressing (qudarap/QTex.cc:293-305) instead of the GPU texture unit.
+ * Transcendentals and the Philox streams come from oa_portable.h, which is
+ * pinned separately against glibc and the Philox known-answer vectors.
+ */
+#ifndef OA_ORACLE_H
+#define OA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_geom {
+    const float* node;  int64_t n_node;
+    const float* prim;  int64_t n_prim;
+    const void*  solid; int64_t n_solid;
+    const float* itra;  int64_t n_tran;
+    const float* inst;  int64_t n_inst;
+} oracle_geom;
+
+typedef struct oracle_phys {
+    const float*   bnd; int32_t n_bnd; int32_t n_wl; float wl_low; float wl_step;
+    const int32_t* optical;
+    const float*   icdf; int32_t n_icdf_layer; int32_t n_icdf; int32_t icdf_hd_factor;
+} oracle_phys;
+
+typedef struct oracle_cfg {
+    int32_t  max_bounce;
+    float    max_time;
+    float    tmin;
+    float    tmin0;
+    uint32_t eps0_mask;
+    float    tmax;
+    uint64_t seed;
+    uint64_t rng_offset;
+    uint64_t event_skipahead;
+    int32_t  rng_kind;          /* 0 philox, 1 precooked */
+    const float* precooked;     /* (precooked_ni, precooked_nk)  */
+    int64_t  precooked_ni;
+    int64_t  precooked_nk;
+    int32_t  traverse;          /* 0 brute force, 1 BVH */
+    int32_t  nthreads;          /* OpenMP threads, 0 = default */
+} oracle_cfg;
+
+/* Build acceleration data for a geometry; returns an opaque handle. */
+void* oracle_world_create(const oracle_geom* g);
+void  oracle_world_destroy(void* w);
+
+/*
+ * Simulate photons [p_begin, p_end) of the event given by gensteps (n_gs,6,4).
+ * photon_out (p_end-p_begin,4,4) f32 and seq_out (p_end-p_begin,2,2) u64 may be
+ * NULL. Returns the total number of photons of the gensteps, or <0 on error.
+ * photon_offset is added to the in-event photon index to form the absolute
+ * RNG subsequence, as CSGOptiX7.cu:442 does with photon_slot_offset.
+ */
+int64_t oracle_simulate(void* world, const oracle_phys* ph, const oracle_cfg* cfg,
+                        int32_t event_idx, const float* gensteps, int64_t n_gs,
+                        int64_t photon_offset, int64_t p_begin, int64_t p_end,
+                        float* photon_out, uint64_t* seq_out);
+
+/* Nearest hit for n rays; same record layout as mi_sim_trace_device. */
+int oracle_trace(void* world, const float* rays, int64_t n, float* isect, int32_t traverse);
+
+/* Single prim intersect in its own frame (CSG/CSGQuery.cc:231-262 shape). */
+int oracle_intersect_prim(const oracle_geom* g, int32_t prim_idx, float t_min,
+                          const float* ray_origin, const float* ray_direction,
+                          float* isect4);
+
+/* The packed Boolean LUT words, for comparison with the reference constants. */
+void oracle_packed_lut(uint32_t* acloser3, uint32_t* bcloser3);
+
+/* Portable math entry points for the accuracy tests. */
+float oracle_logf(float x);
+void  oracle_sincosf(float x, float* s, float* c);
+void  oracle_philox_block(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4);
+void  oracle_uniforms(uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event_idx,
+                      uint64_t photon, int32_t n, float* out);
+
+/* Single physics steps for closed-form checks (qudarap/QSim.cu test kernels). */
+int oracle_step_boundary(const float* mom, const float* pol, const float* normal,
+                         float n1, float n2, float u_reflect,
+                         float* mom_out, float* pol_out, uint32_t* flag_out,
+                         float* transcoeff_out);
+void oracle_rayleigh(const float* mom, const float* pol, uint64_t seed, uint64_t photon,
+                     int32_t n, float* mom_out, float* pol_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
