@@ -1,0 +1,101 @@
+"""
+ctypes binding of the C-ABI in include/opticks_amd.h (libopticks_amd.so).
+
+The product path is the HIP library: importing this module when the library
+has not been built raises ImportError (no CPU fallback exists).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libopticks_amd.so")
+
+MI_OK, MI_NOTHING_TO_DO = 0, 1
+MI_OUT_PHOTON, MI_OUT_SEQ, MI_OUT_HIT = 0x1, 0x2, 0x4
+MI_RNG_PHILOX, MI_RNG_PRECOOKED = 0, 1
+MI_KERNEL_PLAIN, MI_KERNEL_WAVEFRONT = 0, 1
+
+
+class MiGeometry(C.Structure):
+    _fields_ = [("node", C.c_void_p), ("n_node", C.c_int64),
+                ("prim", C.c_void_p), ("n_prim", C.c_int64),
+                ("solid", C.c_void_p), ("n_solid", C.c_int64),
+                ("itra", C.c_void_p), ("n_tran", C.c_int64),
+                ("plan", C.c_void_p), ("n_plan", C.c_int64),
+                ("inst", C.c_void_p), ("n_inst", C.c_int64)]
+
+
+class MiPhysics(C.Structure):
+    _fields_ = [("bnd", C.c_void_p), ("n_bnd", C.c_int32), ("n_wl", C.c_int32),
+                ("wl_low", C.c_float), ("wl_step", C.c_float), ("optical", C.c_void_p),
+                ("icdf", C.c_void_p), ("n_icdf_layer", C.c_int32), ("n_icdf", C.c_int32),
+                ("icdf_hd_factor", C.c_int32)]
+
+
+class MiConfig(C.Structure):
+    _fields_ = [("max_bounce", C.c_int32), ("max_time", C.c_float), ("tmin", C.c_float),
+                ("tmin0", C.c_float), ("eps0_mask", C.c_uint32), ("tmax", C.c_float),
+                ("hit_mask", C.c_uint32), ("seed", C.c_uint64), ("rng_offset", C.c_uint64),
+                ("event_skipahead", C.c_uint64), ("rng_kind", C.c_int32),
+                ("precooked", C.c_void_p), ("precooked_ni", C.c_int64), ("precooked_nk", C.c_int64),
+                ("output_mask", C.c_uint32), ("max_slot", C.c_int64), ("kernel", C.c_int32),
+                ("block_size", C.c_int32), ("refill_period", C.c_int32), ("bvh_leaf_size", C.c_int32),
+                ("waves_per_cu", C.c_int32)]
+
+
+class MiEventOut(C.Structure):
+    _fields_ = [("n_photon", C.c_int64), ("n_hit", C.c_int64), ("photon", C.c_void_p),
+                ("seq", C.c_void_p), ("hit", C.c_void_p), ("device_ms", C.c_double)]
+
+
+SYMBOLS = ["mi_config_default", "mi_last_error", "mi_version", "mi_sim_create", "mi_sim_destroy",
+           "mi_sim_set_config", "mi_sim_get_config", "mi_sim_simulate", "mi_sim_reset",
+           "mi_sim_simulate_device", "mi_sim_simulate_batch_device", "mi_sim_trace_device",
+           "mi_sim_refit_instances", "mi_sim_last_timing"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} is missing: run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    L = C.CDLL(LIBPATH)
+    L.mi_config_default.argtypes = [C.POINTER(MiConfig)]
+    L.mi_last_error.restype = C.c_char_p
+    L.mi_version.restype = C.c_char_p
+    L.mi_sim_create.argtypes = [C.POINTER(MiGeometry), C.POINTER(MiPhysics), C.POINTER(MiConfig),
+                                C.POINTER(C.c_void_p)]
+    L.mi_sim_destroy.argtypes = [C.c_void_p]
+    L.mi_sim_set_config.argtypes = [C.c_void_p, C.POINTER(MiConfig)]
+    L.mi_sim_get_config.argtypes = [C.c_void_p, C.POINTER(MiConfig)]
+    L.mi_sim_simulate.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(MiEventOut)]
+    L.mi_sim_reset.argtypes = [C.c_void_p, C.c_int32]
+    L.mi_sim_simulate_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                         C.POINTER(C.c_int64), C.c_void_p]
+    L.mi_sim_simulate_batch_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                               C.c_void_p, C.c_void_p]
+    L.mi_sim_trace_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.mi_sim_refit_instances.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.mi_sim_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    for name in SYMBOLS:
+        if name not in ("mi_last_error", "mi_version", "mi_config_default", "mi_sim_destroy"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().mi_last_error().decode()
+
+
+def check(rc, what):
+    if rc < 0:
+        raise RuntimeError(f"{what} failed rc={rc}: {last_error()}")
+    return rc

@@ -1,0 +1,39 @@
+"""
+Build libopticks_amd.so in-tree with hipcc for gfx950.
+
+  python -m opticks_amd.build
+
+Flags: -ffp-contract=off keeps a*b+c unfused so results match the CPU checker
+bit for bit; packed FP32 instructions are disabled (-packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = [os.path.join(HERE, "csrc", f) for f in ("oa_kernels.hip", "oa_capi.cpp")]
+OUT = os.path.join(HERE, "libopticks_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+
+
+def build(verbose=False):
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    errs = [l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    if r.returncode != 0:
+        sys.stderr.write("\n".join(errs) + "\n")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+    if verbose and errs:
+        sys.stderr.write("\n".join(errs) + "\n")
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,763 @@
+/*
+ * oa_capi.cpp : the C-ABI runtime (include/opticks_amd.h).
+ *
+ * mi_sim_create uploads the CSGFoundry-layout geometry and the boundary
+ * tables once (the QSim::UploadComponents + CSGFoundry::upload role,
+ * qudarap/QSim.cc:119, CSG/CSGFoundry.cc:3392), builds one BLAS per solid and
+ * a TLAS over instances (the GAS/IAS role of CSGOptiX/SBT.cc), and owns one HIP
+ * stream. mi_sim_simulate* play QSim::simulate (qudarap/QSim.cc:436-620):
+ * genstep prefix (seeding), genstep slicing at max_slot photons with the
+ * absolute photon index carried into each slice (photon_slot_offset), the
+ * propagation launch and the stable hit compaction.
+ */
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <cstring>
+#include <cstdio>
+#include <cmath>
+#include <algorithm>
+#include <array>
+
+#include "../../include/opticks_amd.h"
+#include "oa_layout.h"
+#include "oa_bvh.h"
+#include "oa_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(MI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n && p) return hipSuccess;
+        if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; n = 0; }
+        size_t bytes = std::max<size_t>(want, 1) * sizeof(T);
+        hipError_t e = hipMalloc((void**)&p, bytes);
+        if (e == hipSuccess) n = std::max<size_t>(want, 1);
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+template <typename T>
+hipError_t upload(DBuf<T>& b, const T* src, size_t n, hipStream_t s)
+{
+    hipError_t e = b.reserve(n);
+    if (e != hipSuccess) return e;
+    if (n) e = hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+/* affine inverse in double, rounded once: same cofactor formula as the CPU checker */
+void invert_affine16(const float* m, float* out16)
+{
+    double a[3][3], t[3], inv[3][3];
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) a[r][c] = (double)m[4 * r + c];
+    for (int c = 0; c < 3; ++c) t[c] = (double)m[12 + c];
+    double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+    double c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+    double c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+    inv[0][0] = c00 / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = c01 / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = c02 / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) out16[4 * r + c] = (float)inv[r][c];
+        out16[4 * r + 3] = 0.f;
+    }
+    for (int c = 0; c < 3; ++c) out16[12 + c] = (float)(-(t[0] * inv[0][c] + t[1] * inv[1][c] + t[2] * inv[2][c]));
+    out16[15] = 1.f;
+}
+
+inline uint32_t u32_of(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float f32_of(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+int tree_height(uint32_t n)
+{
+    int h = 0;
+    while (((1u << (h + 1)) - 1u) < n) ++h;
+    return ((1u << (h + 1)) - 1u) == n ? h : -1;
+}
+
+} // namespace
+
+struct mi_sim {
+    mi_config cfg;
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
+    double last_ms[4] = { 0, 0, 0, 0 };
+
+    /* geometry */
+    int64_t n_inst = 0, n_prim = 0, n_tlas = 0;
+    uint32_t tlas_depth = 0, blas_depth = 0;
+    std::vector<uint32_t> solid_blas_root;
+    std::vector<uint32_t> h_inst_blas_root;
+    DBuf<float4> node, itra;
+    DBuf<InstRec> inst;
+    DBuf<PrimRec> prim;
+    DBuf<BvhNode> tlas, blas;
+    DBuf<uint32_t> tlas_items, blas_items, inst_blas_root;
+    DBuf<float> o2w, inst_box;
+
+    /* physics */
+    DBuf<float4> bnd;
+    DBuf<int32_t> optical;
+    DBuf<float> icdf;
+    int32_t n_wl = 0, n_icdf = 0, n_icdf_layer = 0, icdf_hd = 0;
+    float wl_low = 0.f, wl_step = 1.f;
+    DBuf<float> precooked;
+
+    /* work */
+    DBuf<float> gs;
+    DBuf<int64_t> prefix, block_total, evbuf;
+    DBuf<int32_t> word_prefix, ev_id;
+    DBuf<unsigned long long> words, next;
+    /* outputs of the host-array entry point */
+    DBuf<float4> photon, hit;
+    DBuf<uint64_t> seq;
+    std::vector<float> h_photon, h_hit;
+    std::vector<uint64_t> h_seq;
+};
+
+extern "C" {
+
+void mi_config_default(mi_config* c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->max_bounce = 31;
+    c->max_time = 1e27f;
+    c->tmin = 0.05f;
+    c->tmin0 = 0.05f;
+    c->eps0_mask = 0x4u | 0x1u | 0x2u | 0x20u | 0x10u;   /* TO,CK,SI,SC,RE */
+    c->tmax = 1e6f;
+    c->hit_mask = 0x40u;                                 /* SD */
+    c->seed = 0;
+    c->rng_offset = 0;
+    c->event_skipahead = 100000;
+    c->rng_kind = MI_RNG_PHILOX;
+    c->output_mask = MI_OUT_PHOTON | MI_OUT_SEQ | MI_OUT_HIT;
+    c->max_slot = 0;
+    c->kernel = MI_KERNEL_WAVEFRONT;
+    c->block_size = 128;
+    c->refill_period = 4;
+    c->bvh_leaf_size = 2;
+    c->waves_per_cu = 0;
+}
+
+const char* mi_last_error(void) { return g_err.c_str(); }
+const char* mi_version(void) { return "opticks_amd 0.1 gfx950"; }
+
+static int check_config(const mi_config* c)
+{
+    if (c->block_size != 64 && c->block_size != 128 && c->block_size != 256)
+        return fail(MI_ERR_ARG, "block_size must be 64, 128 or 256");
+    if (c->bvh_leaf_size < 1 || c->bvh_leaf_size > 64) return fail(MI_ERR_ARG, "bvh_leaf_size must be 1..64");
+    if (c->max_bounce < 0) return fail(MI_ERR_ARG, "max_bounce < 0");
+    if (c->kernel != MI_KERNEL_PLAIN && c->kernel != MI_KERNEL_WAVEFRONT) return fail(MI_ERR_ARG, "unknown kernel");
+    if (c->rng_kind == MI_RNG_PRECOOKED && (!c->precooked || c->precooked_ni <= 0 || c->precooked_nk <= 0))
+        return fail(MI_ERR_ARG, "precooked rng needs a (ni,nk) array");
+    return MI_OK;
+}
+
+static int validate_geometry(const mi_geometry* g, const mi_physics* ph)
+{
+    if (!g || !ph) return fail(MI_ERR_ARG, "null geometry or physics");
+    if (g->n_inst <= 0 || g->n_solid <= 0 || g->n_prim <= 0 || g->n_node <= 0)
+        return fail(MI_ERR_GEOMETRY, "empty geometry");
+    if (g->n_inst > 0xffff) return fail(MI_ERR_GEOMETRY, "more than 65535 instances (iindex is 16 bits)");
+    const unsigned char* sb = (const unsigned char*)g->solid;
+    for (int64_t s = 0; s < g->n_solid; ++s) {
+        int32_t np, po;
+        std::memcpy(&np, sb + 48 * s + 16, 4);
+        std::memcpy(&po, sb + 48 * s + 20, 4);
+        if (np < 1 || po < 0 || po + np > g->n_prim) return fail(MI_ERR_GEOMETRY, "solid prim range out of bounds");
+    }
+    for (int64_t i = 0; i < g->n_inst; ++i) {
+        int32_t gas;
+        std::memcpy(&gas, g->inst + 16 * i + 7, 4);
+        if (gas < 0 || gas >= g->n_solid) return fail(MI_ERR_GEOMETRY, "instance gas index out of range");
+    }
+    int32_t max_b = -1;
+    for (int64_t p = 0; p < g->n_prim; ++p) {
+        const int32_t* pr = (const int32_t*)(g->prim + 16 * p);
+        int32_t num = pr[0], off = pr[1];
+        if (num < 1 || off < 0 || off + num > g->n_node) return fail(MI_ERR_GEOMETRY, "prim node range out of bounds");
+        const uint32_t* root = (const uint32_t*)(g->node + 16 * off);
+        uint32_t tc = root[14];
+        if (tc >= 101) {
+            if (num != 1) return fail(MI_ERR_GEOMETRY, "leaf prim with numNode != 1");
+        } else if (tc >= 1 && tc <= 3) {
+            int h = tree_height(root[0]);
+            if (h < 0 || (int)root[0] != num) return fail(MI_ERR_GEOMETRY, "tree subNum is not a complete tree size");
+            if (h > OA_CSG_MAX_HEIGHT) return fail(MI_ERR_UNSUPPORTED, "CSG tree deeper than the device stacks");
+        } else {
+            return fail(MI_ERR_UNSUPPORTED, "list-node prims (CSG_CONTIGUOUS/DISCONTIGUOUS/OVERLAP) not supported");
+        }
+        for (int k = 0; k < num; ++k) {
+            const uint32_t* nd = (const uint32_t*)(g->node + 16 * (off + k));
+            uint32_t t = nd[14];
+            bool ok = t == 0 || (t >= 1 && t <= 3) || t == 101 || t == 103 || t == 105 || t == 110;
+            if (!ok) return fail(MI_ERR_UNSUPPORTED, "unsupported CSG node typecode " + std::to_string(t));
+            uint32_t gt = nd[15] & 0x7fffffffu;
+            if (gt > (uint32_t)g->n_tran) return fail(MI_ERR_GEOMETRY, "gtransformIdx beyond itra");
+            if (gt && !g->itra) return fail(MI_ERR_GEOMETRY, "node transform without itra array");
+        }
+        max_b = std::max<int32_t>(max_b, (int32_t)root[6]);
+    }
+    if (max_b >= ph->n_bnd) return fail(MI_ERR_GEOMETRY, "prim boundary beyond the bnd table");
+    if (!ph->bnd || !ph->optical || ph->n_wl < 2) return fail(MI_ERR_ARG, "physics tables missing");
+    return MI_OK;
+}
+
+static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
+{
+    hipStream_t s = sim->stream;
+    const int64_t ni = g->n_inst;
+    /* BLAS per solid over its prims (solid frame boxes from CSGPrim q2/q3) */
+    std::vector<BvhNode> blas;
+    std::vector<uint32_t> blas_items;
+    const unsigned char* sb = (const unsigned char*)g->solid;
+    sim->solid_blas_root.resize(g->n_solid);
+    uint32_t bdepth = 0;
+    std::vector<std::array<double, 6>> solid_box(g->n_solid);
+    for (int64_t so = 0; so < g->n_solid; ++so) {
+        int32_t np, po;
+        std::memcpy(&np, sb + 48 * so + 16, 4);
+        std::memcpy(&po, sb + 48 * so + 20, 4);
+        std::vector<oa_bvh::Item> items(np);
+        std::array<double, 6> bx = { 1e300, 1e300, 1e300, -1e300, -1e300, -1e300 };
+        for (int p = 0; p < np; ++p) {
+            const float* pr = g->prim + 16 * (po + p);
+            oa_bvh::Item& it = items[p];
+            double lo[3] = { pr[8], pr[9], pr[10] }, hi[3] = { pr[11], pr[12], pr[13] };
+            for (int k = 0; k < 3; ++k) {
+                double pad = 1e-3 * (hi[k] - lo[k]) + 1e-2;
+                it.lo[k] = lo[k] - pad; it.hi[k] = hi[k] + pad;
+                bx[k] = std::min(bx[k], it.lo[k]); bx[3 + k] = std::max(bx[3 + k], it.hi[k]);
+            }
+            it.id = (uint32_t)(po + p);
+        }
+        oa_bvh::Builder b(blas, blas_items, 1);
+        sim->solid_blas_root[so] = b.build(items);
+        bdepth = std::max(bdepth, b.max_depth);
+        solid_box[so] = bx;
+    }
+    /* instances: world->object transforms and padded world boxes */
+    std::vector<InstRec> inst(ni);
+    std::vector<oa_bvh::Item> titems(ni);
+    sim->h_inst_blas_root.resize(ni);
+    for (int64_t i = 0; i < ni; ++i) {
+        const float* m = g->inst + 16 * i;
+        int32_t gas, ident;
+        std::memcpy(&gas, m + 7, 4);
+        std::memcpy(&ident, m + 11, 4);
+        InstRec& r = inst[i];
+        invert_affine16(m, r.m);
+        r.m[3] = f32_of((uint32_t)gas);
+        r.m[7] = f32_of((uint32_t)ident);
+        r.m[11] = f32_of(sim->solid_blas_root[gas]);
+        sim->h_inst_blas_root[i] = sim->solid_blas_root[gas];
+        const std::array<double, 6>& bx = solid_box[gas];
+        oa_bvh::Item& it = titems[i];
+        for (int k = 0; k < 3; ++k) { it.lo[k] = 1e300; it.hi[k] = -1e300; }
+        for (int c = 0; c < 8; ++c) {
+            double v[3] = { (c & 1) ? bx[3] : bx[0], (c & 2) ? bx[4] : bx[1], (c & 4) ? bx[5] : bx[2] };
+            for (int k = 0; k < 3; ++k) {
+                double p = v[0] * m[k] + v[1] * m[4 + k] + v[2] * m[8 + k] + m[12 + k];
+                it.lo[k] = std::min(it.lo[k], p); it.hi[k] = std::max(it.hi[k], p);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            double pad = 1e-3 * (it.hi[k] - it.lo[k]) + 1e-2;
+            it.lo[k] -= pad; it.hi[k] += pad;
+        }
+        it.id = (uint32_t)i;
+    }
+    std::vector<BvhNode> tlas;
+    std::vector<uint32_t> tlas_items;
+    oa_bvh::Builder tb(tlas, tlas_items, sim->cfg.bvh_leaf_size);
+    uint32_t troot = tb.build(titems);
+    if (troot != 0) return fail(MI_ERR_GEOMETRY, "TLAS root is not node 0");
+    if (tb.max_depth + bdepth + 2 > OA_STACK_DEPTH)
+        return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
+    sim->tlas_depth = tb.max_depth;
+    sim->blas_depth = bdepth;
+    sim->n_tlas = (int64_t)tlas.size();
+    sim->n_inst = ni;
+    sim->n_prim = g->n_prim;
+
+    std::vector<PrimRec> prim(g->n_prim);
+    for (int64_t p = 0; p < g->n_prim; ++p) {
+        const int32_t* pr = (const int32_t*)(g->prim + 16 * p);
+        const uint32_t* root = (const uint32_t*)(g->node + 16 * pr[1]);
+        prim[p].node_offset = (uint32_t)pr[1];
+        prim[p].boundary = root[6];
+        prim[p].global_prim = (uint32_t)pr[15];
+        prim[p].pad = 0;
+    }
+    HIPCHECK(upload(sim->node, (const float4*)g->node, (size_t)g->n_node * 4, s));
+    if (g->n_tran > 0) HIPCHECK(upload(sim->itra, (const float4*)g->itra, (size_t)g->n_tran * 4, s));
+    else HIPCHECK(sim->itra.reserve(4));
+    HIPCHECK(upload(sim->inst, inst.data(), inst.size(), s));
+    HIPCHECK(upload(sim->prim, prim.data(), prim.size(), s));
+    HIPCHECK(upload(sim->tlas, tlas.data(), tlas.size(), s));
+    HIPCHECK(upload(sim->tlas_items, tlas_items.data(), tlas_items.size(), s));
+    HIPCHECK(upload(sim->blas, blas.data(), blas.size(), s));
+    HIPCHECK(upload(sim->blas_items, blas_items.data(), blas_items.size(), s));
+    HIPCHECK(upload(sim->o2w, g->inst, (size_t)ni * 16, s));
+    HIPCHECK(upload(sim->inst_blas_root, sim->h_inst_blas_root.data(), sim->h_inst_blas_root.size(), s));
+    HIPCHECK(sim->inst_box.reserve((size_t)ni * 6));
+    return MI_OK;
+}
+
+static int upload_physics(mi_sim* sim, const mi_physics* ph)
+{
+    hipStream_t s = sim->stream;
+    const size_t nb = (size_t)ph->n_bnd * 4 * 2 * ph->n_wl;
+    HIPCHECK(upload(sim->bnd, (const float4*)ph->bnd, nb, s));
+    HIPCHECK(upload(sim->optical, ph->optical, (size_t)ph->n_bnd * 16, s));
+    sim->n_wl = ph->n_wl;
+    sim->wl_low = ph->wl_low;
+    sim->wl_step = ph->wl_step;
+    if (ph->icdf && ph->n_icdf > 0 && ph->n_icdf_layer > 0) {
+        HIPCHECK(upload(sim->icdf, ph->icdf, (size_t)ph->n_icdf * ph->n_icdf_layer, s));
+        sim->n_icdf = ph->n_icdf;
+        sim->n_icdf_layer = ph->n_icdf_layer;
+        sim->icdf_hd = ph->icdf_hd_factor;
+    } else {
+        float one = 0.f;
+        HIPCHECK(upload(sim->icdf, &one, 1, s));
+        sim->n_icdf = 1;
+        sim->n_icdf_layer = 1;
+        sim->icdf_hd = 0;
+    }
+    return MI_OK;
+}
+
+static int apply_config(mi_sim* sim, const mi_config* cfg)
+{
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    sim->cfg = *cfg;
+    if (cfg->rng_kind == MI_RNG_PRECOOKED) {
+        HIPCHECK(upload(sim->precooked, cfg->precooked, (size_t)(cfg->precooked_ni * cfg->precooked_nk), sim->stream));
+    }
+    sim->cfg.precooked = nullptr;   /* the handle keeps its own device copy */
+    return MI_OK;
+}
+
+int mi_sim_create(const mi_geometry* geo, const mi_physics* phys, const mi_config* cfg, mi_sim** out)
+{
+    if (!out) return fail(MI_ERR_ARG, "null out");
+    *out = nullptr;
+    mi_config c;
+    if (cfg) c = *cfg; else mi_config_default(&c);
+    int rc = check_config(&c);
+    if (rc) return rc;
+    rc = validate_geometry(geo, phys);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MI_ERR_HIP, "no HIP device");
+    mi_sim* sim = new mi_sim();
+    sim->cfg = c;
+    HIPCHECK(hipGetDevice(&sim->device));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, sim->device) == hipSuccess) sim->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreate(&sim->stream) != hipSuccess) { delete sim; return fail(MI_ERR_HIP, "hipStreamCreate failed"); }
+    for (auto& e : sim->ev) HIPCHECK(hipEventCreate(&e));
+    rc = build_and_upload_geometry(sim, geo);
+    if (!rc) rc = upload_physics(sim, phys);
+    if (!rc) rc = apply_config(sim, &c);
+    if (!rc) {
+        rc = sim->next.reserve(1) == hipSuccess ? MI_OK : fail(MI_ERR_HIP, "alloc counter");
+    }
+    if (rc) { mi_sim_destroy(sim); return rc; }
+    *out = sim;
+    return MI_OK;
+}
+
+void mi_sim_destroy(mi_sim* sim)
+{
+    if (!sim) return;
+    if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+    sim->node.release(); sim->itra.release(); sim->inst.release(); sim->prim.release();
+    sim->tlas.release(); sim->blas.release(); sim->tlas_items.release(); sim->blas_items.release();
+    sim->inst_blas_root.release(); sim->o2w.release(); sim->inst_box.release();
+    sim->bnd.release(); sim->optical.release(); sim->icdf.release(); sim->precooked.release();
+    sim->gs.release(); sim->prefix.release(); sim->block_total.release(); sim->evbuf.release();
+    sim->word_prefix.release(); sim->ev_id.release(); sim->words.release(); sim->next.release();
+    sim->photon.release(); sim->hit.release(); sim->seq.release();
+    for (auto& e : sim->ev) if (e) (void)hipEventDestroy(e);
+    if (sim->stream) (void)hipStreamDestroy(sim->stream);
+    delete sim;
+}
+
+int mi_sim_set_config(mi_sim* sim, const mi_config* cfg)
+{
+    if (!sim || !cfg) return fail(MI_ERR_ARG, "null");
+    if (cfg->bvh_leaf_size != sim->cfg.bvh_leaf_size)
+        return fail(MI_ERR_ARG, "bvh_leaf_size is fixed at create time");
+    return apply_config(sim, cfg);
+}
+
+int mi_sim_get_config(const mi_sim* sim, mi_config* cfg)
+{
+    if (!sim || !cfg) return fail(MI_ERR_ARG, "null");
+    *cfg = sim->cfg;
+    return MI_OK;
+}
+
+static SimParams base_params(const mi_sim* sim)
+{
+    SimParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.node = sim->node.p; P.itra = sim->itra.p; P.inst = sim->inst.p; P.prim = sim->prim.p;
+    P.tlas = sim->tlas.p; P.tlas_items = sim->tlas_items.p; P.blas = sim->blas.p; P.blas_items = sim->blas_items.p;
+    P.bnd = sim->bnd.p; P.optical = sim->optical.p; P.icdf = sim->icdf.p;
+    P.n_wl = sim->n_wl; P.wl_low = sim->wl_low; P.wl_step = sim->wl_step;
+    P.n_icdf = sim->n_icdf; P.n_icdf_layer = sim->n_icdf_layer; P.icdf_hd = sim->icdf_hd;
+    const mi_config& c = sim->cfg;
+    P.max_bounce = c.max_bounce; P.max_time = c.max_time; P.tmin = c.tmin; P.tmin0 = c.tmin0; P.tmax = c.tmax;
+    P.eps0_mask = c.eps0_mask; P.hit_mask = c.hit_mask;
+    P.seed = c.seed; P.rng_offset = c.rng_offset; P.skipahead = c.event_skipahead;
+    P.rng_kind = c.rng_kind;
+    P.precooked = c.rng_kind == MI_RNG_PRECOOKED ? sim->precooked.p : nullptr;
+    P.pre_ni = c.precooked_ni; P.pre_nk = c.precooked_nk;
+    P.next = sim->next.p;
+    P.refill_period = c.refill_period;
+    return P;
+}
+
+static int wave_grid(const mi_sim* sim)
+{
+    int wpc = sim->cfg.waves_per_cu > 0 ? sim->cfg.waves_per_cu : 8;
+    int waves_per_block = sim->cfg.block_size / 64;
+    return std::max(1, sim->n_cu * wpc / waves_per_block);
+}
+
+/*
+ * One propagation launch over gensteps already on the device with a
+ * launch-local prefix; events described by host arrays (copied to evbuf).
+ * Writes photons [0,n_photon) of the launch and the hit bitmask words.
+ */
+static int launch(mi_sim* sim, SimParams& P, hipStream_t s)
+{
+    const int64_t n_words = (P.n_photon + 63) / 64;
+    HIPCHECK(sim->words.reserve((size_t)n_words + 1));
+    P.hitwords = sim->words.p;
+    if (sim->cfg.kernel == MI_KERNEL_WAVEFRONT) {
+        HIPCHECK(oa_launch::zero_u64(sim->words.p, n_words, s));
+        HIPCHECK(oa_launch::zero_u64(sim->next.p, 1, s));
+    }
+    HIPCHECK(oa_launch::simulate(P, sim->cfg.kernel, sim->cfg.block_size, wave_grid(sim), s));
+    return MI_OK;
+}
+
+/* stable hit compaction; the hit count lands in block_total[nb], read after the end event */
+static int compact(mi_sim* sim, int64_t n_photon, const float4* d_photon, float4* d_hit, int64_t hit_cap,
+                   int64_t** d_count, hipStream_t s)
+{
+    const int64_t n_words = (n_photon + 63) / 64;
+    const int64_t nb = (n_words + 255) / 256;
+    HIPCHECK(sim->word_prefix.reserve((size_t)n_words + 1));
+    HIPCHECK(sim->block_total.reserve((size_t)nb + 1));
+    HIPCHECK(oa_launch::compact_hits(sim->words.p, n_words, sim->word_prefix.p, sim->block_total.p,
+                                     d_photon, d_hit, hit_cap, s));
+    *d_count = sim->block_total.p + nb;
+    return MI_OK;
+}
+
+int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
+                           int64_t photon_offset, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                           int64_t* n_hit_out, void* stream)
+{
+    if (!sim) return fail(MI_ERR_ARG, "null sim");
+    if (n_hit_out) *n_hit_out = 0;
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    if (!d_gensteps) return fail(MI_ERR_ARG, "null gensteps");
+    if (d_hit && !d_photon) return fail(MI_ERR_ARG, "hits are gathered from the photon array: pass d_photon");
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    HIPCHECK(hipEventRecord(sim->ev[0], s));
+    HIPCHECK(sim->prefix.reserve((size_t)n_gs + 1));
+    HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, sim->prefix.p, s));
+    std::vector<int64_t> hprefix;
+    const int64_t max_slot = sim->cfg.max_slot;
+    if (n_photon < 0 || max_slot > 0) {
+        hprefix.resize((size_t)n_gs + 1);
+        HIPCHECK(hipMemcpyAsync(hprefix.data(), sim->prefix.p, sizeof(int64_t) * (n_gs + 1), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (n_photon < 0) n_photon = hprefix[n_gs];
+        else if (n_photon != hprefix[n_gs]) return fail(MI_ERR_ARG, "n_photon does not match the gensteps");
+    }
+    HIPCHECK(hipEventRecord(sim->ev[1], s));
+    if (n_photon == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    /* slices of whole gensteps with at most max_slot photons (sysrap/SGenstep.h:248) */
+    std::vector<std::pair<int64_t, int64_t>> slices;
+    if (max_slot > 0 && n_photon > max_slot) {
+        int64_t g0 = 0;
+        while (g0 < n_gs) {
+            int64_t g1 = g0 + 1;
+            while (g1 < n_gs && hprefix[g1 + 1] - hprefix[g0] <= max_slot) ++g1;
+            if (hprefix[g1] - hprefix[g0] > max_slot)
+                return fail(MI_ERR_CAPACITY, "one genstep has more photons than max_slot");
+            slices.emplace_back(g0, g1);
+            g0 = g1;
+        }
+    } else {
+        slices.emplace_back(0, n_gs);
+    }
+    int64_t hits = 0;
+    double prop_ms = 0.0, comp_ms = 0.0;
+    for (auto& sl : slices) {
+        const int64_t g0 = sl.first, g1 = sl.second;
+        const int64_t p0 = hprefix.empty() ? 0 : hprefix[g0];
+        const int64_t np = hprefix.empty() ? n_photon : hprefix[g1] - hprefix[g0];
+        if (slices.size() > 1) {
+            HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps + 24 * g0, g1 - g0, sim->prefix.p, s));
+        }
+        int64_t evh[4] = { 0, g1 - g0, 0, np };
+        int32_t evid = event_id;
+        HIPCHECK(sim->evbuf.reserve(4));
+        HIPCHECK(sim->ev_id.reserve(1));
+        HIPCHECK(hipMemcpyAsync(sim->evbuf.p, evh, sizeof(evh), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(sim->ev_id.p, &evid, sizeof(evid), hipMemcpyHostToDevice, s));
+        SimParams P = base_params(sim);
+        P.gensteps = (const float*)d_gensteps + 24 * g0;
+        P.gs_prefix = sim->prefix.p;
+        P.n_gs = g1 - g0;
+        P.n_event = 1;
+        P.ev_gs_begin = sim->evbuf.p;
+        P.ev_ph_begin = sim->evbuf.p + 2;
+        P.ev_id = sim->ev_id.p;
+        P.n_photon = np;
+        P.photon_offset = photon_offset + p0;
+        P.photon = d_photon ? (float4*)d_photon + 4 * p0 : nullptr;
+        P.seq = d_seq ? (uint64_t*)d_seq + 4 * p0 : nullptr;
+        int rc = launch(sim, P, s);
+        if (rc) return rc;
+        HIPCHECK(hipEventRecord(sim->ev[2], s));
+        int64_t nh = 0;
+        int64_t* d_count = nullptr;
+        rc = compact(sim, np, P.photon, d_hit ? (float4*)d_hit + 4 * hits : nullptr,
+                     d_hit ? std::max<int64_t>(0, hit_capacity - hits) : 0, &d_count, s);
+        if (rc) return rc;
+        HIPCHECK(hipEventRecord(sim->ev[3], s));
+        HIPCHECK(hipMemcpyAsync(&nh, d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        float a = 0.f, b = 0.f;
+        HIPCHECK(hipEventElapsedTime(&a, sim->ev[1], sim->ev[2]));
+        HIPCHECK(hipEventElapsedTime(&b, sim->ev[2], sim->ev[3]));
+        prop_ms += a; comp_ms += b;
+        HIPCHECK(hipEventRecord(sim->ev[1], s));
+        hits += nh;
+    }
+    float seed_ms = 0.f;
+    (void)seed_ms;
+    sim->last_ms[1] = prop_ms;
+    sim->last_ms[2] = comp_ms;
+    float tot = 0.f;
+    HIPCHECK(hipEventElapsedTime(&tot, sim->ev[0], sim->ev[3]));
+    sim->last_ms[3] = tot;
+    sim->last_ms[0] = std::max(0.0, (double)tot - prop_ms - comp_ms);
+    if (n_hit_out) *n_hit_out = hits;
+    if (d_hit && hits > hit_capacity) return fail(MI_ERR_CAPACITY, "hit_capacity too small: " + std::to_string(hits) + " hits");
+    return MI_OK;
+}
+
+int mi_sim_simulate_batch_device(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
+                                 const void* d_gensteps, int64_t* photon_offsets, void* d_photon, void* d_seq,
+                                 void* d_hit, int64_t hit_capacity, int64_t* hit_offsets, void* stream)
+{
+    if (!sim || n_event <= 0 || !event_ids || !gs_offsets || !photon_offsets || !hit_offsets)
+        return fail(MI_ERR_ARG, "bad batch arguments");
+    if (d_hit && !d_photon) return fail(MI_ERR_ARG, "hits are gathered from the photon array: pass d_photon");
+    const int64_t n_gs = gs_offsets[n_event];
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    HIPCHECK(hipEventRecord(sim->ev[0], s));
+    HIPCHECK(sim->prefix.reserve((size_t)n_gs + 1));
+    HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, sim->prefix.p, s));
+    std::vector<int64_t> hp((size_t)n_gs + 1);
+    HIPCHECK(hipMemcpyAsync(hp.data(), sim->prefix.p, sizeof(int64_t) * (n_gs + 1), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    std::vector<int64_t> evh(2 * ((size_t)n_event + 1));
+    for (int e = 0; e <= n_event; ++e) {
+        evh[e] = gs_offsets[e];
+        evh[n_event + 1 + e] = hp[gs_offsets[e]];
+        photon_offsets[e] = hp[gs_offsets[e]];
+    }
+    const int64_t n_photon = hp[n_gs];
+    if (n_photon == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    HIPCHECK(sim->evbuf.reserve(evh.size() + n_event + 1));
+    HIPCHECK(sim->ev_id.reserve((size_t)n_event));
+    HIPCHECK(hipMemcpyAsync(sim->evbuf.p, evh.data(), sizeof(int64_t) * evh.size(), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(sim->ev_id.p, event_ids, sizeof(int32_t) * n_event, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipEventRecord(sim->ev[1], s));
+    SimParams P = base_params(sim);
+    P.gensteps = (const float*)d_gensteps;
+    P.gs_prefix = sim->prefix.p;
+    P.n_gs = n_gs;
+    P.n_event = n_event;
+    P.ev_gs_begin = sim->evbuf.p;
+    P.ev_ph_begin = sim->evbuf.p + (n_event + 1);
+    P.ev_id = sim->ev_id.p;
+    P.n_photon = n_photon;
+    P.photon_offset = 0;
+    P.photon = (float4*)d_photon;
+    P.seq = (uint64_t*)d_seq;
+    int rc = launch(sim, P, s);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(sim->ev[2], s));
+    const int64_t n_words = (n_photon + 63) / 64;
+    const int64_t nb = (n_words + 255) / 256;
+    HIPCHECK(sim->word_prefix.reserve((size_t)n_words + 1));
+    HIPCHECK(sim->block_total.reserve((size_t)nb + 1));
+    HIPCHECK(oa_launch::compact_hits(sim->words.p, n_words, sim->word_prefix.p, sim->block_total.p,
+                                     (const float4*)d_photon, (float4*)d_hit, d_hit ? hit_capacity : 0, s));
+    DBuf<int64_t>& hob = sim->evbuf;   /* reuse tail of evbuf for the offsets */
+    int64_t* d_ho = hob.p + evh.size();
+    HIPCHECK(oa_launch::event_hit_offsets(sim->words.p, sim->word_prefix.p, sim->block_total.p,
+                                          P.ev_ph_begin, n_event, n_words, d_ho, s));
+    HIPCHECK(hipMemcpyAsync(hit_offsets, d_ho, sizeof(int64_t) * (n_event + 1), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipEventRecord(sim->ev[3], s));
+    HIPCHECK(hipEventSynchronize(sim->ev[3]));
+    float a = 0.f, b = 0.f, c = 0.f, t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&a, sim->ev[0], sim->ev[1]));
+    HIPCHECK(hipEventElapsedTime(&b, sim->ev[1], sim->ev[2]));
+    HIPCHECK(hipEventElapsedTime(&c, sim->ev[2], sim->ev[3]));
+    HIPCHECK(hipEventElapsedTime(&t, sim->ev[0], sim->ev[3]));
+    sim->last_ms[0] = a; sim->last_ms[1] = b; sim->last_ms[2] = c; sim->last_ms[3] = t;
+    if (d_hit && hit_offsets[n_event] > hit_capacity)
+        return fail(MI_ERR_CAPACITY, "hit_capacity too small: " + std::to_string(hit_offsets[n_event]) + " hits");
+    return MI_OK;
+}
+
+int mi_sim_simulate(mi_sim* sim, int32_t event_id, const float* gensteps, int64_t n_gs, mi_event_out* out)
+{
+    if (!sim || !out) return fail(MI_ERR_ARG, "null");
+    std::memset(out, 0, sizeof(*out));
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    if (!gensteps) return fail(MI_ERR_ARG, "null gensteps");
+    int64_t n_photon = 0;
+    for (int64_t g = 0; g < n_gs; ++g) {
+        uint32_t np;
+        std::memcpy(&np, gensteps + 24 * g + 3, 4);
+        n_photon += np;
+    }
+    if (n_photon == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    hipStream_t s = sim->stream;
+    HIPCHECK(upload(sim->gs, gensteps, (size_t)n_gs * 24, s));
+    const uint32_t om = sim->cfg.output_mask;
+    HIPCHECK(sim->photon.reserve((size_t)n_photon * 4));
+    if (om & MI_OUT_SEQ) HIPCHECK(sim->seq.reserve((size_t)n_photon * 4));
+    if (om & MI_OUT_HIT) HIPCHECK(sim->hit.reserve((size_t)n_photon * 4));
+    int64_t n_hit = 0;
+    int rc = mi_sim_simulate_device(sim, event_id, sim->gs.p, n_gs, n_photon, 0, sim->photon.p,
+                                    (om & MI_OUT_SEQ) ? sim->seq.p : nullptr,
+                                    (om & MI_OUT_HIT) ? sim->hit.p : nullptr, n_photon, &n_hit, s);
+    if (rc) return rc;
+    out->n_photon = n_photon;
+    out->n_hit = n_hit;
+    out->device_ms = sim->last_ms[3];
+    if (om & MI_OUT_PHOTON) {
+        sim->h_photon.resize((size_t)n_photon * 16);
+        HIPCHECK(hipMemcpy(sim->h_photon.data(), sim->photon.p, sizeof(float) * 16 * n_photon, hipMemcpyDeviceToHost));
+        out->photon = sim->h_photon.data();
+    }
+    if (om & MI_OUT_SEQ) {
+        sim->h_seq.resize((size_t)n_photon * 4);
+        HIPCHECK(hipMemcpy(sim->h_seq.data(), sim->seq.p, sizeof(uint64_t) * 4 * n_photon, hipMemcpyDeviceToHost));
+        out->seq = sim->h_seq.data();
+    }
+    if (om & MI_OUT_HIT) {
+        sim->h_hit.resize((size_t)std::max<int64_t>(n_hit, 1) * 16);
+        if (n_hit) HIPCHECK(hipMemcpy(sim->h_hit.data(), sim->hit.p, sizeof(float) * 16 * n_hit, hipMemcpyDeviceToHost));
+        out->hit = sim->h_hit.data();
+    }
+    return MI_OK;
+}
+
+int mi_sim_reset(mi_sim* sim, int32_t event_id)
+{
+    (void)event_id;
+    if (!sim) return fail(MI_ERR_ARG, "null sim");
+    std::vector<float>().swap(sim->h_photon);
+    std::vector<float>().swap(sim->h_hit);
+    std::vector<uint64_t>().swap(sim->h_seq);
+    return MI_OK;
+}
+
+int mi_sim_trace_device(mi_sim* sim, const void* d_rays, int64_t n, void* d_isect, void* stream)
+{
+    if (!sim || !d_rays || !d_isect || n < 0) return fail(MI_ERR_ARG, "bad trace arguments");
+    if (n == 0) return MI_OK;
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    SimParams P = base_params(sim);
+    HIPCHECK(oa_launch::trace(P, (const float4*)d_rays, n, (float4*)d_isect, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+int mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst)
+{
+    if (!sim || !inst) return fail(MI_ERR_ARG, "null");
+    if (n_inst != sim->n_inst) return fail(MI_ERR_ARG, "refit keeps the instance count");
+    hipStream_t s = sim->stream;
+    std::vector<InstRec> rec(n_inst);
+    std::vector<InstRec> old(n_inst);
+    HIPCHECK(hipMemcpy(old.data(), sim->inst.p, sizeof(InstRec) * n_inst, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n_inst; ++i) {
+        const float* m = inst + 16 * i;
+        int32_t gas;
+        std::memcpy(&gas, m + 7, 4);
+        if (gas < 0 || (size_t)gas >= sim->solid_blas_root.size()) return fail(MI_ERR_GEOMETRY, "gas index");
+        if (sim->solid_blas_root[gas] != sim->h_inst_blas_root[i]) return fail(MI_ERR_ARG, "refit keeps each instance's solid");
+        invert_affine16(m, rec[i].m);
+        rec[i].m[3] = old[i].m[3];
+        int32_t ident;
+        std::memcpy(&ident, m + 11, 4);
+        rec[i].m[7] = f32_of((uint32_t)ident);
+        rec[i].m[11] = old[i].m[11];
+    }
+    HIPCHECK(upload(sim->inst, rec.data(), rec.size(), s));
+    HIPCHECK(upload(sim->o2w, inst, (size_t)n_inst * 16, s));
+    HIPCHECK(oa_launch::refit(sim->o2w.p, sim->blas.p, sim->inst_blas_root.p, n_inst, sim->inst_box.p,
+                              sim->tlas.p, sim->n_tlas, sim->tlas_depth, sim->tlas_items.p, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+int mi_sim_last_timing(const mi_sim* sim, double* ms4)
+{
+    if (!sim || !ms4) return fail(MI_ERR_ARG, "null");
+    for (int i = 0; i < 4; ++i) ms4[i] = sim->last_ms[i];
+    return MI_OK;
+}
+
+} // extern "C"

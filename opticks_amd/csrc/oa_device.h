@@ -1,0 +1,436 @@
+/*
+ * oa_device.h : device-side optical-photon propagation for gfx950 (CDNA4).
+ *
+ * One lane carries one photon through generation and the bounce loop of
+ * CSGOptiX/CSGOptiX7.cu:432-530. The OptiX trace is replaced by a two-level
+ * BVH (TLAS over instances, BLAS over the CSGPrim of each solid) traversed
+ * with a per-lane short stack in LDS; the intersection program is the CSG
+ * single-hit algorithm of CSG/csg_intersect_tree.h; the physics follows
+ * qudarap/qsim.h with boundary tables read as FP32-lerped rows in HBM
+ * instead of hardware textures.
+ *
+ * Floating point: compiled with -ffp-contract=off and written in the
+ * reference's operand order, so that results are bit-identical to the CPU
+ * checker on the same random streams. sqrtf and '/' are IEEE correctly
+ * rounded under hipcc's defaults; log/sin/cos come from oa_portable.h.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "oa_portable.h"
+#include "oa_layout.h"
+
+#define OA_D __device__ static inline __attribute__((always_inline))
+
+namespace oa {
+
+/* ----------------------------------------------------------- vectors */
+
+struct v3 { float x, y, z; };
+
+OA_D v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+OA_D v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+OA_D v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+OA_D v3 operator-(v3 a) { return mk(-a.x, -a.y, -a.z); }
+OA_D v3 operator*(float s, v3 a) { return mk(a.x * s, a.y * s, a.z * s); }
+OA_D v3 operator*(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+OA_D float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+OA_D v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+OA_D float length(v3 v) { return sqrtf(dot(v, v)); }
+OA_D v3 normalize(v3 v) { float inv = 1.0f / sqrtf(dot(v, v)); return v * inv; }
+OA_D uint32_t fbits(float f) { return __float_as_uint(f); }
+OA_D bool sgnbit(float f) { return (__float_as_uint(f) >> 31) != 0u; }
+
+#define OA_PI_F 3.14159274101257324f
+
+/* --------------------------------------------------------- constants */
+
+enum : uint32_t {
+    F_CERENKOV = 0x1u << 0, F_SCINTILLATION = 0x1u << 1, F_TORCH = 0x1u << 2,
+    F_BULK_ABSORB = 0x1u << 3, F_BULK_REEMIT = 0x1u << 4, F_BULK_SCATTER = 0x1u << 5,
+    F_SURFACE_DETECT = 0x1u << 6, F_SURFACE_ABSORB = 0x1u << 7,
+    F_SURFACE_DREFLECT = 0x1u << 8, F_SURFACE_SREFLECT = 0x1u << 9,
+    F_BOUNDARY_REFLECT = 0x1u << 10, F_BOUNDARY_TRANSMIT = 0x1u << 11
+};
+enum : int { CMD_BREAK = 1, CMD_CONTINUE = 2, CMD_BOUNDARY = 3 };
+enum : int { EMS_NOSURFACE = 1, EMS_SURFACE = 2, EMS_ZPLUS_SENSOR_A = 3 };
+enum : uint32_t { TC_ZERO = 0, TC_UNION = 1, TC_INTERSECTION = 2, TC_DIFFERENCE = 3, TC_NODE = 11,
+                  TC_LEAF = 101, TC_SPHERE = 101, TC_ZSPHERE = 103, TC_CYLINDER = 105, TC_BOX3 = 110 };
+
+/* ------------------------------------------------------------ random */
+
+struct Rng {
+    oa_philox_stream ph;
+    const float* pre;     /* precooked row or nullptr */
+    int64_t pre_nk, cursor;
+    int kind;
+};
+
+OA_D float uni(Rng& r)
+{
+    if (r.kind == 1) {
+        float v = (r.cursor < r.pre_nk) ? r.pre[r.cursor] : 1.0f;
+        r.cursor++;
+        return v;
+    }
+    return oa_rng_uniform(&r.ph);
+}
+
+/* ------------------------------------------------------------ photon */
+
+struct Photon {
+    v3 pos; float time;
+    v3 mom; uint32_t hitcount_iindex;
+    v3 pol; float wavelength;
+    uint32_t obf, identity, index, flagmask;   /* orient_boundary_flag ... (sysrap/sphoton.h:171-246) */
+};
+
+OA_D uint32_t ph_flag(const Photon& p) { return p.obf & 0xffffu; }
+OA_D uint32_t ph_boundary(const Photon& p) { return (p.obf & 0x7fff0000u) >> 16; }
+OA_D void set_flag(Photon& p, uint32_t flag)
+{
+    p.obf = (p.obf & 0xffff0000u) | (flag & 0xffffu);
+    p.flagmask |= flag;
+}
+
+/* per-trace record, the quad2 PRD of sysrap/squad.h:180-250 */
+struct Prd {
+    v3 normal; float t;
+    float lposcost;
+    uint32_t iindex_identity, boundary;
+};
+
+/* ------------------------------------------- robust quadratic roots */
+/* CSG/csg_robust_quadratic_roots.h:104-136 */
+
+OA_D void rq_roots(float& t1, float& t2, float& sdisc, float d, float b, float c)
+{
+    float disc = b * b - d * c;
+    sdisc = disc > 0.f ? sqrtf(disc) : 0.f;
+    float q = b > 0.f ? -(b + sdisc) : -(b - sdisc);
+    float root1 = q / d;
+    float root2 = c / q;
+    t1 = fminf(root1, root2);
+    t2 = fmaxf(root1, root2);
+}
+
+OA_D void rq_roots_disq(float t_min, float& t1, float& t2, float& sdisc, float d, float b, float c)
+{
+    float disc = b * b - d * c;
+    sdisc = disc > 0.f ? sqrtf(disc) : 0.f;
+    float q = b > 0.f ? -(b + sdisc) : -(b - sdisc);
+    float root1 = sdisc > 0.f ? q / d : t_min;
+    float root2 = sdisc > 0.f ? c / q : t_min;
+    t1 = fminf(root1, root2);
+    t2 = fmaxf(root1, root2);
+}
+
+/* -------------------------------------------------------------- leaves */
+
+struct Isect { float x, y, z, w; };
+
+OA_D bool leaf_sphere(Isect& is, float4 q0, float t_min, v3 ro, v3 rd)
+{
+    v3 O = ro - mk(q0.x, q0.y, q0.z);
+    float radius = q0.w;
+    float b = dot(O, rd);
+    float c = dot(O, O) - radius * radius;
+    float d = dot(rd, rd);
+    float r1, r2, sdisc;
+    rq_roots(r1, r2, sdisc, d, b, c);
+    float t_cand = sdisc > 0.f ? (r1 > t_min ? r1 : r2) : t_min;
+    bool valid = t_cand > t_min;
+    if (valid) {
+        is.x = (O.x + t_cand * rd.x) / radius;
+        is.y = (O.y + t_cand * rd.y) / radius;
+        is.z = (O.z + t_cand * rd.z) / radius;
+        is.w = t_cand;
+    }
+    return valid;
+}
+
+OA_D bool leaf_zsphere(Isect& is, float4 q0, float4 q1, float t_min, v3 ro, v3 rd)
+{
+    v3 O = ro - mk(q0.x, q0.y, q0.z);
+    float radius = q0.w;
+    float b = dot(O, rd);
+    float c = dot(O, O) - radius * radius;
+    if (c > 0.f && b > 0.f) return false;
+    float zmax = q0.z + q1.y;
+    float zmin = q0.z + q1.x;
+    float d = dot(rd, rd);
+    float t1sph, t2sph, sdisc;
+    rq_roots(t1sph, t2sph, sdisc, d, b, c);
+    float z1sph = ro.z + t1sph * rd.z;
+    float z2sph = ro.z + t2sph * rd.z;
+    float idz = 1.f / rd.z;
+    float t_QCAP = (zmax - ro.z) * idz;
+    float t_PCAP = (zmin - ro.z) * idz;
+    float t1cap = fminf(t_QCAP, t_PCAP);
+    float t2cap = fmaxf(t_QCAP, t_PCAP);
+    if (t1cap < t1sph || t1cap > t2sph) t1cap = t_min;
+    if (t2cap < t1sph || t2cap > t2sph) t2cap = t_min;
+    float t_cand = t_min;
+    if (sdisc > 0.f) {
+        if (t1sph > t_min && z1sph > zmin && z1sph <= zmax) t_cand = t1sph;
+        else if (t1cap > t_min) t_cand = t1cap;
+        else if (t2cap > t_min) t_cand = t2cap;
+        else if (t2sph > t_min && z2sph > zmin && z2sph <= zmax) t_cand = t2sph;
+    }
+    bool valid = t_cand > t_min;
+    if (valid) {
+        is.w = t_cand;
+        if (t_cand == t1sph || t_cand == t2sph) {
+            is.x = (O.x + t_cand * rd.x) / radius;
+            is.y = (O.y + t_cand * rd.y) / radius;
+            is.z = (O.z + t_cand * rd.z) / radius;
+        } else {
+            is.x = 0.f;
+            is.y = 0.f;
+            is.z = t_cand == t_PCAP ? -1.f : 1.f;
+        }
+    }
+    return valid;
+}
+
+OA_D bool leaf_cylinder(Isect& is, float4 q0, float4 q1, float t_min, v3 ro, v3 rd)
+{
+    const float r = q0.w, z1 = q1.x, z2 = q1.y;
+    const float ox = ro.x, oy = ro.y, oz = ro.z, vx = rd.x, vy = rd.y, vz = rd.z;
+    const float r2 = r * r;
+    const float a = vx * vx + vy * vy;
+    const float b = ox * vx + oy * vy;
+    const float c = ox * ox + oy * oy - r2;
+    float t_near, t_far, sdisc;
+    rq_roots_disq(t_min, t_near, t_far, sdisc, a, b, c);
+    float z_near = oz + t_near * vz;
+    float z_far = oz + t_far * vz;
+    const float t_z1cap = (z1 - oz) / vz;
+    const float r2_z1cap = (ox + t_z1cap * vx) * (ox + t_z1cap * vx) + (oy + t_z1cap * vy) * (oy + t_z1cap * vy);
+    const float t_z2cap = (z2 - oz) / vz;
+    const float r2_z2cap = (ox + t_z2cap * vx) * (ox + t_z2cap * vx) + (oy + t_z2cap * vy) * (oy + t_z2cap * vy);
+    float t_cand = __builtin_inff();
+    if (t_near > t_min && z_near > z1 && z_near < z2 && t_near < t_cand) t_cand = t_near;
+    if (t_far > t_min && z_far > z1 && z_far < z2 && t_far < t_cand) t_cand = t_far;
+    if (t_z1cap > t_min && r2_z1cap <= r2 && t_z1cap < t_cand) t_cand = t_z1cap;
+    if (t_z2cap > t_min && r2_z2cap <= r2 && t_z2cap < t_cand) t_cand = t_z2cap;
+    bool valid = t_cand > t_min && t_cand < __builtin_inff();
+    if (valid) {
+        bool sheet = (t_cand == t_near || t_cand == t_far);
+        is.x = sheet ? (ox + t_cand * vx) / r : 0.f;
+        is.y = sheet ? (oy + t_cand * vy) / r : 0.f;
+        is.z = sheet ? 0.f : (t_cand == t_z1cap ? -1.f : 1.f);
+        is.w = t_cand;
+    }
+    return valid;
+}
+
+OA_D bool leaf_box3(Isect& is, float4 q0, float t_min, v3 ro, v3 rd)
+{
+    const v3 bmin = mk(-q0.x / 2.f, -q0.y / 2.f, -q0.z / 2.f);
+    const v3 bmax = mk(q0.x / 2.f, q0.y / 2.f, q0.z / 2.f);
+    const v3 idir = mk(1.f / rd.x, 1.f / rd.y, 1.f / rd.z);
+    v3 t0 = mk((bmin.x - ro.x) * idir.x, (bmin.y - ro.y) * idir.y, (bmin.z - ro.z) * idir.z);
+    v3 t1 = mk((bmax.x - ro.x) * idir.x, (bmax.y - ro.y) * idir.y, (bmax.z - ro.z) * idir.z);
+    float t_near = fmaxf(fmaxf(fminf(t0.x, t1.x), fminf(t0.y, t1.y)), fminf(t0.z, t1.z));
+    float t_far = fminf(fminf(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y)), fmaxf(t0.z, t1.z));
+    bool along_x = rd.x != 0.f && rd.y == 0.f && rd.z == 0.f;
+    bool along_y = rd.x == 0.f && rd.y != 0.f && rd.z == 0.f;
+    bool along_z = rd.x == 0.f && rd.y == 0.f && rd.z != 0.f;
+    bool in_x = ro.x > bmin.x && ro.x < bmax.x;
+    bool in_y = ro.y > bmin.y && ro.y < bmax.y;
+    bool in_z = ro.z > bmin.z && ro.z < bmax.z;
+    bool has;
+    if (along_x) has = in_y && in_z;
+    else if (along_y) has = in_x && in_z;
+    else if (along_z) has = in_x && in_y;
+    else has = (t_far > t_near && t_far > 0.f);
+    if (!has) return false;
+    float t_cand = t_min < t_near ? t_near : (t_min < t_far ? t_far : t_min);
+    v3 p = (ro + t_cand * rd) - mk(0.f, 0.f, 0.f);
+    v3 pa = mk(fabsf(p.x) / (bmax.x - bmin.x), fabsf(p.y) / (bmax.y - bmin.y), fabsf(p.z) / (bmax.z - bmin.z));
+    v3 n = mk(0.f, 0.f, 0.f);
+    if (pa.x >= pa.y && pa.x >= pa.z) n.x = copysignf(1.f, p.x);
+    else if (pa.y >= pa.x && pa.y >= pa.z) n.y = copysignf(1.f, p.y);
+    else if (pa.z >= pa.x && pa.z >= pa.y) n.z = copysignf(1.f, p.z);
+    if (!(t_cand > t_min)) return false;
+    is.x = n.x; is.y = n.y; is.z = n.z; is.w = t_cand;
+    return true;
+}
+
+/* qat4 right_multiply / left_multiply_inplace (sysrap/sqat4.h:45-104) on a node transform */
+OA_D v3 q_right(const float4* q, v3 v, float w)
+{
+    float4 a = q[0], b = q[1], c = q[2], d = q[3];
+    return mk(a.x * v.x + b.x * v.y + c.x * v.z + d.x * w,
+              a.y * v.x + b.y * v.y + c.y * v.z + d.y * w,
+              a.z * v.x + b.z * v.y + c.z * v.z + d.z * w);
+}
+OA_D void q_left_inplace(const float4* q, Isect& v, float w)
+{
+    float4 a = q[0], b = q[1], c = q[2];
+    float x = a.x * v.x + a.y * v.y + a.z * v.z + a.w * w;
+    float y = b.x * v.x + b.y * v.y + b.z * v.z + b.w * w;
+    float z = c.x * v.x + c.y * v.y + c.z * v.z + c.w * w;
+    v.x = x; v.y = y; v.z = z;
+}
+
+/* intersect_leaf (CSG/csg_intersect_leaf.h:174-324). Returns false for a miss. */
+OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, float t_min, v3 ro, v3 rd)
+{
+    is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
+    const float4 q0 = nd[0];
+    const float4 q1 = nd[1];
+    const uint32_t typecode = __float_as_uint(nd[3].z);
+    const uint32_t gtw = __float_as_uint(nd[3].w);
+    const uint32_t gt = gtw & 0x7fffffffu;
+    const bool complement = (gtw & 0x80000000u) != 0u;
+    const float4* q = gt > 0u ? itra + 4u * (gt - 1u) : nullptr;
+    v3 origin = q ? q_right(q, ro, 1.f) : ro;
+    v3 direction = q ? q_right(q, rd, 0.f) : rd;
+    bool valid = false;
+    switch (typecode) {
+        case TC_SPHERE:   valid = leaf_sphere(is, q0, t_min, origin, direction); break;
+        case TC_ZSPHERE:  valid = leaf_zsphere(is, q0, q1, t_min, origin, direction); break;
+        case TC_CYLINDER: valid = leaf_cylinder(is, q0, q1, t_min, origin, direction); break;
+        case TC_BOX3:     valid = leaf_box3(is, q0, t_min, origin, direction); break;
+        default: break;   /* rejected by the host at create time */
+    }
+    if (valid && q) q_left_inplace(q, is, 0.f);
+    if (complement) {
+        is.x = valid ? -is.x : -0.f;
+        is.y = valid ? -is.y : is.y;
+        is.z = valid ? -is.z : is.z;
+    }
+    return valid;
+}
+
+/* ------------------------------------------------------------- CSG tree */
+
+/* Kensler decision table packed as the reference LUT words (CSG/csg_classify.h):
+   nibble (3*stateA+stateB) of word [op-1], for A closer / B closer. */
+__constant__ static const uint32_t kLutA[3] = { 0x22121141u, 0x00014014u, 0x00141141u };
+__constant__ static const uint32_t kLutB[3] = { 0x22115122u, 0x00022055u, 0x00133155u };
+
+enum : int { CT_MISS = 0, CT_RETURN_A = 1, CT_RETURN_B = 2, CT_RETURN_FLIP_B = 3, CT_LOOP_A = 4, CT_LOOP_B = 5 };
+enum : int { ST_ENTER = 0, ST_EXIT = 1, ST_MISS = 2 };
+
+OA_D int classify(const Isect& e, v3 dir, float tmin)
+{
+    return fabsf(e.w) > tmin ? (e.x * dir.x + e.y * dir.y + e.z * dir.z < 0.f ? ST_ENTER : ST_EXIT) : ST_MISS;
+}
+
+OA_D uint32_t tdepth(uint32_t i) { return 31u - (uint32_t)__builtin_clz(i); }
+OA_D uint32_t postorder_next(uint32_t i, uint32_t elev) { return (i & 1u) ? i >> 1 : (i << elev) + (1u << elev); }
+
+/* Device stacks sized for trees of height <= OA_CSG_MAX_HEIGHT (checked at create). */
+#ifndef OA_CSG_STACK
+#define OA_CSG_STACK 8
+#endif
+#define OA_TRANCHE_STACK 4
+
+/* intersect_tree (CSG/csg_intersect_tree.h:277-672) */
+OA_D bool intersect_tree(Isect& out, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
+{
+    const uint32_t numNode = __float_as_uint(root[0].x);
+    const uint32_t height = (uint32_t)(__builtin_ffs((int)numNode + 1) - 2);
+    const float propagate_epsilon = 0.0001f;
+    int ierr = 0;
+    float trt[OA_TRANCHE_STACK];
+    uint32_t trs[OA_TRANCHE_STACK];
+    int trc = 0;
+    trs[0] = (1u << height) | (0u << 8);    /* slice packed as begin | end<<8 */
+    trt[0] = t_min;
+    Isect st[OA_CSG_STACK];
+    int sc = -1;
+
+    while (trc > -1) {
+        const uint32_t slice = trs[trc];
+        const float tmin = trt[trc];
+        trc--;
+        const uint32_t beginIdx = slice & 0xffu;
+        const uint32_t endIdx = (slice >> 8) & 0xffu;
+        uint32_t nodeIdx = beginIdx;
+        bool restart = false;
+        while (nodeIdx != endIdx) {
+            const uint32_t depth = tdepth(nodeIdx);
+            const uint32_t elevation = height - depth;
+            const float4* nd = root + 4u * (nodeIdx - 1u);
+            const uint32_t typecode = __float_as_uint(nd[3].z);
+            if (typecode == TC_ZERO) { nodeIdx = postorder_next(nodeIdx, elevation); continue; }
+            if (typecode >= TC_NODE) {
+                Isect e;
+                intersect_leaf(e, nd, itra, tmin, ro, rd);
+                e.w = copysignf(e.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
+                if (sc >= OA_CSG_STACK - 1) { ierr = 1; break; }
+                st[++sc] = e;
+            } else {
+                if (sc < 1) { ierr = 1; break; }
+                const bool firstLeft = sgnbit(st[sc].w);
+                const bool secondLeft = sgnbit(st[sc - 1].w);
+                if (!(firstLeft ^ secondLeft)) { ierr = 1; break; }
+                const int left = firstLeft ? sc : sc - 1;
+                const int right = firstLeft ? sc - 1 : sc;
+                const Isect L = st[left];
+                const Isect R = st[right];
+                int l_state = classify(L, rd, tmin);
+                int r_state = classify(R, rd, tmin);
+                const float t_left = fabsf(L.w);
+                const float t_right = fabsf(R.w);
+                bool leftIsCloser = t_left <= t_right;
+                const bool l_promote = l_state == ST_MISS && (sgnbit(L.x) || sgnbit(L.y));
+                const bool r_promote = r_state == ST_MISS && (sgnbit(R.x) || sgnbit(R.y));
+                if (r_promote) { r_state = ST_EXIT; leftIsCloser = true; }
+                if (l_promote) { l_state = ST_EXIT; leftIsCloser = false; }
+                const uint32_t off = 3u * (uint32_t)l_state + (uint32_t)r_state;
+                const uint32_t word = (typecode >= TC_UNION && typecode <= TC_DIFFERENCE)
+                                    ? (leftIsCloser ? kLutA[typecode - 1u] : kLutB[typecode - 1u]) : 0u;
+                const int ctrl = off < 8u ? (int)((word >> (4u * off)) & 0xfu) : CT_MISS;
+                if (ctrl < CT_LOOP_A) {
+                    Isect res;
+                    if (ctrl == CT_MISS) { res.x = 0.f; res.y = 0.f; res.z = 0.f; res.w = 0.f; }
+                    else res = (ctrl == CT_RETURN_A) ? L : R;
+                    if (ctrl == CT_RETURN_FLIP_B) { res.x = -res.x; res.y = -res.y; res.z = -res.z; }
+                    res.w = copysignf(res.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
+                    sc -= 2;
+                    st[++sc] = res;
+                } else {
+                    const bool loopA = ctrl == CT_LOOP_A;
+                    const uint32_t leftIdx = 2u * nodeIdx;
+                    const uint32_t rightIdx = leftIdx + 1u;
+                    const float tminAdvanced = fabsf((loopA ? L : R).w) + propagate_epsilon;
+                    const Isect other = loopA ? R : L;
+                    sc -= 2;
+                    st[++sc] = other;
+                    const uint32_t endTree = nodeIdx | (endIdx << 8);
+                    const uint32_t leftTree = (leftIdx << (elevation - 1u)) | ((rightIdx << (elevation - 1u)) << 8);
+                    const uint32_t rightTree = (rightIdx << (elevation - 1u)) | (nodeIdx << 8);
+                    if (trc >= OA_TRANCHE_STACK - 2) { ierr = 1; break; }
+                    trs[++trc] = endTree;  trt[trc] = tmin;
+                    trs[++trc] = loopA ? leftTree : rightTree;  trt[trc] = tminAdvanced;
+                    restart = true;
+                    break;
+                }
+            }
+            nodeIdx = postorder_next(nodeIdx, elevation);
+        }
+        if (ierr) break;
+        (void)restart;
+    }
+    out.x = 0.f; out.y = 0.f; out.z = 0.f; out.w = 0.f;
+    if (!ierr && sc == 0) out = st[0];
+    else if (ierr && sc == 0) out = st[0];
+    return out.w > 0.f;
+}
+
+OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
+{
+    const uint32_t typecode = __float_as_uint(root[3].z);
+    if (typecode >= TC_LEAF) return intersect_leaf(is, root, itra, t_min, ro, rd);
+    is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
+    if (typecode < TC_NODE) return intersect_tree(is, root, itra, t_min, ro, rd);
+    return false;
+}
+
+} // namespace oa

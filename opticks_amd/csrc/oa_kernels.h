@@ -1,0 +1,20 @@
+/* oa_kernels.h : host-side launchers of the gfx950 kernels (oa_kernels.hip). */
+#pragma once
+#include <hip/hip_runtime.h>
+#include "oa_layout.h"
+
+namespace oa_launch {
+hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s);
+hipError_t simulate(const SimParams& P, int kernel, int block, int grid, hipStream_t s);
+hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
+hipError_t compact_hits(const unsigned long long* d_words, int64_t n_words, int32_t* d_word_prefix,
+                        int64_t* d_block_total, const float4* d_photon, float4* d_hit, int64_t hit_cap,
+                        hipStream_t s);
+hipError_t event_hit_offsets(const unsigned long long* d_words, const int32_t* d_word_prefix,
+                             const int64_t* d_block_total, const int64_t* d_ev_ph_begin, int32_t n_event,
+                             int64_t n_words, int64_t* d_hit_offsets, hipStream_t s);
+hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out, hipStream_t s);
+hipError_t refit(const float* d_o2w, const BvhNode* d_blas, const uint32_t* d_inst_blas_root, int64_t n_inst,
+                 float* d_inst_box, BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth,
+                 const uint32_t* d_tlas_items, hipStream_t s);
+}

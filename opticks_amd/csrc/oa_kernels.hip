@@ -1,0 +1,449 @@
+/*
+ * oa_kernels.hip : gfx950 kernels of the propagation path.
+ *
+ *   k_gs_prefix       exclusive prefix of genstep numphoton (QEvt.cu:181-226 seeding
+ *                     reduce+expand; photons find their genstep by binary search
+ *                     instead of a per-photon seed array)
+ *   k_sim_plain       one lane per photon, the CSGOptiX7.cu:432-530 shape
+ *   k_sim_wave        persistent waves: lanes whose photon finished take a new
+ *                     one from a global counter every refill_period bounces
+ *                     (wavefront compaction of live photons)
+ *   k_hit_count/k_hit_scan/k_hit_scatter
+ *                     stable stream compaction of flagmask&hit_mask photons via
+ *                     a per-64-photon bitmask (QEvt::gatherHit_, SU.cu:145-160)
+ *   k_trace           nearest-hit query (simtrace-style)
+ *   k_inst_boxes/k_refit_level  GPU refit of the TLAS after instance moves
+ */
+#include <hip/hip_runtime.h>
+#include "oa_physics.h"
+#include "oa_kernels.h"
+
+using namespace oa;
+
+/* ------------------------------------------------------------ seeding */
+
+__global__ void __launch_bounds__(1024) k_gs_prefix(const float* gs, int64_t n_gs, int64_t* prefix)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n_gs + 1023) / 1024;
+    const int64_t b = t * per, e = (b + per < n_gs) ? b + per : n_gs;
+    int64_t s = 0;
+    for (int64_t g = b; g < e; ++g) s += (int64_t)__float_as_uint(gs[24 * g + 3]);
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - s;    /* exclusive */
+    for (int64_t g = b; g < e; ++g) { prefix[g] = run; run += (int64_t)__float_as_uint(gs[24 * g + 3]); }
+    if (t == 1023) prefix[n_gs] = part[1023];
+}
+
+/* --------------------------------------------------------- per photon */
+
+struct Lane {
+    Photon p;
+    Rng rng;
+    uint64_t sh0, sh1, sb0, sb1;
+    int bounce;
+    int64_t g;
+};
+
+OA_D void seq_add(Lane& L, unsigned slot, uint32_t flag, uint32_t boundary)
+{
+    /* sseq::add_nibble (sysrap/sseq.h:174-183) */
+    const unsigned iseq = slot / 16u;
+    const unsigned shift = 4u * (slot - iseq * 16u);
+    const uint64_t f = (uint64_t)((flag ? (uint32_t)__builtin_ffs((int)flag) : 0u) & 0xfu) << shift;
+    const uint64_t bn = (uint64_t)(boundary & 0xfu) << shift;
+    if (iseq == 0u) { L.sh0 |= f; L.sb0 |= bn; }
+    else if (iseq == 1u) { L.sh1 |= f; L.sb1 |= bn; }
+}
+
+OA_D int64_t upper_index(const int64_t* a, int64_t lo, int64_t hi, int64_t v)
+{
+    /* largest k in [lo, hi) with a[k] <= v */
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* generation for global photon g: genstep lookup, RNG stream, generate_photon */
+OA_D void lane_start(const SimParams& P, Lane& L, int64_t g)
+{
+    L.g = g;
+    int e = 0;
+    if (P.n_event > 1) e = (int)upper_index(P.ev_ph_begin, 0, P.n_event, g);
+    const int64_t gsi = upper_index(P.gs_prefix, P.ev_gs_begin[e], P.ev_gs_begin[e + 1], g);
+    const int64_t in_event = g - P.ev_ph_begin[e];
+    const uint64_t photon_idx = (uint64_t)(P.photon_offset + in_event);
+    L.rng.kind = P.rng_kind;
+    if (P.rng_kind == 1) {
+        L.rng.cursor = 0;
+        const bool ok = (int64_t)photon_idx < P.pre_ni;
+        L.rng.pre = ok ? P.precooked + (int64_t)photon_idx * P.pre_nk : nullptr;
+        L.rng.pre_nk = ok ? P.pre_nk : 0;
+    } else {
+        oa_rng_init(&L.rng.ph, P.seed, P.rng_offset, P.skipahead, (uint64_t)P.ev_id[e], photon_idx);
+    }
+    const float4* gq = reinterpret_cast<const float4*>(P.gensteps + 24 * gsi);
+    const int gencode = (int)__float_as_uint(gq[0].x);
+    if (gencode == 6) gen_torch(L.p, L.rng, gq, photon_idx);
+    else gen_scint(P, L.p, L.rng, gq);
+    /* sphoton::set_index (sysrap/sphoton.h:230-237) */
+    L.p.index = (uint32_t)(photon_idx & 0xffffffffu);
+    L.p.identity = ((uint32_t)((photon_idx >> 32) & 0xffu) << 24) | (L.p.identity & 0xffffffu);
+    L.sh0 = L.sh1 = L.sb0 = L.sb1 = 0ull;
+    L.bounce = 0;
+    seq_add(L, 0u, ph_flag(L.p), ph_boundary(L.p));
+}
+
+/* one iteration of the CSGOptiX7.cu:477-509 loop; returns true when the photon is done */
+template <int STRIDE>
+OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk)
+{
+    if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return true;
+    const float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
+    Prd prd;
+    trace<STRIDE>(P, stk, L.p.pos, L.p.mom, tmin, P.tmax, prd);
+    if (prd.boundary == 0xffffu) return true;
+    prd.normal = normalize(prd.normal);
+    const int command = propagate(P, L.rng, L.p, prd);
+    L.bounce++;
+    seq_add(L, (unsigned)L.bounce, ph_flag(L.p), ph_boundary(L.p));
+    if (command == CMD_BREAK) return true;
+    return !(L.bounce < P.max_bounce && L.p.time < P.max_time);
+}
+
+OA_D void lane_finish(const SimParams& P, const Lane& L)
+{
+    const int64_t g = L.g;
+    if (P.photon) {
+        float4* o = P.photon + 4 * g;
+        o[0] = make_float4(L.p.pos.x, L.p.pos.y, L.p.pos.z, L.p.time);
+        o[1] = make_float4(L.p.mom.x, L.p.mom.y, L.p.mom.z, __uint_as_float(L.p.hitcount_iindex));
+        o[2] = make_float4(L.p.pol.x, L.p.pol.y, L.p.pol.z, L.p.wavelength);
+        o[3] = make_float4(__uint_as_float(L.p.obf), __uint_as_float(L.p.identity),
+                           __uint_as_float(L.p.index), __uint_as_float(L.p.flagmask));
+    }
+    if (P.seq) {
+        ulonglong2* s = reinterpret_cast<ulonglong2*>(P.seq + 4 * g);
+        s[0] = make_ulonglong2(L.sh0, L.sh1);
+        s[1] = make_ulonglong2(L.sb0, L.sb1);
+    }
+}
+
+/* ----------------------------------------------------- plain kernel */
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_sim_plain(SimParams P)
+{
+    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
+    uint32_t* stk = s_stack + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    bool hit = false;
+    if (g < P.n_photon) {
+        Lane L;
+        lane_start(P, L, g);
+        while (!lane_step<BLOCK>(P, L, stk)) {}
+        lane_finish(P, L);
+        hit = (L.p.flagmask & P.hit_mask) != 0u;
+    }
+    const unsigned long long m = __ballot(hit);
+    if ((threadIdx.x & 63) == 0 && g < P.n_photon) P.hitwords[g >> 6] = m;
+}
+
+/* ------------------------------------------------ persistent kernel */
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_sim_wave(SimParams P)
+{
+    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
+    uint32_t* stk = s_stack + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    Lane L;
+    bool active = false;
+    bool exhausted = false;
+    const int period = P.refill_period > 0 ? P.refill_period : 1;
+    for (int iter = 0;; ++iter) {
+        const unsigned long long idle = __ballot(!active);
+        const unsigned long long all = __ballot(true);
+        if (!exhausted && idle != 0ull && (idle == all || (iter % period) == 0)) {
+            const int n_idle = __popcll(idle);
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0ull;
+            if (lane == leader) base = atomicAdd(P.next, (unsigned long long)n_idle);
+            base = __shfl(base, leader);
+            if (!active) {
+                const unsigned long long below = idle & ((1ull << lane) - 1ull);
+                const int64_t g = (int64_t)(base + (unsigned long long)__popcll(below));
+                if (g < P.n_photon) {
+                    lane_start(P, L, g);
+                    active = true;
+                }
+            }
+            if ((int64_t)(base + (unsigned long long)n_idle) >= P.n_photon) exhausted = true;
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            if (lane_step<BLOCK>(P, L, stk)) {
+                lane_finish(P, L);
+                if ((L.p.flagmask & P.hit_mask) != 0u)
+                    atomicOr(P.hitwords + (L.g >> 6), 1ull << (L.g & 63));
+                active = false;
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------- hit compaction */
+
+/* per word popcount -> in-block exclusive prefix; block totals */
+__global__ void __launch_bounds__(256) k_hit_count(const unsigned long long* words, int64_t n_words,
+                                                   int32_t* word_prefix, int64_t* block_total)
+{
+    __shared__ int32_t s[256];
+    const int t = threadIdx.x;
+    const int64_t w = (int64_t)blockIdx.x * 256 + t;
+    const int32_t c = w < n_words ? __popcll(words[w]) : 0;
+    s[t] = c;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int32_t v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    if (w < n_words) word_prefix[w] = s[t] - c;
+    if (t == 255) block_total[blockIdx.x] = s[255];
+}
+
+/* exclusive scan of block totals in place; total into block_total[n_blocks] */
+__global__ void __launch_bounds__(1024) k_hit_scan(int64_t* block_total, int64_t n_blocks)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n_blocks + 1023) / 1024;
+    const int64_t b = t * per, e = (b + per < n_blocks) ? b + per : n_blocks;
+    int64_t s = 0;
+    for (int64_t i = b; i < e; ++i) s += block_total[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - s;
+    for (int64_t i = b; i < e; ++i) { int64_t v = block_total[i]; block_total[i] = run; run += v; }
+    if (t == 1023) block_total[n_blocks] = part[1023];
+}
+
+/* one wave per bitmask word; lane i copies photon 64*w+i when its bit is set */
+__global__ void __launch_bounds__(256) k_hit_scatter(const unsigned long long* words, int64_t n_words,
+                                                     const int32_t* word_prefix, const int64_t* block_total,
+                                                     const float4* photon, float4* hit, int64_t hit_cap)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    if (w >= n_words) return;
+    const unsigned long long m = words[w];
+    if (!((m >> lane) & 1ull)) return;
+    const int64_t base = block_total[w >> 8] + word_prefix[w];
+    const int64_t r = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (r >= hit_cap) return;
+    const float4* src = photon + 4 * (w * 64 + lane);
+    float4* dst = hit + 4 * r;
+    dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3];
+}
+
+/* hits among photons [0, ph) for each event boundary ph = ev_ph_begin[e] */
+__global__ void k_hit_event_offsets(const unsigned long long* words, const int32_t* word_prefix,
+                                    const int64_t* block_total, const int64_t* ev_ph_begin, int32_t n_event,
+                                    int64_t n_words, int64_t* hit_offsets)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > n_event) return;
+    const int64_t ph = ev_ph_begin[e];
+    const int64_t w = ph >> 6;
+    int64_t r;
+    if (w >= n_words) r = block_total[(n_words + 255) / 256];
+    else r = block_total[w >> 8] + word_prefix[w] + __popcll(words[w] & ((1ull << (ph & 63)) - 1ull));
+    hit_offsets[e] = r;
+}
+
+/* --------------------------------------------------------- trace query */
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace(SimParams P, const float4* rays, int64_t n, float4* out)
+{
+    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = rays[2 * i], b = rays[2 * i + 1];
+    Prd prd;
+    trace<BLOCK>(P, s_stack + threadIdx.x, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, prd);
+    const bool miss = prd.boundary == 0xffffu;
+    const v3 nn = miss ? prd.normal : normalize(prd.normal);
+    out[2 * i] = make_float4(nn.x, nn.y, nn.z, prd.t);
+    const uint32_t inst = miss ? 0xffffffffu : (prd.iindex_identity >> 16);
+    out[2 * i + 1] = make_float4(__uint_as_float(prd.iindex_identity), __uint_as_float(prd.boundary),
+                                 prd.lposcost, __uint_as_float(inst));
+}
+
+/* --------------------------------------------------------------- refit */
+
+/* world box of each instance: transform the 8 corners of its solid's BLAS root box */
+__global__ void k_inst_boxes(const float* o2w, const BvhNode* blas, const uint32_t* inst_blas_root,
+                             int64_t n_inst, float* inst_box, float pad_rel, float pad_abs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    const BvhNode& r = blas[inst_blas_root[i]];
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(r.c0 != OA_BVH_EMPTY ? r.lo0[k] : 1e30f, r.c1 != OA_BVH_EMPTY ? r.lo1[k] : 1e30f);
+        hi[k] = fmaxf(r.c0 != OA_BVH_EMPTY ? r.hi0[k] : -1e30f, r.c1 != OA_BVH_EMPTY ? r.hi1[k] : -1e30f);
+    }
+    const float* m = o2w + 16 * i;
+    float wl[3] = { 1e30f, 1e30f, 1e30f }, wh[3] = { -1e30f, -1e30f, -1e30f };
+    for (int c = 0; c < 8; ++c) {
+        const float v0 = (c & 1) ? hi[0] : lo[0], v1 = (c & 2) ? hi[1] : lo[1], v2 = (c & 4) ? hi[2] : lo[2];
+        for (int k = 0; k < 3; ++k) {
+            const float p = v0 * m[k] + v1 * m[4 + k] + v2 * m[8 + k] + m[12 + k];
+            wl[k] = fminf(wl[k], p); wh[k] = fmaxf(wh[k], p);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        const float pad = pad_rel * (wh[k] - wl[k]) + pad_abs;
+        inst_box[6 * i + k] = wl[k] - pad;
+        inst_box[6 * i + 3 + k] = wh[k] + pad;
+    }
+}
+
+__device__ static void link_box(const BvhNode* nodes, const uint32_t* items, const float* inst_box, uint32_t c,
+                                float* lo, float* hi)
+{
+    for (int k = 0; k < 3; ++k) { lo[k] = 1e30f; hi[k] = -1e30f; }
+    if (c == OA_BVH_EMPTY) return;
+    if (c & OA_BVH_LEAF_BIT) {
+        const uint32_t f = c & 0x00ffffffu, n = (c >> 24) & 0x7fu;
+        for (uint32_t j = 0; j < n; ++j) {
+            const float* b = inst_box + 6 * items[f + j];
+            for (int k = 0; k < 3; ++k) { lo[k] = fminf(lo[k], b[k]); hi[k] = fmaxf(hi[k], b[3 + k]); }
+        }
+    } else {
+        const BvhNode& n = nodes[c];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(n.c0 != OA_BVH_EMPTY ? n.lo0[k] : 1e30f, n.c1 != OA_BVH_EMPTY ? n.lo1[k] : 1e30f);
+            hi[k] = fmaxf(n.c0 != OA_BVH_EMPTY ? n.hi0[k] : -1e30f, n.c1 != OA_BVH_EMPTY ? n.hi1[k] : -1e30f);
+        }
+    }
+}
+
+/* refit all TLAS nodes of one depth (deepest level first) */
+__global__ void k_refit_level(BvhNode* nodes, int64_t n_nodes, uint32_t depth, const uint32_t* items,
+                              const float* inst_box)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    BvhNode& n = nodes[i];
+    if (n.depth != depth) return;
+    link_box(nodes, items, inst_box, n.c0, n.lo0, n.hi0);
+    link_box(nodes, items, inst_box, n.c1, n.lo1, n.hi1);
+}
+
+/* ------------------------------------------------------ host launchers */
+
+namespace oa_launch {
+
+hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gs_prefix, dim3(1), dim3(1024), 0, s, d_gs, n_gs, d_prefix);
+    return hipGetLastError();
+}
+
+template <int B>
+static hipError_t sim_t(const SimParams& P, int kernel, int grid, hipStream_t s)
+{
+    if (kernel == 0) {
+        const int64_t nb = (P.n_photon + B - 1) / B;
+        hipLaunchKernelGGL(k_sim_plain<B>, dim3((unsigned)nb), dim3(B), 0, s, P);
+    } else {
+        hipLaunchKernelGGL(k_sim_wave<B>, dim3((unsigned)grid), dim3(B), 0, s, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t simulate(const SimParams& P, int kernel, int block, int grid, hipStream_t s)
+{
+    switch (block) {
+        case 64: return sim_t<64>(P, kernel, grid, s);
+        case 128: return sim_t<128>(P, kernel, grid, s);
+        case 256: return sim_t<256>(P, kernel, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t compact_hits(const unsigned long long* d_words, int64_t n_words, int32_t* d_word_prefix,
+                        int64_t* d_block_total, const float4* d_photon, float4* d_hit, int64_t hit_cap,
+                        hipStream_t s)
+{
+    const int64_t nb = (n_words + 255) / 256;
+    if (nb > 0) hipLaunchKernelGGL(k_hit_count, dim3((unsigned)nb), dim3(256), 0, s, d_words, n_words, d_word_prefix, d_block_total);
+    hipLaunchKernelGGL(k_hit_scan, dim3(1), dim3(1024), 0, s, d_block_total, nb);
+    if (d_hit && d_photon && n_words > 0) {
+        const int64_t nt = n_words * 64;
+        hipLaunchKernelGGL(k_hit_scatter, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                           d_words, n_words, d_word_prefix, d_block_total, d_photon, d_hit, hit_cap);
+    }
+    return hipGetLastError();
+}
+
+hipError_t event_hit_offsets(const unsigned long long* d_words, const int32_t* d_word_prefix,
+                             const int64_t* d_block_total, const int64_t* d_ev_ph_begin, int32_t n_event,
+                             int64_t n_words, int64_t* d_hit_offsets, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hit_event_offsets, dim3((n_event + 1 + 255) / 256), dim3(256), 0, s,
+                       d_words, d_word_prefix, d_block_total, d_ev_ph_begin, n_event, n_words, d_hit_offsets);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trace<128>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, P, rays, n, out);
+    return hipGetLastError();
+}
+
+hipError_t refit(const float* d_o2w, const BvhNode* d_blas, const uint32_t* d_inst_blas_root, int64_t n_inst,
+                 float* d_inst_box, BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth,
+                 const uint32_t* d_tlas_items, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_inst_boxes, dim3((unsigned)((n_inst + 255) / 256)), dim3(256), 0, s,
+                       d_o2w, d_blas, d_inst_blas_root, n_inst, d_inst_box, 1e-3f, 1e-2f);
+    for (int d = (int)max_depth; d >= 0; --d)
+        hipLaunchKernelGGL(k_refit_level, dim3((unsigned)((n_tlas + 255) / 256)), dim3(256), 0, s,
+                           d_tlas, n_tlas, (uint32_t)d, d_tlas_items, d_inst_box);
+    return hipGetLastError();
+}
+
+} // namespace oa_launch

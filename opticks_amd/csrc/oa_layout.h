@@ -1,0 +1,88 @@
+/*
+ * oa_layout.h : records shared by the host runtime and the gfx950 kernels.
+ *
+ * HBM layout (all 16-byte aligned, read with dwordx4 loads):
+ *   BvhNode   64 B  two child boxes + two child links ("wide-2" node: one
+ *                   fetch tests both children). Link: bit31 set = leaf with
+ *                   items [link & 0xffffff, +((link>>24)&0x7f)); 0xffffffff =
+ *                   empty; otherwise the child node index.
+ *   InstRec   64 B  world->object 3x4 (rows of qat4::right_multiply: m[0..2],
+ *                   m[4..6], m[8..10] linear, m[12..14] translation), then
+ *                   gas (solid) index, identity (sensor_identifier+1), blas root.
+ *   PrimRec   16 B  node offset of the prim's root CSGNode, boundary, globalPrimIdx.
+ *   nodes/itra      the reference CSGNode / qat4 arrays unchanged (64 B each).
+ *   bnd             (nbnd*4*2 rows, nwl, float4): row 2*line+k is one property
+ *                   group of one line, contiguous in wavelength so a lerp reads
+ *                   32 adjacent bytes.
+ */
+#pragma once
+#include <stdint.h>
+
+#define OA_BVH_LEAF_BIT 0x80000000u
+#define OA_BVH_EMPTY    0xffffffffu
+#define OA_STACK_DEPTH  32
+#define OA_CSG_MAX_HEIGHT 7
+
+struct BvhNode {
+    float lo0[3], hi0[3];
+    float lo1[3], hi1[3];
+    uint32_t c0, c1;
+    uint32_t depth, pad;
+};
+
+struct InstRec {
+    float m[16];        /* w2o: m[0..2], m[4..6], m[8..10] linear rows, m[12..14] translation;
+                           m[3] = gas (as bits), m[7] = identity (bits), m[11] = blas root (bits) */
+};
+
+struct PrimRec {
+    uint32_t node_offset;
+    uint32_t boundary;
+    uint32_t global_prim;
+    uint32_t pad;
+};
+
+struct SimParams {
+    /* geometry */
+    const float4*  node;
+    const float4*  itra;
+    const InstRec* inst;
+    const PrimRec* prim;
+    const BvhNode* tlas;
+    const uint32_t* tlas_items;   /* instance indices */
+    const BvhNode* blas;
+    const uint32_t* blas_items;   /* prim indices (global) */
+    /* physics */
+    const float4*  bnd;
+    const int32_t* optical;       /* (nbnd*4, 4) */
+    const float*   icdf;
+    int32_t n_wl;
+    float   wl_low, wl_step;
+    int32_t n_icdf, n_icdf_layer, icdf_hd;
+    /* config */
+    int32_t  max_bounce;
+    float    max_time, tmin, tmin0, tmax;
+    uint32_t eps0_mask;
+    uint32_t hit_mask;
+    uint64_t seed, rng_offset, skipahead;
+    int32_t  rng_kind;
+    const float* precooked;
+    int64_t  pre_ni, pre_nk;
+    /* event(s): photons are numbered globally over all events of the launch */
+    const float*   gensteps;      /* (n_gs,6,4) all events concatenated */
+    const int64_t* gs_prefix;     /* (n_gs+1) exclusive prefix of numphoton */
+    int64_t        n_gs;
+    int32_t        n_event;
+    const int64_t* ev_gs_begin;   /* (n_event+1) genstep index boundaries */
+    const int64_t* ev_ph_begin;   /* (n_event+1) global photon boundaries */
+    const int32_t* ev_id;         /* (n_event) event index for skipahead */
+    int64_t        n_photon;      /* total photons of the launch */
+    int64_t        photon_offset; /* added to the in-event index (single-event slicing) */
+    /* outputs */
+    float4*        photon;        /* (n_photon,4) float4 or null */
+    uint64_t*      seq;           /* (n_photon,2,2) or null */
+    unsigned long long* hitwords; /* (n_photon+63)/64 bitmask, zeroed before launch */
+    /* persistent kernel */
+    unsigned long long* next;     /* global photon counter */
+    int32_t refill_period;
+};

@@ -1,0 +1,577 @@
+/*
+ * oa_physics.h : qsim propagation physics, photon generation and world trace
+ * for gfx950. Function-by-function counterparts:
+ *   fill                 qbnd::fill_state            qudarap/qbnd.h:192-223
+ *   to_boundary          qsim::propagate_to_boundary qudarap/qsim.h:721-866
+ *   rayleigh             qsim::rayleigh_scatter      qudarap/qsim.h:604-692
+ *   at_boundary          qsim::propagate_at_boundary qudarap/qsim.h:1001-1208
+ *   at_surface           qsim::propagate_at_surface  qudarap/qsim.h:1680-1758
+ *   reflect_diffuse/specular, lambertian, marsaglia  qudarap/qsim.h:433-575,1988-2093
+ *   propagate            qsim::propagate             qudarap/qsim.h:2229-2338
+ *   gen_torch/gen_scint  storch::generate / qscint::generate
+ *                        sysrap/storch.h:192-520, qudarap/qscint.h:66-200
+ *   trace                optixTrace + IS/CH/MS       CSGOptiX/CSGOptiX7.cu:113-218,682-967
+ */
+#pragma once
+#include "oa_device.h"
+
+namespace oa {
+
+/* ----------------------------------------------------------- tables */
+
+/* One lerped float4 of bnd row (2*line+k) at wavelength nm: the FP32 form of
+   the linearly filtered, wrap-addressed texture fetch of qbnd::boundary_lookup. */
+OA_D float4 bnd_lookup(const SimParams& P, float nm, int line, int k)
+{
+    const int nwl = P.n_wl;
+    float fx = (nm - P.wl_low) / P.wl_step;
+    float fl = floorf(fx);
+    float a = fx - fl;
+    int i0 = (int)fl;
+    i0 = ((i0 % nwl) + nwl) % nwl;
+    int i1 = (i0 + 1) % nwl;
+    const float4* row = P.bnd + (int64_t)(2 * line + k) * nwl;
+    float4 v0 = row[i0];
+    float4 v1 = row[i1];
+    return make_float4(v0.x + a * (v1.x - v0.x), v0.y + a * (v1.y - v0.y),
+                       v0.z + a * (v1.z - v0.z), v0.w + a * (v1.w - v0.w));
+}
+
+struct State {
+    float4 material1;     /* n, absorption, scattering, reemission */
+    float  group_velocity;
+    float  n2;
+    float4 surface;       /* detect, absorb, specular, diffuse */
+    int    ems;
+};
+
+OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float cosTheta)
+{
+    const int line = (int)boundary * 4;
+    const int m1_line = cosTheta > 0.f ? line + 3 : line + 0;
+    const int m2_line = cosTheta > 0.f ? line + 0 : line + 3;
+    const int su_line = cosTheta > 0.f ? line + 2 : line + 1;
+    s.material1 = bnd_lookup(P, wl, m1_line, 0);
+    s.group_velocity = bnd_lookup(P, wl, m1_line, 1).x;
+    s.n2 = bnd_lookup(P, wl, m2_line, 0).x;
+    s.ems = P.optical[4 * su_line + 1];
+    s.surface = (s.ems != EMS_NOSURFACE) ? bnd_lookup(P, wl, su_line, 0) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+OA_D float icdf_tex(const SimParams& P, float x, int layer)
+{
+    const int n = P.n_icdf;
+    float xb = x * (float)n - 0.5f;
+    float fl = floorf(xb);
+    float a = xb - fl;
+    int i0 = (int)fl;
+    i0 = ((i0 % n) + n) % n;
+    int i1 = (i0 + 1) % n;
+    int L = layer < P.n_icdf_layer ? layer : 0;
+    const float* row = P.icdf + (int64_t)L * n;
+    float v0 = row[i0], v1 = row[i1];
+    return v0 + a * (v1 - v0);
+}
+
+OA_D float scint_wavelength(const SimParams& P, float u0)
+{
+    if (P.icdf_hd == 0) return icdf_tex(P, u0, 0);
+    if (P.icdf_hd == 10) {
+        bool lhs = u0 < 0.1f, rhs = u0 > 0.9f;
+        float x = lhs ? (u0 * 10.f) : (rhs ? (u0 - 0.9f) * 10.f : u0);
+        return icdf_tex(P, x, lhs ? 1 : (rhs ? 2 : 0));
+    }
+    if (P.icdf_hd == 20) {
+        bool lhs = u0 < 0.05f, rhs = u0 > 0.95f;
+        float x = lhs ? (u0 * 20.f) : (rhs ? (u0 - 0.95f) * 20.f : u0);
+        return icdf_tex(P, x, lhs ? 1 : (rhs ? 2 : 0));
+    }
+    return 0.f;
+}
+
+/* ---------------------------------------------------------- helpers */
+
+OA_D void rotateUz(v3& d, v3 u)   /* sysrap/smath.h:95-115 */
+{
+    float up = u.x * u.x + u.y * u.y;
+    if (up > 0.f) {
+        up = sqrtf(up);
+        float px = d.x, py = d.y, pz = d.z;
+        d.x = (u.x * u.z * px - u.y * py) / up + u.x * pz;
+        d.y = (u.y * u.z * px + u.x * py) / up + u.y * pz;
+        d.z = -up * px + u.z * pz;
+    } else if (u.z < 0.f) {
+        d.x = -d.x;
+        d.z = -d.z;
+    }
+}
+
+OA_D v3 uniform_sphere(float u0, float u1)
+{
+    float phi = u0 * 2.f * OA_PI_F;
+    float cosTheta = 2.f * u1 - 1.f;
+    float sinTheta = sqrtf(1.f - cosTheta * cosTheta);
+    float sp, cp;
+    oa_sincosf(phi, &sp, &cp);
+    return mk(cp * sinTheta, sp * sinTheta, cosTheta);
+}
+
+OA_D void marsaglia(v3& dir, Rng& rng)
+{
+    float u, v, b, a;
+    do {
+        float u0 = uni(rng);
+        float u1 = uni(rng);
+        u = 2.f * u0 - 1.f;
+        v = 2.f * u1 - 1.f;
+        b = u * u + v * v;
+    } while (b > 1.f);
+    a = 2.f * sqrtf(1.f - b);
+    dir = mk(a * u, a * v, 2.f * b - 1.f);
+}
+
+OA_D void lambertian(v3& dir, v3 normal, float orient, Rng& rng)
+{
+    float ndotv, u;
+    int count = 0;
+    do {
+        count++;
+        marsaglia(dir, rng);
+        ndotv = dot(dir, normal) * orient;
+        if (ndotv < 0.f) { dir = -1.f * dir; ndotv = -1.f * ndotv; }
+        u = uni(rng);
+    } while (!(u < ndotv) && (count < 1024));
+}
+
+OA_D void rayleigh(Photon& p, Rng& rng)
+{
+    v3 direction, polarization;
+    bool looping = true;
+    do {
+        float u0 = uni(rng), u1 = uni(rng), u2 = uni(rng), u3 = uni(rng), u4 = uni(rng);
+        float cosTheta = u0;
+        float sinTheta = sqrtf(1.0f - u0 * u0);
+        if (u1 < 0.5f) cosTheta = -cosTheta;
+        float sinPhi, cosPhi;
+        oa_sincosf(2.f * OA_PI_F * u2, &sinPhi, &cosPhi);
+        direction = mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
+        rotateUz(direction, p.mom);
+        float constant = -dot(direction, p.pol);
+        polarization = mk(p.pol.x + constant * direction.x, p.pol.y + constant * direction.y,
+                          p.pol.z + constant * direction.z);
+        if (dot(polarization, polarization) == 0.f) {
+            oa_sincosf(2.f * OA_PI_F * u3, &sinPhi, &cosPhi);
+            polarization = mk(cosPhi, sinPhi, 0.f);
+            rotateUz(polarization, direction);
+        } else {
+            if (u3 < 0.5f) polarization = -polarization;
+        }
+        polarization = normalize(polarization);
+        float doCosTheta = dot(polarization, p.pol);
+        looping = doCosTheta * doCosTheta < u4;
+    } while (looping);
+    p.mom = direction;
+    p.pol = polarization;
+}
+
+/* ----------------------------------------------------------- physics */
+
+OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, const State& s, float dist)
+{
+    const float absorption_length = s.material1.y;
+    const float scattering_length = s.material1.z;
+    const float reemission_prob = s.material1.w;
+    const float gv = s.group_velocity;
+    const float u_scattering = uni(rng);
+    const float u_absorption = uni(rng);
+    const float scattering_distance = -scattering_length * oa_logf(u_scattering);
+    const float absorption_distance = -absorption_length * oa_logf(u_absorption);
+    if (absorption_distance <= scattering_distance) {
+        if (absorption_distance <= dist) {
+            p.time += absorption_distance / gv;
+            p.pos = p.pos + absorption_distance * p.mom;
+            float u_reemit = reemission_prob == 0.f ? 2.f : uni(rng);
+            if (u_reemit < reemission_prob) {
+                float u_wl = uni(rng), u_mph = uni(rng), u_mct = uni(rng), u_pph = uni(rng), u_pct = uni(rng);
+                p.wavelength = scint_wavelength(P, u_wl);
+                p.mom = uniform_sphere(u_mph, u_mct);
+                p.pol = normalize(cross(uniform_sphere(u_pph, u_pct), p.mom));
+                flag = F_BULK_REEMIT;
+                return CMD_CONTINUE;
+            }
+            flag = F_BULK_ABSORB;
+            return CMD_BREAK;
+        }
+    } else if (scattering_distance <= dist) {
+        p.time += scattering_distance / gv;
+        p.pos = p.pos + scattering_distance * p.mom;
+        rayleigh(p, rng);
+        flag = F_BULK_SCATTER;
+        return CMD_CONTINUE;
+    }
+    p.pos = p.pos + dist * p.mom;
+    p.time += dist / gv;
+    return CMD_BOUNDARY;
+}
+
+OA_D int at_boundary(uint32_t& flag, Rng& rng, Photon& p, float n1, float n2, v3 normal)
+{
+    const float eta = n1 / n2;
+    const float _c1 = -dot(p.mom, normal);
+    const v3 on = _c1 < 0.f ? -normal : normal;
+    const v3 trans = cross(p.mom, on);
+    const float trans_length = length(trans);
+    const bool normal_incidence = trans_length < 1e-6f;
+    const v3 A_trans = normal_incidence ? p.pol : trans * (1.0f / trans_length);
+    const float E1_perp = dot(p.pol, A_trans);
+    const float c1 = fabsf(_c1);
+    const float c2c2 = 1.f - eta * eta * (1.f - c1 * c1);
+    const bool tir = c2c2 < 0.f;
+    const float EdotN = dot(p.pol, on);
+    const float c2 = tir ? 0.f : sqrtf(c2c2);
+    const float n1c1 = n1 * c1, n2c2 = n2 * c2, n2c1 = n2 * c1, n1c2 = n1 * c2;
+    float E1x, E1y;
+    if (normal_incidence) { E1x = 0.f; E1y = 1.f; }
+    else { E1x = E1_perp; E1y = length(p.pol - E1_perp * A_trans); }
+    const float E2_tx = 2.f * n1c1 * E1x / (n1c1 + n2c2);
+    const float E2_ty = 2.f * n1c1 * E1y / (n2c1 + n1c2);
+    const float E2_rx = E2_tx - E1x;
+    const float E2_ry = (n2 * E2_ty / n1) - E1y;
+    const float invR = 1.0f / sqrtf(E2_rx * E2_rx + E2_ry * E2_ry);
+    const float invT = 1.0f / sqrtf(E2_tx * E2_tx + E2_ty * E2_ty);
+    const float TransCoeff = (tir || n1c1 == 0.f) ? 0.f : n2c2 * (E2_tx * E2_tx + E2_ty * E2_ty) / n1c1;
+    const float u_reflect = uni(rng);
+    const bool reflect = u_reflect > TransCoeff;
+    p.mom = reflect ? p.mom + (2.0f * c1) * on : eta * p.mom + (eta * c1 - c2) * on;
+    const v3 A_paral = normalize(cross(p.mom, A_trans));
+    if (normal_incidence) {
+        p.pol = reflect ? p.pol * (n2 > n1 ? -1.f : 1.f) : p.pol;
+    } else if (reflect) {
+        p.pol = tir ? -p.pol + (2.f * EdotN) * on : (E2_rx * invR) * A_trans + (E2_ry * invR) * A_paral;
+    } else {
+        p.pol = (E2_tx * invT) * A_trans + (E2_ty * invT) * A_paral;
+    }
+    flag = reflect ? F_BOUNDARY_REFLECT : F_BOUNDARY_TRANSMIT;
+    return CMD_CONTINUE;
+}
+
+OA_D int at_surface(uint32_t& flag, Rng& rng, Photon& p, const State& s, v3 normal)
+{
+    const float detect = s.surface.x, absorb = s.surface.y, diffuse = s.surface.w;
+    const float u_surface = uni(rng);
+    const int action = u_surface < absorb + detect ? CMD_BREAK : CMD_CONTINUE;
+    if (action == CMD_BREAK) {
+        flag = u_surface < absorb ? F_SURFACE_ABSORB : F_SURFACE_DETECT;
+    } else if (u_surface < absorb + detect + diffuse) {
+        flag = F_SURFACE_DREFLECT;
+        const v3 old_mom = p.mom;
+        const float orient = dot(old_mom, normal) > 0.f ? -1.f : 1.f;
+        lambertian(p.mom, normal, orient, rng);
+        const v3 facet = normalize(p.mom - old_mom);
+        const float EdotN = dot(p.pol, facet);
+        p.pol = -1.f * p.pol + (2.f * EdotN) * facet;
+    } else {
+        flag = F_SURFACE_SREFLECT;
+        const float PdotN = dot(p.mom, normal);
+        p.mom = p.mom - (2.f * PdotN) * normal;
+        const float EdotN = dot(p.pol, normal);
+        p.pol = -1.f * p.pol + (2.f * EdotN) * normal;
+    }
+    return action;
+}
+
+OA_D int propagate(const SimParams& P, Rng& rng, Photon& p, const Prd& prd)
+{
+    const uint32_t boundary = prd.boundary;
+    const uint32_t identity = prd.iindex_identity & 0xffffu;
+    const uint32_t iindex = prd.iindex_identity >> 16;
+    const v3 normal = prd.normal;
+    const float cosTheta = dot(p.mom, normal);
+    /* sphoton::set_prd (sysrap/sphoton.h:499-505) */
+    p.obf = (p.obf & 0x8000ffffu) | ((boundary & 0x7fffu) << 16);
+    p.identity = (p.identity & 0xff000000u) | (identity & 0x00ffffffu);
+    p.obf = (p.obf & 0x7fffffffu) | ((cosTheta < 0.f ? 0x1u : 0x0u) << 31);
+    p.hitcount_iindex = 0x00010000u | (0x0000ffffu & iindex);
+    State s;
+    fill(P, s, boundary, p.wavelength, cosTheta);
+    uint32_t flag = 0u;
+    int command = to_boundary(P, flag, rng, p, s, prd.t);
+    if (command == CMD_BOUNDARY) {
+        if (s.ems == EMS_NOSURFACE) command = at_boundary(flag, rng, p, s.material1.x, s.n2, normal);
+        else if (s.ems == EMS_SURFACE) command = at_surface(flag, rng, p, s, normal);
+        else if (prd.lposcost < 0.f) command = at_surface(flag, rng, p, s, normal);
+        else if (s.ems == EMS_ZPLUS_SENSOR_A) { (void)uni(rng); flag = F_SURFACE_DETECT; command = CMD_BREAK; }
+    }
+    set_flag(p, flag);
+    return command;
+}
+
+/* -------------------------------------------------------- generation */
+
+enum : uint32_t { T_DISC = 1, T_LINE = 2, T_POINT = 3, T_CIRCLE = 4, T_SPHERE_MARSAGLIA = 6, T_SPHERE = 7,
+                  T_POINT_ISO = 100 };
+
+OA_D void zero_flags_set(Photon& p, uint32_t flag)
+{
+    p.obf = 0u; p.identity = 0u; p.index = 0u; p.flagmask = 0u; p.hitcount_iindex = 0u;
+    set_flag(p, flag);
+}
+
+OA_D void gen_torch(Photon& p, Rng& rng, const float4* gq, uint64_t photon_id)
+{
+    const float4 q0 = gq[0], q1 = gq[1], q2 = gq[2], q3 = gq[3], q4 = gq[4], q5 = gq[5];
+    const uint32_t numphoton = __float_as_uint(q0.w);
+    const uint32_t type = __float_as_uint(q5.w);
+    const v3 gpos = mk(q1.x, q1.y, q1.z);
+    const float gtime = q1.w;
+    const v3 gmom = mk(q2.x, q2.y, q2.z);
+    const float gweight = q2.w;
+    const float gwl = q3.w;
+    const float zx = q4.x, zy = q4.y, ax = q4.z, ay = q4.w;
+    const float radius = q5.x, distance = q5.y;
+    p.pos = mk(0.f, 0.f, 0.f); p.time = 0.f; p.mom = mk(0.f, 0.f, 0.f); p.pol = mk(0.f, 0.f, 0.f);
+    p.wavelength = 0.f;
+    if (type == T_SPHERE || type == T_POINT_ISO) {
+        p.wavelength = gwl; p.time = gtime;
+        float u_zenith = zx + uni(rng) * (zy - zx);
+        float u_azimuth = ax + uni(rng) * (ay - ax);
+        float phi = 2.f * OA_PI_F * u_azimuth;
+        float sinPhi, cosPhi; oa_sincosf(phi, &sinPhi, &cosPhi);
+        float cosTheta = 1.f - 2.0f * u_zenith;
+        float sinTheta = sqrtf(1.0f - cosTheta * cosTheta);
+        float flip = copysignf(1.f, radius);
+        p.mom = mk(flip * sinTheta * cosPhi, flip * sinTheta * sinPhi, flip * cosTheta);
+        float rad = fabsf(radius);
+        p.pos = mk(sinTheta * cosPhi * rad, sinTheta * sinPhi * rad, cosTheta * rad);
+        float phase = 2.f * OA_PI_F * distance;
+        float sph, cph; oa_sincosf(phase, &sph, &cph);
+        p.pol = mk(cph, sph, 0.f);
+        rotateUz(p.pol, p.mom);
+        if (type == T_POINT_ISO) {
+            p.pos = p.pos + gpos;
+            if (gweight > 0.f) p.wavelength = gwl + uni(rng) * gweight;
+        }
+    } else if (type == T_DISC) {
+        p.wavelength = gwl; p.time = gtime; p.mom = gmom;
+        float u_zenith = zx + uni(rng) * (zy - zx);
+        float u_azimuth = ax + uni(rng) * (ay - ax);
+        float r = radius * u_zenith;
+        float phi = 2.f * OA_PI_F * u_azimuth;
+        float sinPhi, cosPhi; oa_sincosf(phi, &sinPhi, &cosPhi);
+        p.pos = mk(r * cosPhi, r * sinPhi, 0.f);
+        rotateUz(p.pos, p.mom);
+        p.pos = p.pos + gpos;
+        p.pol = mk(sinPhi, -cosPhi, 0.f);
+        rotateUz(p.pol, p.mom);
+    } else if (type == T_SPHERE_MARSAGLIA) {
+        p.wavelength = gwl; p.time = gtime;
+        float u, v, b, a;
+        do {
+            float u0z = zx + uni(rng) * (zy - zx);
+            float u1a = ax + uni(rng) * (ay - ax);
+            u = 2.f * u0z - 1.f; v = 2.f * u1a - 1.f; b = u * u + v * v;
+        } while (b > 1.f);
+        a = 2.f * sqrtf(1.f - b);
+        float rad = fabsf(radius);
+        float flip = copysignf(1.f, radius);
+        p.mom = mk(flip * a * u, flip * a * v, flip * (2.f * b - 1.f));
+        p.pos = mk(a * u * rad, a * v * rad, (2.f * b - 1.f) * rad);
+        float phase = 2.f * OA_PI_F * distance;
+        float sph, cph; oa_sincosf(phase, &sph, &cph);
+        p.pol = mk(cph, sph, 0.f);
+        rotateUz(p.pol, p.mom);
+    } else if (type == T_LINE) {
+        p.wavelength = gwl; p.time = gtime; p.mom = gmom;
+        float frac = (float)photon_id / (float)numphoton;
+        float sfrac = 2.f * (frac - 0.5f);
+        float r = radius * sfrac;
+        p.pos = mk(r, 0.f, 0.f);
+        rotateUz(p.pos, p.mom);
+        p.pos = p.pos + gpos;
+        p.pol = mk(0.f, -1.f, 0.f);
+        rotateUz(p.pol, p.mom);
+    } else if (type == T_POINT) {
+        p.wavelength = gwl; p.time = gtime; p.mom = gmom;
+        p.pos = mk(0.f, 0.f, 0.f);
+        rotateUz(p.pos, p.mom);
+        p.pos = p.pos + gpos;
+        p.pol = mk(0.f, -1.f, 0.f);
+        rotateUz(p.pol, p.mom);
+    } else if (type == T_CIRCLE) {
+        p.wavelength = gwl; p.time = gtime;
+        float f = (float)photon_id / (float)numphoton;
+        float frac = ax * (1.f - f) + ay * (f);
+        float phi = 2.f * OA_PI_F * frac;
+        float sinPhi, cosPhi; oa_sincosf(phi, &sinPhi, &cosPhi);
+        float r = radius < 0.f ? -radius : radius;
+        p.mom = mk(radius < 0.f ? -cosPhi : cosPhi, 0.f, radius < 0.f ? -sinPhi : sinPhi);
+        p.pos = mk(r * cosPhi, 0.f, r * sinPhi);
+        p.pos = p.pos + gpos;
+        p.pol = mk(0.f, -1.f, 0.f);
+        rotateUz(p.pol, p.mom);
+    }
+    zero_flags_set(p, F_TORCH);
+}
+
+OA_D void gen_scint(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
+{
+    float u0 = uni(rng), u1 = uni(rng), u2 = uni(rng), u3 = uni(rng);
+    float cost = 1.f - 2.f * u0;
+    float sint = sqrtf((1.f - cost) * (1.f + cost));
+    float phi = 2.f * OA_PI_F * u1;
+    float sinp, cosp; oa_sincosf(phi, &sinp, &cosp);
+    p.mom = mk(sint * cosp, sint * sinp, cost);
+    p.pol = mk(cost * cosp, cost * sinp, -sint);
+    phi = 2.f * OA_PI_F * u2;
+    oa_sincosf(phi, &sinp, &cosp);
+    p.pol = normalize(cosp * p.pol + sinp * cross(p.mom, p.pol));
+    p.wavelength = scint_wavelength(P, u3);
+    const float4 q1 = gq[1], q2 = gq[2], q3 = gq[3], q5 = gq[5];
+    const float charge = q3.y, meanVelocity = q3.w, step_length = q2.w, scint_time = q5.x;
+    float fraction = charge == 0.f ? 1.f : uni(rng);
+    p.pos = mk(q1.x, q1.y, q1.z) + fraction * mk(q2.x, q2.y, q2.z);
+    float u4 = uni(rng);
+    float deltaTime = fraction * step_length / meanVelocity - scint_time * oa_logf(u4);
+    p.time = q1.w + deltaTime;
+    zero_flags_set(p, F_SCINTILLATION);
+}
+
+/* ------------------------------------------------------------- trace */
+
+OA_D bool box_hit(const float* lo, const float* hi, v3 o, v3 inv, float tmin, float tmax)
+{
+    float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
+    float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
+    float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    return tn <= tf;
+}
+
+OA_D void load_node(const BvhNode* nodes, uint32_t idx, float* lo0, float* hi0, float* lo1, float* hi1,
+                    uint32_t& c0, uint32_t& c1)
+{
+    const float4* q = reinterpret_cast<const float4*>(nodes + idx);
+    float4 a = q[0], b = q[1], c = q[2], d = q[3];
+    lo0[0] = a.x; lo0[1] = a.y; lo0[2] = a.z; hi0[0] = a.w;
+    hi0[1] = b.x; hi0[2] = b.y; lo1[0] = b.z; lo1[1] = b.w;
+    lo1[2] = c.x; hi1[0] = c.y; hi1[1] = c.z; hi1[2] = c.w;
+    c0 = __float_as_uint(d.x); c1 = __float_as_uint(d.y);
+}
+
+OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
+{
+    const float4 r0 = m[0], r1 = m[1], r2 = m[2], r3 = m[3];
+    oo = mk(r0.x * o.x + r1.x * o.y + r2.x * o.z + r3.x,
+            r0.y * o.x + r1.y * o.y + r2.y * o.z + r3.y,
+            r0.z * o.x + r1.z * o.y + r2.z * o.z + r3.z);
+    od = mk(r0.x * d.x + r1.x * d.y + r2.x * d.z,
+            r0.y * d.x + r1.y * d.y + r2.y * d.z,
+            r0.z * d.x + r1.z * d.y + r2.z * d.z);
+}
+
+/*
+ * Nearest hit over instances x prims with t in (tmin, tmax); exact ties go to
+ * the smaller (instance, prim) key so the answer does not depend on traversal
+ * order. stk: this lane's LDS stack, entry k at stk[k*stride].
+ */
+template <int STRIDE>
+OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float tmax, Prd& prd)
+{
+    const v3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    float best_t = tmax;
+    uint64_t best_key = ~0ull;
+    uint32_t best_inst = 0xffffffffu, best_prim = 0u;
+    Isect best_is; best_is.x = best_is.y = best_is.z = best_is.w = 0.f;
+    int sp = 0;
+    uint32_t node = 0u;
+    bool have = true;
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    while (have) {
+        uint32_t c0, c1;
+        load_node(P.tlas, node, lo0, hi0, lo1, hi1, c0, c1);
+        bool h0 = c0 != OA_BVH_EMPTY && box_hit(lo0, hi0, o, inv, tmin, best_t);
+        bool h1 = c1 != OA_BVH_EMPTY && box_hit(lo1, hi1, o, inv, tmin, best_t);
+        uint32_t next = 0xffffffffu;
+        for (int side = 0; side < 2; ++side) {
+            const bool h = side == 0 ? h0 : h1;
+            const uint32_t c = side == 0 ? c0 : c1;
+            if (!h) continue;
+            if (c & OA_BVH_LEAF_BIT) {
+                const uint32_t first = c & 0x00ffffffu, count = (c >> 24) & 0x7fu;
+                for (uint32_t k = 0; k < count; ++k) {
+                    const uint32_t ii = P.tlas_items[first + k];
+                    const float4* m = reinterpret_cast<const float4*>(P.inst + ii);
+                    v3 oo, od;
+                    inst_ray(m, o, d, oo, od);
+                    const v3 inv2 = mk(1.f / od.x, 1.f / od.y, 1.f / od.z);
+                    const uint32_t broot = __float_as_uint(m[2].w);
+                    /* BLAS of this instance's solid */
+                    const int sp_base = sp;
+                    uint32_t bn = broot;
+                    bool bhave = true;
+                    while (bhave) {
+                        float bl0[3], bh0[3], bl1[3], bh1[3];
+                        uint32_t b0, b1;
+                        load_node(P.blas, bn, bl0, bh0, bl1, bh1, b0, b1);
+                        bool g0 = b0 != OA_BVH_EMPTY && box_hit(bl0, bh0, oo, inv2, tmin, best_t);
+                        bool g1 = b1 != OA_BVH_EMPTY && box_hit(bl1, bh1, oo, inv2, tmin, best_t);
+                        uint32_t bnext = 0xffffffffu;
+                        for (int bs = 0; bs < 2; ++bs) {
+                            const bool g = bs == 0 ? g0 : g1;
+                            const uint32_t bc = bs == 0 ? b0 : b1;
+                            if (!g) continue;
+                            if (bc & OA_BVH_LEAF_BIT) {
+                                const uint32_t bf = bc & 0x00ffffffu, bcount = (bc >> 24) & 0x7fu;
+                                for (uint32_t j = 0; j < bcount; ++j) {
+                                    const uint32_t pi = P.blas_items[bf + j];
+                                    const PrimRec pr = P.prim[pi];
+                                    Isect is;
+                                    if (!intersect_prim(is, P.node + 4u * pr.node_offset, P.itra, tmin, oo, od)) continue;
+                                    const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
+                                    if (is.w < best_t || (is.w == best_t && key < best_key)) {
+                                        best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;
+                                    }
+                                }
+                            } else if (bnext == 0xffffffffu) {
+                                bnext = bc;
+                            } else {
+                                stk[sp * STRIDE] = bc; sp++;
+                            }
+                        }
+                        if (bnext != 0xffffffffu) bn = bnext;
+                        else if (sp > sp_base) { sp--; bn = stk[sp * STRIDE]; }
+                        else bhave = false;
+                    }
+                }
+            } else if (next == 0xffffffffu) {
+                next = c;
+            } else {
+                stk[sp * STRIDE] = c; sp++;
+            }
+        }
+        if (next != 0xffffffffu) node = next;
+        else if (sp > 0) { sp--; node = stk[sp * STRIDE]; }
+        else have = false;
+    }
+    if (best_inst == 0xffffffffu) {
+        prd.normal = mk(0.f, 0.f, 0.f); prd.t = 0.f; prd.lposcost = 0.f;
+        prd.iindex_identity = 0xffffffffu; prd.boundary = 0xffffu;
+        return;
+    }
+    const float4* m = reinterpret_cast<const float4*>(P.inst + best_inst);
+    const float4 r0 = m[0], r1 = m[1], r2 = m[2];
+    prd.normal = mk(r0.x * best_is.x + r0.y * best_is.y + r0.z * best_is.z,
+                    r1.x * best_is.x + r1.y * best_is.y + r1.z * best_is.z,
+                    r2.x * best_is.x + r2.y * best_is.y + r2.z * best_is.z);
+    prd.t = best_t;
+    v3 oo, od;
+    inst_ray(m, o, d, oo, od);
+    const v3 lpos = oo + best_t * od;
+    prd.lposcost = lpos.z / sqrtf(dot(lpos, lpos));
+    const uint32_t ident = __float_as_uint(r1.w);
+    prd.iindex_identity = ((best_inst & 0xffffu) << 16) | (ident & 0xffffu);
+    prd.boundary = P.prim[best_prim].boundary & 0xffffu;
+}
+
+} // namespace oa
