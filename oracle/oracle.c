@@ -1401,3 +1401,13 @@ void oracle_rayleigh(const float* mom, const float* pol, uint64_t seed, uint64_t
         pol_out[3 * i + 0] = p.pol.x; pol_out[3 * i + 1] = p.pol.y; pol_out[3 * i + 2] = p.pol.z;
     }
 }
+
+void oracle_logf_n(const float* x, float* y, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) y[i] = oa_logf(x[i]);
+}
+
+void oracle_sincosf_n(const float* x, float* s, float* c, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) oa_sincosf(x[i], &s[i], &c[i]);
+}

@@ -93,6 +93,8 @@ void oracle_packed_lut(uint32_t* acloser3, uint32_t* bcloser3);
 
 /* Portable math entry points for the accuracy tests. */
 float oracle_logf(float x);
+void  oracle_logf_n(const float* x, float* y, int64_t n);
+void  oracle_sincosf_n(const float* x, float* s, float* c, int64_t n);
 void  oracle_sincosf(float x, float* s, float* c);
 void  oracle_philox_block(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4);
 void  oracle_uniforms(uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event_idx,

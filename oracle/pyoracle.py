@@ -72,6 +72,8 @@ def lib():
         L.oracle_logf.restype = C.c_float
         L.oracle_logf.argtypes = [C.c_float]
         L.oracle_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.oracle_logf_n.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.oracle_sincosf_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_philox_block.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.oracle_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                       C.c_int32, C.c_void_p]
@@ -214,6 +216,21 @@ def philox_block(ctr, k0, k1):
 
 def logf(x):
     return lib().oracle_logf(float(x))
+
+
+def logf_n(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().oracle_logf_n(x.ctypes.data, y.ctypes.data, len(x))
+    return y
+
+
+def sincosf_n(x):
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().oracle_sincosf_n(x.ctypes.data, s.ctypes.data, c.ctypes.data, len(x))
+    return s, c
 
 
 def sincosf(x):

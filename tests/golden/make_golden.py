@@ -1,0 +1,75 @@
+"""
+Writes the golden fixtures of tests/golden from the reference checkout:
+
+  csg_golden.npz  rays against the prims of the 2,000-detector workload geometry
+                  (world box, acrylic and scintillator spheres, and the detector's
+                  difference(union(scaled zsphere, cylinder), cylinder) tree),
+                  intersected by the reference's own CSG code
+                  (CSG/csg_intersect_tree.h intersect_prim) compiled into
+                  oracle/_ref/libref_csg.so by oracle/Makefile.
+  csg_lut.json    the packed Boolean LUT words parsed from CSG/csg_classify.h.
+
+Run in the container that holds the reference (python tests/golden/make_golden.py);
+the outputs are data only and are what the CPU tests read.
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+REF = os.environ.get("OPTICKS_REFERENCE", "/root/reference")
+
+
+def lut():
+    src = open(os.path.join(REF, "CSG", "csg_classify.h")).read()
+    a = re.search(r"packed_boolean_lut_ACloser\[4\]\s*=\s*\{([^}]*)\}", src).group(1)
+    b = re.search(r"packed_boolean_lut_BCloser\[4\]\s*=\s*\{([^}]*)\}", src).group(1)
+    to = lambda s: [int(x.strip(), 16) for x in s.split(",") if x.strip()]
+    return dict(ACloser=to(a), BCloser=to(b), source="CSG/csg_classify.h")
+
+
+def csg():
+    import pyoracle as po
+    from opticks_amd import configs
+    po.build()
+    assert po.reflib() is not None, "oracle/_ref/libref_csg.so missing"
+    w = configs.config1(num_gs=1, per_gs=1)
+    g = w.geometry
+    rng = np.random.default_rng(20261202)
+    rows = []
+    for prim, n in ((3, 3000), (0, 300), (1, 300), (2, 300)):
+        for k in range(n):
+            if prim == 3:
+                o = rng.uniform(-420, 420, 3)
+                o[2] = rng.uniform(-420, 320)
+                if k % 5 == 0:       # aim at the bore and the neck from below
+                    o = np.array([rng.uniform(-150, 150), rng.uniform(-150, 150), -600.0])
+            else:
+                o = rng.uniform(-6600, 6600, 3)
+            d = rng.normal(size=3)
+            if prim == 3 and k % 5 == 0:
+                d = np.array([rng.normal() * 0.05, rng.normal() * 0.05, 1.0])
+            if k % 11 == 0:
+                d = np.zeros(3); d[k % 3] = 1.0 if k % 2 else -1.0
+            d /= np.linalg.norm(d)
+            tmin = [0.0, 0.05, 1.0][k % 3]
+            v, isect = po.ref_intersect_prim(g, prim, np.float32(tmin), o.astype(np.float32), d.astype(np.float32))
+            rows.append((prim, tmin, *o.astype(np.float32), *d.astype(np.float32), int(v), *isect))
+    a = np.array(rows, np.float64)
+    return dict(prim=a[:, 0].astype(np.int32), tmin=a[:, 1].astype(np.float32),
+                origin=a[:, 2:5].astype(np.float32), direction=a[:, 5:8].astype(np.float32),
+                valid=a[:, 8].astype(np.int32), isect=a[:, 9:13].astype(np.float32))
+
+
+if __name__ == "__main__":
+    with open(os.path.join(HERE, "csg_lut.json"), "w") as f:
+        json.dump(lut(), f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "csg_golden.npz"), **csg())
+    print("wrote", os.listdir(HERE))

@@ -1,0 +1,97 @@
+"""C-ABI library loads and exports every declared symbol; host-side logic."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from opticks_amd import _lib as L
+from opticks_amd import configs
+from opticks_amd.qsim import SEventConfig, seqhis_label, flag_mask, FLAG
+from opticks_amd.foundry import CSG_DIFFERENCE, CSG_UNION, CSG_ZSPHERE, CSG_CYLINDER, CSG_ZERO
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "opticks_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mi_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(L.SYMBOLS)
+
+
+def test_defaults_and_errors_without_compute():
+    c = L.MiConfig()
+    L.lib().mi_config_default(C.byref(c))
+    assert c.max_bounce == 31 and abs(c.tmin - 0.05) < 1e-9 and c.hit_mask == FLAG["SD"]
+    assert c.eps0_mask == flag_mask("TO,CK,SI,SC,RE") and c.event_skipahead == 100000
+    assert L.lib().mi_version().decode().startswith("opticks_amd")
+    # bad arguments are rejected before any device call
+    assert L.lib().mi_sim_create(None, None, C.byref(c), None) < 0
+    assert L.last_error()
+    h = C.c_void_p()
+    assert L.lib().mi_sim_create(None, None, C.byref(c), C.byref(h)) < 0 and not h.value
+    bad = L.MiConfig(); L.lib().mi_config_default(C.byref(bad)); bad.block_size = 96
+    assert L.lib().mi_sim_create(None, None, C.byref(bad), C.byref(h)) == -1
+    assert "block_size" in L.last_error()
+
+
+def test_seqhis_labels_and_masks():
+    assert seqhis_label(0x7c3) == "TO BT SD"      # nibbles 3, 0xc, 7
+    assert seqhis_label(0x4 << 0 | 0, 0) == "AB"
+    assert flag_mask("SD") == 0x40 and flag_mask("TO,BT") == 0x804
+
+
+def test_config_from_env(monkeypatch):
+    monkeypatch.setenv("OPTICKS_MAX_BOUNCE", "9")
+    monkeypatch.setenv("QRng__SEED_OFFSET", "7:3")
+    c = SEventConfig.from_env()
+    assert c.max_bounce == 9 and c.seed == 7 and c.rng_offset == 3
+
+
+def test_foundry_layout_of_detector_tree():
+    w = configs.config1(num_gs=1, per_gs=1)
+    g = w.geometry
+    prim = g["prim"].view(np.int32)
+    num, off = prim[3, 0, 0], prim[3, 0, 1]
+    assert num == 7
+    nodes = g["node"][off:off + 7].view(np.uint32)
+    assert list(nodes[:, 3, 2]) == [CSG_DIFFERENCE, CSG_UNION, CSG_CYLINDER, CSG_ZSPHERE, CSG_CYLINDER,
+                                    CSG_ZERO, CSG_ZERO]
+    assert nodes[0, 0, 0] == 7                                  # subNum on the root
+    assert (nodes[3, 3, 3] & 0x7FFFFFFF) == 1                   # ellipsoid scale transform
+    assert len(g["inst"]) == 2001
+    ident = g["inst"].view(np.int32)[:, 2, 3]
+    assert ident[0] == 0 and ident[1] == 1 and ident[-1] == 2000
+    # detectors point their +z at the centre
+    z_axis = g["inst"][1:, 2, :3]
+    pos = g["inst"][1:, 3, :3]
+    cosang = np.einsum("ij,ij->i", z_axis, -pos / np.linalg.norm(pos, axis=1, keepdims=True))
+    assert np.allclose(cosang, 1.0, atol=1e-5)
+
+
+def test_physics_tables():
+    w = configs.config1(num_gs=1, per_gs=1)
+    ph = w.physics
+    assert ph["bnd"].shape == (4, 4, 2, 761, 4)
+    assert ph["optical"].shape == (4, 4, 4)
+    ls = ph["boundary_names"].index("Acrylic///LS")
+    wl = 60 + np.arange(761)
+    reem = ph["bnd"][ls, 3, 0, :, 3]
+    assert np.all(reem[wl < 410] == np.float32(0.4)) and np.all(reem[wl >= 410] == 0)
+    assert np.allclose(ph["bnd"][ls, 3, 1, :, 0], 299.792458 / 1.49)
+    assert ph["optical"][ls, 1, 1] == 1 and ph["optical"][ls, 3, 1] == 0   # NoSurface / material
+    det = ph["boundary_names"].index("Water/detSurf//Vacuum")
+    assert ph["optical"][det, 1, 1] == 2 and np.allclose(ph["bnd"][det, 1, 0, :, :2], [0.25, 0.75])
+    icdf = ph["icdf"]
+    assert icdf.shape == (3, 4096) and np.all(np.diff(icdf[0]) >= 0)
+    assert abs(np.median(icdf[0]) - 430) < 0.1

@@ -1,0 +1,230 @@
+"""HIP path vs the CPU checker on the same seeded inputs (MI355X only).
+
+Bar: integer/bit outputs (seqhis, seqbnd, flags, indices) identical; photon
+floats identical bit for bit (both sides share operand order, no FMA
+contraction, correctly rounded sqrt/div and the same log/sin/cos), with the
+north-star floor of >= 99.9% seqhis agreement and 1e-3 mm / 1e-3 ns kept as
+the asserted tolerance for the rare photons that could differ.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from opticks_amd import configs
+from opticks_amd import _lib as L
+from opticks_amd.qsim import QSim, SEventConfig, FLAG
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+PLAIN, WAVE = L.MI_KERNEL_PLAIN, L.MI_KERNEL_WAVEFRONT
+
+
+def compare(gpu_photon, gpu_seq, ref_photon, ref_seq, floor=0.999):
+    assert gpu_seq.shape == ref_seq.shape
+    same_seq = np.all(gpu_seq == ref_seq, axis=(1, 2))
+    assert same_seq.mean() >= floor
+    bits = np.all(gpu_photon.view(np.uint32) == ref_photon.view(np.uint32), axis=(1, 2))
+    assert bits.mean() >= floor
+    # photons with identical history agree in position and time to 1e-3 mm / ns
+    m = same_seq
+    assert np.abs(gpu_photon[m, 0] - ref_photon[m, 0]).max() <= 1e-3
+    # integer words identical on identical histories
+    assert np.array_equal(gpu_photon[m, 3].view(np.uint32), ref_photon[m, 3].view(np.uint32))
+    return same_seq.mean(), bits.mean()
+
+
+def stable_hits(photon, mask=FLAG["SD"]):
+    return photon[(photon[:, 3, 3].view(np.uint32) & mask) != 0]
+
+
+@pytest.fixture(scope="module")
+def w0():
+    return configs.config0(num_photon=200_000)
+
+
+@pytest.fixture(scope="module")
+def w1():
+    return configs.config1(num_gs=40, per_gs=2500)
+
+
+@pytest.mark.parametrize("kernel", [PLAIN, WAVE])
+def test_config0_bitwise(w0, kernel):
+    sim = QSim(w0.geometry, w0.physics, SEventConfig(seed=w0.seed, kernel=kernel))
+    hits = sim.simulate(w0.gensteps, 0)
+    ref_p, ref_s = po.World(w0.geometry, w0.physics).simulate(w0.gensteps, seed=w0.seed)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s == 1.0 and b == 1.0
+    assert np.array_equal(hits.view(np.uint32), stable_hits(ref_p).view(np.uint32))
+    sim.close()
+
+
+@pytest.mark.parametrize("kernel", [PLAIN, WAVE])
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_config1_subset_bitwise(w1, kernel, block):
+    sim = QSim(w1.geometry, w1.physics, SEventConfig(seed=w1.seed, kernel=kernel, block_size=block))
+    hits = sim.simulate(w1.gensteps, 0)
+    ref_p, ref_s = po.World(w1.geometry, w1.physics).simulate(w1.gensteps, seed=w1.seed)
+    compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert np.array_equal(hits.view(np.uint32), stable_hits(ref_p).view(np.uint32))
+    sim.close()
+
+
+def test_config1_full_size_properties_and_samples():
+    """10M photons on the GPU: size-independent properties over all of them,
+    and bitwise parity with the checker on three sampled photon ranges."""
+    w = configs.config1()
+    N = w.num_photon
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs = torch.from_numpy(w.gensteps).to(DEV)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
+    hit = torch.empty((N // 2, 4, 4), dtype=torch.float32, device=DEV)
+    n_hit = sim.simulate_device(gs, photon, seq, hit, eventID=0, n_photon=N)
+    p = photon.cpu().numpy()
+    s = seq.cpu().numpy().view(np.uint64)
+    h = hit[:n_hit].cpu().numpy()
+    # properties of the whole event
+    assert np.array_equal(p[:, 3, 2].view(np.uint32), np.arange(N, dtype=np.uint32))
+    fm = p[:, 3, 3].view(np.uint32)
+    is_hit = (fm & FLAG["SD"]) != 0
+    assert n_hit == is_hit.sum()
+    assert np.array_equal(h.view(np.uint32), p[is_hit].view(np.uint32))       # stable order
+    final = p[:, 3, 0].view(np.uint32) & 0xFFFF
+    terminal = final & (FLAG["SD"] | FLAG["SA"] | FLAG["AB"])
+    assert np.mean(terminal != 0) > 0.99
+    assert np.all(np.isfinite(p[:, 0]))
+    assert np.abs(np.linalg.norm(p[:, 1, :3].astype(np.float64), axis=1) - 1).max() < 1e-4
+    first = s[:, 0, 0] & 0xF
+    assert np.all(first == 3)                                                   # TO
+    # three sampled ranges against the checker
+    W = po.World(w.geometry, w.physics)
+    for b in (0, N // 2 + 123, N - 4000):
+        rp, rs = W.simulate(w.gensteps, seed=w.seed, p_begin=b, p_end=b + 4000)
+        compare(p[b:b + 4000], s[b:b + 4000], rp, rs)
+    sim.close()
+
+
+def test_plain_and_wavefront_identical_full_size():
+    w = configs.config1()
+    N = w.num_photon
+    gs = torch.from_numpy(w.gensteps).to(DEV)
+    out = []
+    for k in (PLAIN, WAVE):
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=k))
+        photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+        seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
+        sim.simulate_device(gs, photon, seq, None, n_photon=N)
+        out.append((photon, seq))
+        sim.close()
+    assert torch.equal(out[0][1], out[1][1])
+    assert torch.equal(out[0][0].view(torch.int32), out[1][0].view(torch.int32))
+
+
+def test_trace_matches_checker_brute_force():
+    w = configs.config1(num_gs=1, per_gs=1)
+    sim = QSim(w.geometry, w.physics, SEventConfig())
+    rng = np.random.default_rng(7)
+    n = 60000
+    rays = np.zeros((n, 2, 4), np.float32)
+    rays[:, 0, :3] = rng.uniform(-6400, 6400, (n, 3))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 1, :3] = d
+    rays[:, 0, 3] = rng.choice([0.0, 0.05, 10.0], n)
+    rays[:, 1, 3] = 1e6
+    rt = torch.from_numpy(rays).to(DEV)
+    it = torch.zeros((n, 2, 4), dtype=torch.float32, device=DEV)
+    sim.trace_device(rt, it)
+    g = it.cpu().numpy()
+    r = po.World(w.geometry, w.physics).trace(rays, traverse=0)
+    assert np.array_equal(g[:, 0].view(np.uint32), r[:, 0].view(np.uint32))       # normal, t
+    assert np.array_equal(g[:, 1, :3].view(np.uint32), r[:, 1, :3].view(np.uint32))  # ids, boundary, lposcost
+    assert np.mean(g[:, 1, 1].view(np.uint32) == 0xFFFF) < 0.01
+    sim.close()
+
+
+def test_multi_slice_equals_single_launch(w1):
+    a = QSim(w1.geometry, w1.physics, SEventConfig(seed=3))
+    b = QSim(w1.geometry, w1.physics, SEventConfig(seed=3, max_slot=7000))
+    ha = a.simulate(w1.gensteps, 2)
+    hb = b.simulate(w1.gensteps, 2)
+    assert np.array_equal(a.seq, b.seq)
+    assert np.array_equal(a.photon.view(np.uint32), b.photon.view(np.uint32))
+    assert np.array_equal(ha.view(np.uint32), hb.view(np.uint32))
+    a.close(); b.close()
+
+
+def test_batch_equals_per_event_and_checker():
+    w = configs.config2(num_events=4, mean_gs=6, mean_photons=800)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=5))
+    gs_all = np.concatenate(w.events)
+    offs = np.cumsum([0] + [len(e) for e in w.events])
+    N = w.num_photon
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
+    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    ev_ids = [10, 11, 12, 13]
+    po_, ho = sim.simulate_batch(torch.from_numpy(gs_all).to(DEV), offs, ev_ids, photon, seq, hit)
+    P = photon.cpu().numpy(); S = seq.cpu().numpy().view(np.uint64); H = hit.cpu().numpy()
+    W = po.World(w.geometry, w.physics)
+    for e, gs in enumerate(w.events):
+        a, b = po_[e], po_[e + 1]
+        single = QSim(w.geometry, w.physics, SEventConfig(seed=5))
+        hs = single.simulate(gs, ev_ids[e])
+        assert np.array_equal(single.seq, S[a:b])
+        assert np.array_equal(single.photon.view(np.uint32), P[a:b].view(np.uint32))
+        assert np.array_equal(hs.view(np.uint32), H[ho[e]:ho[e + 1]].view(np.uint32))
+        single.close()
+        rp, rs = W.simulate(gs, event_idx=ev_ids[e], seed=5)
+        compare(P[a:b], S[a:b], rp, rs)
+    sim.close()
+
+
+def test_precooked_uniforms(w0):
+    n = 20000
+    gs = w0.gensteps.copy()
+    gs.view(np.uint32)[0, 0, 3] = n
+    pre = np.random.default_rng(9).random((n, 96)).astype(np.float32) * 0.999 + 0.0005
+    sim = QSim(w0.geometry, w0.physics, SEventConfig(precooked=pre))
+    sim.simulate(gs, 0)
+    rp, rs = po.World(w0.geometry, w0.physics).simulate(gs, precooked=pre)
+    s, b = compare(sim.photon, sim.seq, rp, rs)
+    assert s == 1.0 and b == 1.0
+    sim.close()
+
+
+def test_zero_gensteps_and_empty_counts(w0):
+    sim = QSim(w0.geometry, w0.physics, SEventConfig())
+    assert sim.simulate(np.zeros((0, 6, 4), np.float32), 0) is None
+    gs = w0.gensteps.copy()
+    gs.view(np.uint32)[0, 0, 3] = 0
+    assert sim.simulate(gs, 0) is None
+    sim.close()
+
+
+@pytest.mark.parametrize("kw", [dict(max_bounce=1), dict(max_bounce=0), dict(max_time=3.0)])
+def test_truncation_matches_checker(w1, kw):
+    cfg = SEventConfig(seed=8, **kw)
+    sim = QSim(w1.geometry, w1.physics, cfg)
+    sim.simulate(w1.gensteps, 0)
+    rp, rs = po.World(w1.geometry, w1.physics).simulate(w1.gensteps, seed=8, **kw)
+    s, b = compare(sim.photon, sim.seq, rp, rs)
+    assert s == 1.0
+    sim.close()
+
+
+def test_refit_after_moving_instances():
+    w = configs.config1(num_gs=10, per_gs=1000)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=4))
+    moved = w.geometry["inst"].copy()
+    moved[1:, 3, :3] *= np.float32(1.03)                   # detectors out to 5665 mm
+    sim.refit_instances(moved)
+    g2 = dict(w.geometry); g2["inst"] = moved
+    sim.simulate(w.gensteps, 0)
+    rp, rs = po.World(g2, w.physics).simulate(w.gensteps, seed=4)
+    compare(sim.photon, sim.seq, rp, rs)
+    sim.close()
