@@ -73,8 +73,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kernel", default="wavefront", choices=["wavefront", "plain"])
     ap.add_argument("--block", type=int, default=128)
-    ap.add_argument("--refill", type=int, default=4)
-    ap.add_argument("--leaf", type=int, default=2)
+    ap.add_argument("--refill", type=int, default=1)
+    ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true")

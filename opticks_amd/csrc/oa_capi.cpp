@@ -165,8 +165,8 @@ void mi_config_default(mi_config* c)
     c->max_slot = 0;
     c->kernel = MI_KERNEL_WAVEFRONT;
     c->block_size = 128;
-    c->refill_period = 4;
-    c->bvh_leaf_size = 2;
+    c->refill_period = 1;
+    c->bvh_leaf_size = 1;
     c->waves_per_cu = 0;
 }
 
@@ -454,11 +454,15 @@ static SimParams base_params(const mi_sim* sim)
     return P;
 }
 
+/* persistent grid: every resident slot of every CU (occupancy API), or the
+   configured waves per CU */
 static int wave_grid(const mi_sim* sim)
 {
-    int wpc = sim->cfg.waves_per_cu > 0 ? sim->cfg.waves_per_cu : 8;
-    int waves_per_block = sim->cfg.block_size / 64;
-    return std::max(1, sim->n_cu * wpc / waves_per_block);
+    const int waves_per_block = sim->cfg.block_size / 64;
+    if (sim->cfg.waves_per_cu > 0) return std::max(1, sim->n_cu * sim->cfg.waves_per_cu / waves_per_block);
+    int nb = oa_launch::wave_blocks_per_cu(sim->cfg.block_size);
+    if (nb <= 0) nb = std::max(1, 8 / waves_per_block);
+    return std::max(1, sim->n_cu * nb);
 }
 
 /*

@@ -130,6 +130,15 @@ OA_D void rq_roots_disq(float t_min, float& t1, float& t2, float& sdisc, float d
 
 struct Isect { float x, y, z, w; };
 
+/* value select: keeps Isect arrays in registers (a select of two addresses
+   would pin them to scratch memory) */
+OA_D Isect pick(bool c, const Isect& a, const Isect& b)
+{
+    Isect r;
+    r.x = c ? a.x : b.x; r.y = c ? a.y : b.y; r.z = c ? a.z : b.z; r.w = c ? a.w : b.w;
+    return r;
+}
+
 OA_D bool leaf_sphere(Isect& is, float4 q0, float t_min, v3 ro, v3 rd)
 {
     v3 O = ro - mk(q0.x, q0.y, q0.z);
@@ -324,35 +333,47 @@ OA_D int classify(const Isect& e, v3 dir, float tmin)
 OA_D uint32_t tdepth(uint32_t i) { return 31u - (uint32_t)__builtin_clz(i); }
 OA_D uint32_t postorder_next(uint32_t i, uint32_t elev) { return (i & 1u) ? i >> 1 : (i << elev) + (1u << elev); }
 
-/* Device stacks sized for trees of height <= OA_CSG_MAX_HEIGHT (checked at create). */
-#ifndef OA_CSG_STACK
-#define OA_CSG_STACK 8
-#endif
+/*
+ * Device stacks. The evaluation of a complete tree of height h never holds
+ * more than h+1 intersects (one pending left result per ancestor, plus the
+ * loop's "other" and the re-evaluated subtree), so trees of height <= 3 use a
+ * 4-deep stack and deeper ones (<= OA_CSG_MAX_HEIGHT, checked at create) an
+ * 8-deep one. Both stacks are shift registers with the top at index 0: every
+ * index is a compile-time constant, so they live in VGPRs, not scratch.
+ * The tranche stack keeps the reference's 4 entries and overflow behaviour
+ * (CSG/csg_tranche.h TRANCHE_STACK_SIZE).
+ */
 #define OA_TRANCHE_STACK 4
 
 /* intersect_tree (CSG/csg_intersect_tree.h:277-672) */
-OA_D bool intersect_tree(Isect& out, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
+template <int D>
+OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
 {
     const uint32_t numNode = __float_as_uint(root[0].x);
     const uint32_t height = (uint32_t)(__builtin_ffs((int)numNode + 1) - 2);
     const float propagate_epsilon = 0.0001f;
-    int ierr = 0;
-    float trt[OA_TRANCHE_STACK];
+    bool ierr = false;
     uint32_t trs[OA_TRANCHE_STACK];
-    int trc = 0;
-    trs[0] = (1u << height) | (0u << 8);    /* slice packed as begin | end<<8 */
+    float trt[OA_TRANCHE_STACK];
+    int trn = 1;
+    trs[0] = (1u << height);          /* slice: begin | end<<8, fullTree end = 0 */
     trt[0] = t_min;
-    Isect st[OA_CSG_STACK];
-    int sc = -1;
+#pragma unroll
+    for (int i = 1; i < OA_TRANCHE_STACK; ++i) { trs[i] = 0u; trt[i] = 0.f; }
+    Isect st[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) { st[i].x = 0.f; st[i].y = 0.f; st[i].z = 0.f; st[i].w = 0.f; }
+    int sn = 0;
 
-    while (trc > -1) {
-        const uint32_t slice = trs[trc];
-        const float tmin = trt[trc];
-        trc--;
+    while (trn > 0) {
+        const uint32_t slice = trs[0];
+        const float tmin = trt[0];
+#pragma unroll
+        for (int i = 0; i < OA_TRANCHE_STACK - 1; ++i) { trs[i] = trs[i + 1]; trt[i] = trt[i + 1]; }
+        trn--;
         const uint32_t beginIdx = slice & 0xffu;
         const uint32_t endIdx = (slice >> 8) & 0xffu;
         uint32_t nodeIdx = beginIdx;
-        bool restart = false;
         while (nodeIdx != endIdx) {
             const uint32_t depth = tdepth(nodeIdx);
             const uint32_t elevation = height - depth;
@@ -363,17 +384,18 @@ OA_D bool intersect_tree(Isect& out, const float4* root, const float4* itra, flo
                 Isect e;
                 intersect_leaf(e, nd, itra, tmin, ro, rd);
                 e.w = copysignf(e.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
-                if (sc >= OA_CSG_STACK - 1) { ierr = 1; break; }
-                st[++sc] = e;
+                if (sn >= D) { ierr = true; break; }
+#pragma unroll
+                for (int i = D - 1; i > 0; --i) st[i] = st[i - 1];
+                st[0] = e;
+                sn++;
             } else {
-                if (sc < 1) { ierr = 1; break; }
-                const bool firstLeft = sgnbit(st[sc].w);
-                const bool secondLeft = sgnbit(st[sc - 1].w);
-                if (!(firstLeft ^ secondLeft)) { ierr = 1; break; }
-                const int left = firstLeft ? sc : sc - 1;
-                const int right = firstLeft ? sc - 1 : sc;
-                const Isect L = st[left];
-                const Isect R = st[right];
+                if (sn < 2) { ierr = true; break; }
+                const bool firstLeft = sgnbit(st[0].w);
+                const bool secondLeft = sgnbit(st[1].w);
+                if (!(firstLeft ^ secondLeft)) { ierr = true; break; }
+                const Isect L = pick(firstLeft, st[0], st[1]);
+                const Isect R = pick(firstLeft, st[1], st[0]);
                 int l_state = classify(L, rd, tmin);
                 int r_state = classify(R, rd, tmin);
                 const float t_left = fabsf(L.w);
@@ -387,41 +409,52 @@ OA_D bool intersect_tree(Isect& out, const float4* root, const float4* itra, flo
                 const uint32_t word = (typecode >= TC_UNION && typecode <= TC_DIFFERENCE)
                                     ? (leftIsCloser ? kLutA[typecode - 1u] : kLutB[typecode - 1u]) : 0u;
                 const int ctrl = off < 8u ? (int)((word >> (4u * off)) & 0xfu) : CT_MISS;
+                /* pop two, push one: st[0] = result, entries below move up by one */
+#pragma unroll
+                for (int i = 1; i < D - 1; ++i) st[i] = st[i + 1];
+                sn--;
                 if (ctrl < CT_LOOP_A) {
                     Isect res;
                     if (ctrl == CT_MISS) { res.x = 0.f; res.y = 0.f; res.z = 0.f; res.w = 0.f; }
-                    else res = (ctrl == CT_RETURN_A) ? L : R;
+                    else res = pick(ctrl == CT_RETURN_A, L, R);
                     if (ctrl == CT_RETURN_FLIP_B) { res.x = -res.x; res.y = -res.y; res.z = -res.z; }
                     res.w = copysignf(res.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
-                    sc -= 2;
-                    st[++sc] = res;
+                    st[0] = res;
                 } else {
                     const bool loopA = ctrl == CT_LOOP_A;
                     const uint32_t leftIdx = 2u * nodeIdx;
                     const uint32_t rightIdx = leftIdx + 1u;
-                    const float tminAdvanced = fabsf((loopA ? L : R).w) + propagate_epsilon;
-                    const Isect other = loopA ? R : L;
-                    sc -= 2;
-                    st[++sc] = other;
+                    const float tminAdvanced = fabsf(loopA ? L.w : R.w) + propagate_epsilon;
+                    st[0] = pick(loopA, R, L);
                     const uint32_t endTree = nodeIdx | (endIdx << 8);
                     const uint32_t leftTree = (leftIdx << (elevation - 1u)) | ((rightIdx << (elevation - 1u)) << 8);
                     const uint32_t rightTree = (rightIdx << (elevation - 1u)) | (nodeIdx << 8);
-                    if (trc >= OA_TRANCHE_STACK - 2) { ierr = 1; break; }
-                    trs[++trc] = endTree;  trt[trc] = tmin;
-                    trs[++trc] = loopA ? leftTree : rightTree;  trt[trc] = tminAdvanced;
-                    restart = true;
+                    if (trn > OA_TRANCHE_STACK - 2) { ierr = true; break; }
+#pragma unroll
+                    for (int i = OA_TRANCHE_STACK - 1; i > 1; --i) { trs[i] = trs[i - 2]; trt[i] = trt[i - 2]; }
+                    trs[1] = endTree;  trt[1] = tmin;
+                    trs[0] = loopA ? leftTree : rightTree;  trt[0] = tminAdvanced;
+                    trn += 2;
                     break;
                 }
             }
             nodeIdx = postorder_next(nodeIdx, elevation);
         }
         if (ierr) break;
-        (void)restart;
     }
     out.x = 0.f; out.y = 0.f; out.z = 0.f; out.w = 0.f;
-    if (!ierr && sc == 0) out = st[0];
-    else if (ierr && sc == 0) out = st[0];
+    if (sn == 1) out = st[0];
     return out.w > 0.f;
+}
+
+/* deep trees (height 4..OA_CSG_MAX_HEIGHT): out of line, result returned by value */
+__device__ static __attribute__((noinline)) float4 intersect_tree_deep(const float4* root, const float4* itra,
+                                                                      float t_min, float ox, float oy, float oz,
+                                                                      float dx, float dy, float dz)
+{
+    Isect r;
+    intersect_tree_t<8>(r, root, itra, t_min, mk(ox, oy, oz), mk(dx, dy, dz));
+    return make_float4(r.x, r.y, r.z, r.w);
 }
 
 OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
@@ -429,7 +462,12 @@ OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, floa
     const uint32_t typecode = __float_as_uint(root[3].z);
     if (typecode >= TC_LEAF) return intersect_leaf(is, root, itra, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
-    if (typecode < TC_NODE) return intersect_tree(is, root, itra, t_min, ro, rd);
+    if (typecode < TC_NODE) {
+        if (__float_as_uint(root[0].x) <= 15u) return intersect_tree_t<4>(is, root, itra, t_min, ro, rd);
+        const float4 r = intersect_tree_deep(root, itra, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
+        is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
+        return is.w > 0.f;
+    }
     return false;
 }
 

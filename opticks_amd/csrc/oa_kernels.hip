@@ -20,6 +20,17 @@
 
 using namespace oa;
 
+/* Occupancy request for the propagation kernels: at least OA_WAVES_PER_EU
+   waves per SIMD caps their VGPR budget (512/N). 0 leaves it to the compiler. */
+#ifndef OA_WAVES_PER_EU
+#define OA_WAVES_PER_EU 3
+#endif
+#if OA_WAVES_PER_EU > 0
+#define OA_OCC __attribute__((amdgpu_waves_per_eu(OA_WAVES_PER_EU)))
+#else
+#define OA_OCC
+#endif
+
 /* ------------------------------------------------------------ seeding */
 
 __global__ void __launch_bounds__(1024) k_gs_prefix(const float* gs, int64_t n_gs, int64_t* prefix)
@@ -60,8 +71,11 @@ OA_D void seq_add(Lane& L, unsigned slot, uint32_t flag, uint32_t boundary)
     const unsigned shift = 4u * (slot - iseq * 16u);
     const uint64_t f = (uint64_t)((flag ? (uint32_t)__builtin_ffs((int)flag) : 0u) & 0xfu) << shift;
     const uint64_t bn = (uint64_t)(boundary & 0xfu) << shift;
-    if (iseq == 0u) { L.sh0 |= f; L.sb0 |= bn; }
-    else if (iseq == 1u) { L.sh1 |= f; L.sb1 |= bn; }
+    /* value selects, not a branch per field: a pointer phi would pin Lane to scratch */
+    L.sh0 |= iseq == 0u ? f : 0ull;
+    L.sb0 |= iseq == 0u ? bn : 0ull;
+    L.sh1 |= iseq == 1u ? f : 0ull;
+    L.sb1 |= iseq == 1u ? bn : 0ull;
 }
 
 OA_D int64_t upper_index(const int64_t* a, int64_t lo, int64_t hi, int64_t v)
@@ -142,7 +156,7 @@ OA_D void lane_finish(const SimParams& P, const Lane& L)
 /* ----------------------------------------------------- plain kernel */
 
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_sim_plain(SimParams P)
+__global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
     uint32_t* stk = s_stack + threadIdx.x;
@@ -162,7 +176,7 @@ __global__ void __launch_bounds__(BLOCK) k_sim_plain(SimParams P)
 /* ------------------------------------------------ persistent kernel */
 
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_sim_wave(SimParams P)
+__global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
     uint32_t* stk = s_stack + threadIdx.x;
@@ -380,6 +394,24 @@ static hipError_t sim_t(const SimParams& P, int kernel, int grid, hipStream_t s)
         hipLaunchKernelGGL(k_sim_wave<B>, dim3((unsigned)grid), dim3(B), 0, s, P);
     }
     return hipGetLastError();
+}
+
+template <int B>
+static int occ_t()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sim_wave<B>, B, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+int wave_blocks_per_cu(int block)
+{
+    switch (block) {
+        case 64: return occ_t<64>();
+        case 128: return occ_t<128>();
+        case 256: return occ_t<256>();
+        default: return 0;
+    }
 }
 
 hipError_t simulate(const SimParams& P, int kernel, int block, int grid, hipStream_t s)

@@ -438,12 +438,13 @@ OA_D void gen_scint(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
 
 /* ------------------------------------------------------------- trace */
 
-OA_D bool box_hit(const float* lo, const float* hi, v3 o, v3 inv, float tmin, float tmax)
+/* slab test; on a hit tn is the entry distance clamped to tmin */
+OA_D bool box_enter(const float* lo, const float* hi, v3 o, v3 inv, float tmin, float tmax, float& tn)
 {
     float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
     float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
     float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     return tn <= tf;
 }
@@ -457,6 +458,29 @@ OA_D void load_node(const BvhNode* nodes, uint32_t idx, float* lo0, float* hi0, 
     hi0[1] = b.x; hi0[2] = b.y; lo1[0] = b.z; lo1[1] = b.w;
     lo1[2] = c.x; hi1[0] = c.y; hi1[1] = c.z; hi1[2] = c.w;
     c0 = __float_as_uint(d.x); c1 = __float_as_uint(d.y);
+}
+
+/* Visit one wide-2 node: returns the nearer hit child link (or EMPTY) and
+   pushes the farther one. Leaf links go on the stack like node links, so
+   both levels are traversed strictly nearest-first and a shrinking tmax
+   culls what is still pending. */
+template <int STRIDE>
+OA_D uint32_t visit(const BvhNode* nodes, uint32_t idx, v3 o, v3 inv, float tmin, float tmax,
+                    uint32_t* stk, int& sp)
+{
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    uint32_t c0, c1;
+    load_node(nodes, idx, lo0, hi0, lo1, hi1, c0, c1);
+    float tn0, tn1;
+    const bool h0 = c0 != OA_BVH_EMPTY && box_enter(lo0, hi0, o, inv, tmin, tmax, tn0);
+    const bool h1 = c1 != OA_BVH_EMPTY && box_enter(lo1, hi1, o, inv, tmin, tmax, tn1);
+    if (h0 && h1) {
+        const bool first0 = tn0 <= tn1;
+        stk[sp * STRIDE] = first0 ? c1 : c0;
+        sp++;
+        return first0 ? c0 : c1;
+    }
+    return h0 ? c0 : (h1 ? c1 : OA_BVH_EMPTY);
 }
 
 OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
@@ -473,7 +497,7 @@ OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
 /*
  * Nearest hit over instances x prims with t in (tmin, tmax); exact ties go to
  * the smaller (instance, prim) key so the answer does not depend on traversal
- * order. stk: this lane's LDS stack, entry k at stk[k*stride].
+ * order. stk: this lane's LDS stack, entry k at stk[k*STRIDE].
  */
 template <int STRIDE>
 OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float tmax, Prd& prd)
@@ -484,75 +508,53 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     uint32_t best_inst = 0xffffffffu, best_prim = 0u;
     Isect best_is; best_is.x = best_is.y = best_is.z = best_is.w = 0.f;
     int sp = 0;
-    uint32_t node = 0u;
-    bool have = true;
-    float lo0[3], hi0[3], lo1[3], hi1[3];
-    while (have) {
-        uint32_t c0, c1;
-        load_node(P.tlas, node, lo0, hi0, lo1, hi1, c0, c1);
-        bool h0 = c0 != OA_BVH_EMPTY && box_hit(lo0, hi0, o, inv, tmin, best_t);
-        bool h1 = c1 != OA_BVH_EMPTY && box_hit(lo1, hi1, o, inv, tmin, best_t);
-        uint32_t next = 0xffffffffu;
-        for (int side = 0; side < 2; ++side) {
-            const bool h = side == 0 ? h0 : h1;
-            const uint32_t c = side == 0 ? c0 : c1;
-            if (!h) continue;
-            if (c & OA_BVH_LEAF_BIT) {
-                const uint32_t first = c & 0x00ffffffu, count = (c >> 24) & 0x7fu;
-                for (uint32_t k = 0; k < count; ++k) {
-                    const uint32_t ii = P.tlas_items[first + k];
-                    const float4* m = reinterpret_cast<const float4*>(P.inst + ii);
-                    v3 oo, od;
-                    inst_ray(m, o, d, oo, od);
-                    const v3 inv2 = mk(1.f / od.x, 1.f / od.y, 1.f / od.z);
-                    const uint32_t broot = __float_as_uint(m[2].w);
-                    /* BLAS of this instance's solid */
-                    const int sp_base = sp;
-                    uint32_t bn = broot;
-                    bool bhave = true;
-                    while (bhave) {
-                        float bl0[3], bh0[3], bl1[3], bh1[3];
-                        uint32_t b0, b1;
-                        load_node(P.blas, bn, bl0, bh0, bl1, bh1, b0, b1);
-                        bool g0 = b0 != OA_BVH_EMPTY && box_hit(bl0, bh0, oo, inv2, tmin, best_t);
-                        bool g1 = b1 != OA_BVH_EMPTY && box_hit(bl1, bh1, oo, inv2, tmin, best_t);
-                        uint32_t bnext = 0xffffffffu;
-                        for (int bs = 0; bs < 2; ++bs) {
-                            const bool g = bs == 0 ? g0 : g1;
-                            const uint32_t bc = bs == 0 ? b0 : b1;
-                            if (!g) continue;
-                            if (bc & OA_BVH_LEAF_BIT) {
-                                const uint32_t bf = bc & 0x00ffffffu, bcount = (bc >> 24) & 0x7fu;
-                                for (uint32_t j = 0; j < bcount; ++j) {
-                                    const uint32_t pi = P.blas_items[bf + j];
-                                    const PrimRec pr = P.prim[pi];
-                                    Isect is;
-                                    if (!intersect_prim(is, P.node + 4u * pr.node_offset, P.itra, tmin, oo, od)) continue;
-                                    const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
-                                    if (is.w < best_t || (is.w == best_t && key < best_key)) {
-                                        best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;
-                                    }
-                                }
-                            } else if (bnext == 0xffffffffu) {
-                                bnext = bc;
-                            } else {
-                                stk[sp * STRIDE] = bc; sp++;
-                            }
-                        }
-                        if (bnext != 0xffffffffu) bn = bnext;
-                        else if (sp > sp_base) { sp--; bn = stk[sp * STRIDE]; }
-                        else bhave = false;
+    uint32_t cur = 0u;                       /* TLAS root node */
+    for (;;) {
+        if (cur == OA_BVH_EMPTY) {
+            if (sp == 0) break;
+            sp--;
+            cur = stk[sp * STRIDE];
+            continue;
+        }
+        if (!(cur & OA_BVH_LEAF_BIT)) {
+            cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
+            continue;
+        }
+        const uint32_t first = cur & 0x00ffffffu, count = (cur >> 24) & 0x7fu;
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t ii = P.tlas_items[first + k];
+            const float4* m = reinterpret_cast<const float4*>(P.inst + ii);
+            v3 oo, od;
+            inst_ray(m, o, d, oo, od);
+            const v3 inv2 = mk(1.f / od.x, 1.f / od.y, 1.f / od.z);
+            const int sp_base = sp;
+            uint32_t bc = __float_as_uint(m[2].w);     /* BLAS root node of this solid */
+            for (;;) {
+                if (bc == OA_BVH_EMPTY) {
+                    if (sp == sp_base) break;
+                    sp--;
+                    bc = stk[sp * STRIDE];
+                    continue;
+                }
+                if (!(bc & OA_BVH_LEAF_BIT)) {
+                    bc = visit<STRIDE>(P.blas, bc, oo, inv2, tmin, best_t, stk, sp);
+                    continue;
+                }
+                const uint32_t bf = bc & 0x00ffffffu, bn = (bc >> 24) & 0x7fu;
+                for (uint32_t j = 0; j < bn; ++j) {
+                    const uint32_t pi = P.blas_items[bf + j];
+                    const PrimRec pr = P.prim[pi];
+                    Isect is;
+                    if (!intersect_prim(is, P.node + 4u * pr.node_offset, P.itra, tmin, oo, od)) continue;
+                    const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
+                    if (is.w < best_t || (is.w == best_t && key < best_key)) {
+                        best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;
                     }
                 }
-            } else if (next == 0xffffffffu) {
-                next = c;
-            } else {
-                stk[sp * STRIDE] = c; sp++;
+                bc = OA_BVH_EMPTY;
             }
         }
-        if (next != 0xffffffffu) node = next;
-        else if (sp > 0) { sp--; node = stk[sp * STRIDE]; }
-        else have = false;
+        cur = OA_BVH_EMPTY;
     }
     if (best_inst == 0xffffffffu) {
         prd.normal = mk(0.f, 0.f, 0.f); prd.t = 0.f; prd.lposcost = 0.f;

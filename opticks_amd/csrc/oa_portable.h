@@ -163,15 +163,16 @@ OA_FN float oa_u32_to_uniform(uint32_t x)
 }
 
 /*
- * Per-photon stream. Registers only: no dynamic indexing of the cached block,
- * so the HIP compiler keeps it out of scratch memory.
+ * Per-photon stream. The unread words of the current block sit in b0..b3 as a
+ * FIFO (next word always in b0), so no word is ever selected by a run-time
+ * index: the HIP compiler keeps the whole stream in VGPRs. The next block is
+ * generated as soon as the current one is used up.
  */
 typedef struct {
-    uint32_t k0, k1;          /* key = seed                        */
-    uint32_t p0, p1;          /* photon index (counter words 2,3)  */
-    uint64_t pos;             /* absolute output-word position     */
-    uint32_t b0, b1, b2, b3;  /* cached block for pos/4            */
-    uint64_t cached_block;    /* block index held in b0..b3        */
+    uint64_t key;             /* seed: key words (lo, hi)                  */
+    uint64_t photon;          /* photon index: counter words 2,3 (lo, hi)  */
+    uint64_t pos;             /* absolute output-word position             */
+    uint32_t b0, b1, b2, b3;  /* block pos/4                               */
 } oa_philox_stream;
 
 OA_FN void oa_philox_fill(oa_philox_stream* s, uint64_t block)
@@ -179,11 +180,10 @@ OA_FN void oa_philox_fill(oa_philox_stream* s, uint64_t block)
     oa_u32x4 c;
     c.v[0] = (uint32_t)block;
     c.v[1] = (uint32_t)(block >> 32);
-    c.v[2] = s->p0;
-    c.v[3] = s->p1;
-    oa_u32x4 o = oa_philox4x32_10(c, s->k0, s->k1);
+    c.v[2] = (uint32_t)s->photon;
+    c.v[3] = (uint32_t)(s->photon >> 32);
+    oa_u32x4 o = oa_philox4x32_10(c, (uint32_t)s->key, (uint32_t)(s->key >> 32));
     s->b0 = o.v[0]; s->b1 = o.v[1]; s->b2 = o.v[2]; s->b3 = o.v[3];
-    s->cached_block = block;
 }
 
 /*
@@ -195,21 +195,22 @@ OA_FN void oa_rng_init(oa_philox_stream* s, uint64_t seed, uint64_t offset,
                        uint64_t skipahead_event_offset, uint64_t event_idx,
                        uint64_t photon)
 {
-    s->k0 = (uint32_t)seed;
-    s->k1 = (uint32_t)(seed >> 32);
-    s->p0 = (uint32_t)photon;
-    s->p1 = (uint32_t)(photon >> 32);
+    s->key = seed;
+    s->photon = photon;
     s->pos = offset + skipahead_event_offset * event_idx;
     oa_philox_fill(s, s->pos >> 2);
+    const uint32_t skip = (uint32_t)(s->pos & 3u);     /* start inside the block */
+    for (uint32_t i = 0; i < 3u; ++i) {
+        if (i < skip) { s->b0 = s->b1; s->b1 = s->b2; s->b2 = s->b3; }
+    }
 }
 
 OA_FN uint32_t oa_rng_next_u32(oa_philox_stream* s)
 {
-    uint64_t blk = s->pos >> 2;
-    if (blk != s->cached_block) oa_philox_fill(s, blk);
-    uint32_t w = (uint32_t)(s->pos & 3u);
-    uint32_t x = (w == 0u) ? s->b0 : (w == 1u) ? s->b1 : (w == 2u) ? s->b2 : s->b3;
+    uint32_t x = s->b0;
+    s->b0 = s->b1; s->b1 = s->b2; s->b2 = s->b3;
     s->pos += 1u;
+    if ((s->pos & 3u) == 0u) oa_philox_fill(s, s->pos >> 2);
     return x;
 }
 

@@ -70,8 +70,8 @@ class SEventConfig:
     max_slot: int = 0
     kernel: int = L.MI_KERNEL_WAVEFRONT
     block_size: int = 128
-    refill_period: int = 4
-    bvh_leaf_size: int = 2
+    refill_period: int = 1
+    bvh_leaf_size: int = 1
     waves_per_cu: int = 0
 
     ENV = dict(max_bounce=("OPTICKS_MAX_BOUNCE", int), max_time=("OPTICKS_MAX_TIME", float),
