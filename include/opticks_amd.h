@@ -19,6 +19,9 @@
  *   SSimulator::reset(eventID) (G4CXOpticks.cc:491-511)  mi_sim_reset
  *   CSGOptiX simtrace (CSGOptiX7.cu:563-604)            mi_sim_trace_device
  *   (no counterpart: batched events in one launch)      mi_sim_simulate_batch_device
+ *   QBnd upload of bnd (qudarap/QBnd.cc:130-175)         mi_sim_set_bnd (new values,
+ *                                                         same shape, no rebuild)
+ *   (no counterpart: differentiation)                   mi_sim_adjoint_device
  *
  * Array layouts are the reference's, so arrays exported by the reference
  * (CSGFoundry / SSim folders) can be passed unchanged:
@@ -209,6 +212,46 @@ int  mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst);
 
 /* Last launch timings in ms (HIP events): [seed, propagate, compact, total]. */
 int  mi_sim_last_timing(const mi_sim* sim, double* ms4);
+
+/* Replace the boundary table values (host array, the shape given at create). */
+int  mi_sim_set_bnd(mi_sim* sim, const float* bnd);
+
+/*
+ * Binned length parameters for the adjoint (no counterpart in the reference).
+ * theta is n_block blocks of [n_bin absorption lengths, n_bin Rayleigh lengths];
+ * the bnd table sample i of a material in block b holds theta[b][sample_bin[i]]
+ * (the caller writes those values with mi_sim_set_bnd).
+ * mat_block (host, n_mat+1): block of each 1-based material index, -1 = none.
+ * sample_bin (host, n_wl): bin of each wavelength sample, in [0, n_bin).
+ * n_det: detectors, hits carry sensor identifiers 1..n_det (others ignored).
+ */
+typedef struct mi_adjoint {
+    const int32_t* mat_block;
+    int32_t n_mat;
+    const int32_t* sample_bin;
+    int32_t n_bin;
+    int32_t n_block;
+    int32_t n_det;
+} mi_adjoint;
+
+/*
+ * Path-replay adjoint of one event simulated with mi_sim_simulate_device
+ * (same handle, config, bnd values, gensteps, event_id, n_photon and
+ * photon_offset). d_hit (n_hit,4,4) are that call's hits. Each hit photon is
+ * replayed from its RNG stream and its score
+ *   d log p(path)/d L = sum over segments ell/L^2 - [segment ends in the process]/L
+ * is spread over the bins of the two table samples each lookup interpolates.
+ * d_grad (n_det, n_block*2*n_bin) f32 receives d E[hits of detector d]/d theta,
+ * summed per detector in photon order with Kahan compensation.
+ * Device scratch: n_hit*n_param*4 bytes of score rows plus sort buffers.
+ * Synchronous on return; mi_sim_last_adjoint_timing gives [replay, reduce, total] ms.
+ */
+int  mi_sim_adjoint_device(mi_sim* sim, int32_t event_id,
+                           const void* d_gensteps, int64_t n_gs,
+                           int64_t n_photon, int64_t photon_offset,
+                           const void* d_hit, int64_t n_hit,
+                           const mi_adjoint* adj, void* d_grad, void* stream);
+int  mi_sim_last_adjoint_timing(const mi_sim* sim, double* ms3);
 
 #ifdef __cplusplus
 }

@@ -50,10 +50,16 @@ class MiEventOut(C.Structure):
                 ("seq", C.c_void_p), ("hit", C.c_void_p), ("device_ms", C.c_double)]
 
 
+class MiAdjoint(C.Structure):
+    _fields_ = [("mat_block", C.c_void_p), ("n_mat", C.c_int32), ("sample_bin", C.c_void_p),
+                ("n_bin", C.c_int32), ("n_block", C.c_int32), ("n_det", C.c_int32)]
+
+
 SYMBOLS = ["mi_config_default", "mi_last_error", "mi_version", "mi_sim_create", "mi_sim_destroy",
            "mi_sim_set_config", "mi_sim_get_config", "mi_sim_simulate", "mi_sim_reset",
            "mi_sim_simulate_device", "mi_sim_simulate_batch_device", "mi_sim_trace_device",
-           "mi_sim_refit_instances", "mi_sim_last_timing"]
+           "mi_sim_refit_instances", "mi_sim_last_timing", "mi_sim_set_bnd", "mi_sim_adjoint_device",
+           "mi_sim_last_adjoint_timing"]
 
 _lib = None
 
@@ -84,6 +90,10 @@ def lib():
     L.mi_sim_trace_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
     L.mi_sim_refit_instances.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.mi_sim_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.mi_sim_set_bnd.argtypes = [C.c_void_p, C.c_void_p]
+    L.mi_sim_adjoint_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
+                                        C.c_void_p, C.c_int64, C.POINTER(MiAdjoint), C.c_void_p, C.c_void_p]
+    L.mi_sim_last_adjoint_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     for name in SYMBOLS:
         if name not in ("mi_last_error", "mi_version", "mi_config_default", "mi_sim_destroy"):
             getattr(L, name).restype = C.c_int

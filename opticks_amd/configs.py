@@ -16,6 +16,8 @@ The benchmark workloads, built in code (the reference ships no data fixtures).
   config2  64 events x Poisson(200) scintillation gensteps x Poisson(5000)
            photons in the config1 geometry scaled to 20,000 detectors at r=19.4 m
            around a 17.7 m LS sphere.
+  config3  config1 with LS and water absorption/Rayleigh lengths as 64 bins over
+           200-800 nm each (256 parameters), each bin x(1+U(-0.2,0.2)) (seed 42).
 
 Choices the workload statements leave open (documented, fixed here):
   * world box half-side 6500 mm (config1) / 21000 mm (config2), inner surface absorbs;
@@ -120,10 +122,10 @@ def detector_tree():
     return Node.op(CSG_DIFFERENCE, Node.op(CSG_UNION, ell, neck), bore)
 
 
-def config0(num_photon=1_000_000, seed=42):
+def config0(num_photon=1_000_000, seed=42, water_absorption=20000.0, water_rayleigh=30000.0):
     lib = BoundaryLib()
     lib.add_material(Material("Vacuum", 1.0, 1e9, 1e9))
-    lib.add_material(Material("Water", 1.333, 20000.0, 30000.0))
+    lib.add_material(Material("Water", 1.333, water_absorption, water_rayleigh))
     lib.add_surface(Surface("detSurf", detect=1.0, absorb=0.0))
     b_world = lib.boundary("Vacuum/detSurf/detSurf/Vacuum")
     b_sphere = lib.boundary("Vacuum///Water")
@@ -200,6 +202,23 @@ def config2(num_events=64, mean_gs=200, mean_photons=5000, seed=42):
         events.append(gs)
     return Workload("config2", geo, phys, None, seed=seed, events=events,
                     meta=dict(ls_r=17700.0, det_r=19400.0, num_det=20000, world_half=21000.0))
+
+
+CONFIG3_BASE_LENGTHS = {"LS": (25000.0, 40000.0), "Water": (20000.0, 30000.0)}
+
+
+def config3(num_gs=1000, per_gs=10_000, seed=42, n_bin=64):
+    """config1 geometry and photons; LS and water absorption and Rayleigh lengths
+    as n_bin bins over 200-800 nm (theta, mm), each the config1 value times
+    (1 + U(-0.2, 0.2)) from numpy.random.default_rng(seed)."""
+    from .adjoint import LengthParams, perturbed_theta
+    w = config1(num_gs=num_gs, per_gs=per_gs, seed=seed)
+    params = LengthParams(w.physics, ("LS", "Water"), n_bin=n_bin, lo=200.0, hi=800.0)
+    theta = perturbed_theta(params, CONFIG3_BASE_LENGTHS, seed=seed, frac=0.2)
+    w.physics["bnd"] = params.apply(w.physics["bnd"], theta)
+    w.name = "config3"
+    w.meta.update(params=params, theta=theta)
+    return w
 
 
 def scaled_config1(num_gs, per_gs, seed=42):

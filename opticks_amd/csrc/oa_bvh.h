@@ -91,6 +91,26 @@ struct Builder {
                 if (cost < best_cost) { best_cost = cost; best_axis = ax; best_bin = s; }
             }
         }
+        /* candidate: the largest-area item alone (an enclosing world box binned
+           by its centroid would otherwise inflate one subtree to the whole scene) */
+        int big = b;
+        double big_area = -1.0;
+        for (int i = b; i < e; ++i) {
+            double a = area(v[i].lo, v[i].hi);
+            if (a > big_area) { big_area = a; big = i; }
+        }
+        {
+            double rlo[3] = { DBL_MAX, DBL_MAX, DBL_MAX }, rhi[3] = { -DBL_MAX, -DBL_MAX, -DBL_MAX };
+            for (int i = b; i < e; ++i) {
+                if (i == big) continue;
+                for (int k = 0; k < 3; ++k) { rlo[k] = std::min(rlo[k], v[i].lo[k]); rhi[k] = std::max(rhi[k], v[i].hi[k]); }
+            }
+            double cost = big_area + (e - b - 1) * area(rlo, rhi);
+            if (cost < best_cost) {
+                std::swap(v[b], v[big]);
+                return b + 1;
+            }
+        }
         int mid;
         if (best_axis < 0) {
             mid = b + (e - b) / 2;

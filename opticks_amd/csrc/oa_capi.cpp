@@ -112,6 +112,7 @@ struct mi_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
     double last_ms[4] = { 0, 0, 0, 0 };
+    double adj_ms[3] = { 0, 0, 0 };
 
     /* geometry */
     int64_t n_inst = 0, n_prim = 0, n_tlas = 0;
@@ -129,7 +130,7 @@ struct mi_sim {
     DBuf<float4> bnd;
     DBuf<int32_t> optical;
     DBuf<float> icdf;
-    int32_t n_wl = 0, n_icdf = 0, n_icdf_layer = 0, icdf_hd = 0;
+    int32_t n_bnd = 0, n_wl = 0, n_icdf = 0, n_icdf_layer = 0, icdf_hd = 0;
     float wl_low = 0.f, wl_step = 1.f;
     DBuf<float> precooked;
 
@@ -143,6 +144,12 @@ struct mi_sim {
     DBuf<uint64_t> seq;
     std::vector<float> h_photon, h_hit;
     std::vector<uint64_t> h_seq;
+    /* adjoint */
+    DBuf<int32_t> adj_mat_block, adj_sample_bin;
+    DBuf<float> adj_rows;
+    DBuf<uint32_t> adj_key, adj_val, adj_key2, adj_val2;
+    DBuf<unsigned char> adj_tmp;
+    DBuf<int64_t> adj_seg;
 };
 
 extern "C" {
@@ -342,6 +349,7 @@ static int upload_physics(mi_sim* sim, const mi_physics* ph)
     const size_t nb = (size_t)ph->n_bnd * 4 * 2 * ph->n_wl;
     HIPCHECK(upload(sim->bnd, (const float4*)ph->bnd, nb, s));
     HIPCHECK(upload(sim->optical, ph->optical, (size_t)ph->n_bnd * 16, s));
+    sim->n_bnd = ph->n_bnd;
     sim->n_wl = ph->n_wl;
     sim->wl_low = ph->wl_low;
     sim->wl_step = ph->wl_step;
@@ -581,8 +589,6 @@ int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps
         HIPCHECK(hipEventRecord(sim->ev[1], s));
         hits += nh;
     }
-    float seed_ms = 0.f;
-    (void)seed_ms;
     sim->last_ms[1] = prop_ms;
     sim->last_ms[2] = comp_ms;
     float tot = 0.f;
@@ -754,6 +760,94 @@ int mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst)
     HIPCHECK(oa_launch::refit(sim->o2w.p, sim->blas.p, sim->inst_blas_root.p, n_inst, sim->inst_box.p,
                               sim->tlas.p, sim->n_tlas, sim->tlas_depth, sim->tlas_items.p, s));
     HIPCHECK(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+int mi_sim_set_bnd(mi_sim* sim, const float* bnd)
+{
+    if (!sim || !bnd) return fail(MI_ERR_ARG, "null sim or bnd");
+    const size_t nb = (size_t)sim->n_bnd * 4 * 2 * sim->n_wl;
+    HIPCHECK(hipMemcpyAsync(sim->bnd.p, bnd, nb * sizeof(float4), hipMemcpyHostToDevice, sim->stream));
+    HIPCHECK(hipStreamSynchronize(sim->stream));
+    return MI_OK;
+}
+
+int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
+                          int64_t photon_offset, const void* d_hit, int64_t n_hit, const mi_adjoint* adj,
+                          void* d_grad, void* stream)
+{
+    if (!sim || !adj) return fail(MI_ERR_ARG, "null sim or adjoint description");
+    if (!d_grad) return fail(MI_ERR_ARG, "null gradient buffer");
+    if (n_gs <= 0 || !d_gensteps) return fail(MI_ERR_ARG, "the event's gensteps are required");
+    if (n_hit < 0 || (n_hit > 0 && !d_hit)) return fail(MI_ERR_ARG, "bad hit array");
+    if (n_hit > (int64_t)INT32_MAX) return fail(MI_ERR_CAPACITY, "more than 2^31-1 hits in one adjoint call");
+    if (adj->n_bin <= 0 || adj->n_block <= 0 || adj->n_det <= 0 || adj->n_det > 0xffff || adj->n_mat < 0)
+        return fail(MI_ERR_ARG, "adjoint: n_bin, n_block must be > 0, n_det in [1, 65535], n_mat >= 0");
+    if (!adj->mat_block || !adj->sample_bin) return fail(MI_ERR_ARG, "adjoint: null mat_block or sample_bin");
+    for (int32_t m = 0; m <= adj->n_mat; ++m)
+        if (adj->mat_block[m] < -1 || adj->mat_block[m] >= adj->n_block)
+            return fail(MI_ERR_ARG, "adjoint: mat_block entry out of [-1, n_block)");
+    for (int32_t i = 0; i < sim->n_wl; ++i)
+        if (adj->sample_bin[i] < 0 || adj->sample_bin[i] >= adj->n_bin)
+            return fail(MI_ERR_ARG, "adjoint: sample_bin entry out of [0, n_bin)");
+    const int64_t n_param64 = (int64_t)adj->n_block * 2 * adj->n_bin;
+    if (n_param64 > (1 << 20)) return fail(MI_ERR_ARG, "adjoint: too many parameters");
+    const int32_t n_param = (int32_t)n_param64;
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    HIPCHECK(upload(sim->adj_mat_block, adj->mat_block, (size_t)adj->n_mat + 1, s));
+    HIPCHECK(upload(sim->adj_sample_bin, adj->sample_bin, (size_t)sim->n_wl, s));
+    HIPCHECK(sim->prefix.reserve((size_t)n_gs + 1));
+    HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, sim->prefix.p, s));
+    if (n_photon < 0) {
+        HIPCHECK(hipMemcpyAsync(&n_photon, sim->prefix.p + n_gs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    int64_t evh[4] = { 0, n_gs, 0, n_photon };
+    int32_t evid = event_id;
+    HIPCHECK(sim->evbuf.reserve(4));
+    HIPCHECK(sim->ev_id.reserve(1));
+    HIPCHECK(hipMemcpyAsync(sim->evbuf.p, evh, sizeof(evh), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(sim->ev_id.p, &evid, sizeof(evid), hipMemcpyHostToDevice, s));
+    SimParams P = base_params(sim);
+    P.gensteps = (const float*)d_gensteps;
+    P.gs_prefix = sim->prefix.p;
+    P.n_gs = n_gs;
+    P.n_event = 1;
+    P.ev_gs_begin = sim->evbuf.p;
+    P.ev_ph_begin = sim->evbuf.p + 2;
+    P.ev_id = sim->ev_id.p;
+    P.n_photon = n_photon;
+    P.photon_offset = photon_offset;
+    const uint32_t n_det = (uint32_t)adj->n_det;
+    const size_t tmp_bytes = std::max<size_t>(1, oa_launch::adjoint_sort_bytes(n_hit, n_det));
+    if (sim->adj_rows.reserve((size_t)(n_hit * n_param)) != hipSuccess)
+        return fail(MI_ERR_CAPACITY, "adjoint: cannot allocate " + std::to_string(n_hit * n_param * 4) + " bytes of score rows");
+    HIPCHECK(sim->adj_key.reserve((size_t)n_hit));
+    HIPCHECK(sim->adj_val.reserve((size_t)n_hit));
+    HIPCHECK(sim->adj_key2.reserve((size_t)n_hit));
+    HIPCHECK(sim->adj_val2.reserve((size_t)n_hit));
+    HIPCHECK(sim->adj_tmp.reserve(tmp_bytes));
+    HIPCHECK(sim->adj_seg.reserve((size_t)n_det + 1));
+    HIPCHECK(hipEventRecord(sim->ev[0], s));
+    HIPCHECK(oa_launch::adjoint_replay(P, sim->adj_mat_block.p, adj->n_mat, sim->adj_sample_bin.p, adj->n_bin, n_param,
+                                       (const float4*)d_hit, n_hit, sim->adj_rows.p, s));
+    HIPCHECK(hipEventRecord(sim->ev[1], s));
+    HIPCHECK(oa_launch::adjoint_reduce((const float4*)d_hit, n_hit, n_det, n_param, sim->adj_rows.p, sim->adj_key.p,
+                                       sim->adj_val.p, sim->adj_key2.p, sim->adj_val2.p, sim->adj_tmp.p, tmp_bytes,
+                                       sim->adj_seg.p, (float*)d_grad, s));
+    HIPCHECK(hipEventRecord(sim->ev[2], s));
+    HIPCHECK(hipEventSynchronize(sim->ev[2]));
+    float a = 0.f, b = 0.f;
+    HIPCHECK(hipEventElapsedTime(&a, sim->ev[0], sim->ev[1]));
+    HIPCHECK(hipEventElapsedTime(&b, sim->ev[1], sim->ev[2]));
+    sim->adj_ms[0] = a; sim->adj_ms[1] = b; sim->adj_ms[2] = (double)a + b;
+    return MI_OK;
+}
+
+int mi_sim_last_adjoint_timing(const mi_sim* sim, double* ms3)
+{
+    if (!sim || !ms3) return fail(MI_ERR_ARG, "null argument");
+    for (int i = 0; i < 3; ++i) ms3[i] = sim->adj_ms[i];
     return MI_OK;
 }
 

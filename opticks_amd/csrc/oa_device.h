@@ -23,6 +23,15 @@
 
 #define OA_D __device__ static inline __attribute__((always_inline))
 
+#ifdef OA_STATS
+__device__ unsigned long long oa_stats[OA_ST_N];
+#define OA_COUNT(i) atomicAdd(&oa_stats[(i)], 1ull)
+#define OA_COUNT_N(i, n) atomicAdd(&oa_stats[(i)], (unsigned long long)(n))
+#else
+#define OA_COUNT(i) ((void)0)
+#define OA_COUNT_N(i, n) ((void)0)
+#endif
+
 namespace oa {
 
 /* ----------------------------------------------------------- vectors */
@@ -382,6 +391,7 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, f
             if (typecode == TC_ZERO) { nodeIdx = postorder_next(nodeIdx, elevation); continue; }
             if (typecode >= TC_NODE) {
                 Isect e;
+                OA_COUNT(OA_ST_LEAF);
                 intersect_leaf(e, nd, itra, tmin, ro, rd);
                 e.w = copysignf(e.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
                 if (sn >= D) { ierr = true; break; }
@@ -463,6 +473,7 @@ OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, floa
     if (typecode >= TC_LEAF) return intersect_leaf(is, root, itra, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     if (typecode < TC_NODE) {
+        OA_COUNT(OA_ST_TREE);
         if (__float_as_uint(root[0].x) <= 15u) return intersect_tree_t<4>(is, root, itra, t_min, ro, rd);
         const float4 r = intersect_tree_deep(root, itra, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
         is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;

@@ -119,8 +119,8 @@ OA_D void lane_start(const SimParams& P, Lane& L, int64_t g)
 }
 
 /* one iteration of the CSGOptiX7.cu:477-509 loop; returns true when the photon is done */
-template <int STRIDE>
-OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk)
+template <int STRIDE, class Tape>
+OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk, Tape& tape)
 {
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return true;
     const float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
@@ -128,7 +128,8 @@ OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk)
     trace<STRIDE>(P, stk, L.p.pos, L.p.mom, tmin, P.tmax, prd);
     if (prd.boundary == 0xffffu) return true;
     prd.normal = normalize(prd.normal);
-    const int command = propagate(P, L.rng, L.p, prd);
+    OA_COUNT(OA_ST_STEP);
+    const int command = propagate(P, L.rng, L.p, prd, tape);
     L.bounce++;
     seq_add(L, (unsigned)L.bounce, ph_flag(L.p), ph_boundary(L.p));
     if (command == CMD_BREAK) return true;
@@ -164,8 +165,9 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
     bool hit = false;
     if (g < P.n_photon) {
         Lane L;
+        NoTape tape;
         lane_start(P, L, g);
-        while (!lane_step<BLOCK>(P, L, stk)) {}
+        while (!lane_step<BLOCK>(P, L, stk, tape)) {}
         lane_finish(P, L);
         hit = (L.p.flagmask & P.hit_mask) != 0u;
     }
@@ -182,6 +184,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
     uint32_t* stk = s_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
     Lane L;
+    NoTape tape;
     bool active = false;
     bool exhausted = false;
     const int period = P.refill_period > 0 ? P.refill_period : 1;
@@ -204,9 +207,13 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
             }
             if ((int64_t)(base + (unsigned long long)n_idle) >= P.n_photon) exhausted = true;
         }
-        if (__ballot(active) == 0ull) break;
+        const unsigned long long act = __ballot(active);
+        if (act == 0ull) break;
+#ifdef OA_STATS
+        if (lane == 0) { OA_COUNT(OA_ST_ITER); OA_COUNT_N(OA_ST_ACTIVE, __popcll(act)); }
+#endif
         if (active) {
-            if (lane_step<BLOCK>(P, L, stk)) {
+            if (lane_step<BLOCK>(P, L, stk, tape)) {
                 lane_finish(P, L);
                 if ((L.p.flagmask & P.hit_mask) != 0u)
                     atomicOr(P.hitwords + (L.g >> 6), 1ull << (L.g & 63));
@@ -378,6 +385,22 @@ __global__ void k_refit_level(BvhNode* nodes, int64_t n_nodes, uint32_t depth, c
 
 namespace oa_launch {
 
+/* counters of an OA_STATS build (zeros otherwise) */
+extern "C" int oa_stats_read(unsigned long long* out, int reset)
+{
+#ifdef OA_STATS
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(oa_stats), sizeof(unsigned long long) * OA_ST_N) != hipSuccess) return -1;
+    if (reset) {
+        unsigned long long z[OA_ST_N] = {};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(oa_stats), z, sizeof(z)) != hipSuccess) return -1;
+    }
+#else
+    for (int i = 0; i < OA_ST_N; ++i) out[i] = 0ull;
+    (void)reset;
+#endif
+    return 0;
+}
+
 hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gs_prefix, dim3(1), dim3(1024), 0, s, d_gs, n_gs, d_prefix);
@@ -476,6 +499,172 @@ hipError_t refit(const float* d_o2w, const BvhNode* d_blas, const uint32_t* d_in
         hipLaunchKernelGGL(k_refit_level, dim3((unsigned)((n_tlas + 255) / 256)), dim3(256), 0, s,
                            d_tlas, n_tlas, (uint32_t)d, d_tlas_items, d_inst_box);
     return hipGetLastError();
+}
+
+} // namespace oa_launch
+
+/* ============================================================ adjoint
+ *
+ * Gradient of expected hits per detector with respect to binned absorption
+ * and Rayleigh lengths (no counterpart in the reference). Score-function
+ * (likelihood-ratio) estimator: for a detected photon
+ *   d/dL log p(path) = sum over segments in the material of
+ *       ell/L^2   (survival over the segment length ell)
+ *     - 1/L       (on the segment that ends with that process: absorption for
+ *                  L_abs -- reemitted photons -- or scattering for L_Rayleigh)
+ * chained through the FP32 lerp of the table samples (weights 1-a, a) into the
+ * parameter bins. Paths are not stored: each hit photon is replayed from its
+ * Philox stream (same code, same bits), and its score row is written to S.
+ * Rows are then reduced per detector in hit order with Kahan summation.
+ */
+
+struct AdjParams {
+    const int32_t* mat_block;   /* (n_mat+1) parameter block per 1-based material index, -1 = none */
+    int32_t n_mat;
+    const int32_t* sample_bin;  /* (n_wl) parameter bin of each table sample */
+    int32_t n_bin;              /* bins per property */
+    int32_t n_param;            /* blocks * 2 * n_bin */
+};
+
+struct ScoreTape {
+    const int32_t* optical;
+    int32_t n_wl;
+    float wl_low, wl_step;
+    AdjParams A;
+    float* row;
+    __device__ inline __attribute__((always_inline)) void seg(int m1_line, float wl, float La, float Ls, float ell, int ev)
+    {
+        const int mat = optical[4 * m1_line];
+        if (mat < 0 || mat > A.n_mat) return;
+        const int blk = A.mat_block[mat];
+        if (blk < 0) return;
+        float fx = (wl - wl_low) / wl_step;
+        float fl = floorf(fx);
+        float a = fx - fl;
+        int i0 = (int)fl;
+        i0 = ((i0 % n_wl) + n_wl) % n_wl;
+        int i1 = (i0 + 1) % n_wl;
+        const int b0 = A.sample_bin[i0], b1 = A.sample_bin[i1];
+        const float sa = ell / (La * La) - (ev == SEG_ABSORB ? 1.f / La : 0.f);
+        const float ss = ell / (Ls * Ls) - (ev == SEG_SCATTER ? 1.f / Ls : 0.f);
+        float* r = row + blk * 2 * A.n_bin;
+        r[b0] += sa * (1.f - a);
+        r[b1] += sa * a;
+        r[A.n_bin + b0] += ss * (1.f - a);
+        r[A.n_bin + b1] += ss * a;
+    }
+};
+
+/* one lane replays one hit photon (single-event launches) */
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint(SimParams P, AdjParams A, const float4* hits, int64_t n_hit,
+                                                          float* S)
+{
+    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
+    const int64_t h = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (h >= n_hit) return;
+    const float4 q3 = hits[4 * h + 3];
+    const uint32_t ident = __float_as_uint(q3.y), index = __float_as_uint(q3.z);
+    const int64_t photon_idx = ((int64_t)(ident >> 24) << 32) | (int64_t)index;
+    const int64_t g = photon_idx - P.photon_offset;
+    if (g < 0 || g >= P.n_photon) return;
+    ScoreTape tape;
+    tape.optical = P.optical; tape.n_wl = P.n_wl; tape.wl_low = P.wl_low; tape.wl_step = P.wl_step;
+    tape.A = A;
+    tape.row = S + h * (int64_t)A.n_param;
+    Lane L;
+    lane_start(P, L, g);
+    while (!lane_step<BLOCK>(P, L, s_stack + threadIdx.x, tape)) {}
+}
+
+__global__ void k_zero_f32(float* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+/* detector key (sensor_identifier) of each hit; non-sensors get n_det */
+__global__ void k_hit_det(const float4* hits, int64_t n_hit, uint32_t n_det, uint32_t* key, uint32_t* val)
+{
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n_hit) return;
+    const uint32_t id = __float_as_uint(hits[4 * h + 3].y) & 0x00ffffffu;
+    key[h] = (id >= 1u && id <= n_det) ? id - 1u : n_det;
+    val[h] = (uint32_t)h;
+}
+
+/* start of each detector's run in the sorted keys */
+__global__ void k_det_segments(const uint32_t* key, int64_t n, uint32_t n_det, int64_t* seg)
+{
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d > (int64_t)n_det) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) { int64_t mid = (lo + hi) >> 1; if (key[mid] < (uint32_t)d) lo = mid + 1; else hi = mid; }
+    seg[d] = lo;
+}
+
+/* G[d,k] = Kahan sum over the detector's hits (in photon order) of S[h,k] */
+__global__ void __launch_bounds__(256) k_det_reduce(const float* S, const uint32_t* order, const int64_t* seg,
+                                                    int32_t n_param, float* G)
+{
+    const int d = blockIdx.x;
+    const int64_t b = seg[d], e = seg[d + 1];
+    for (int k = threadIdx.x; k < n_param; k += blockDim.x) {
+        float sum = 0.f, c = 0.f;
+        for (int64_t i = b; i < e; ++i) {
+            const float y = S[(int64_t)order[i] * n_param + k] - c;
+            const float t = sum + y;
+            c = (t - sum) - y;
+            sum = t;
+        }
+        G[(int64_t)d * n_param + k] = sum;
+    }
+}
+
+#include <hipcub/hipcub.hpp>
+
+namespace oa_launch {
+
+hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_t n_mat, const int32_t* d_sample_bin,
+                          int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t n_hit, float* d_S, hipStream_t s)
+{
+    AdjParams A;
+    A.mat_block = d_mat_block; A.n_mat = n_mat; A.sample_bin = d_sample_bin; A.n_bin = n_bin; A.n_param = n_param;
+    const int64_t ns = n_hit * (int64_t)n_param;
+    if (ns > 0) hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, d_S, ns);
+    if (n_hit > 0)
+        hipLaunchKernelGGL(k_adjoint<128>, dim3((unsigned)((n_hit + 127) / 128)), dim3(128), 0, s, P, A, d_hit, n_hit, d_S);
+    return hipGetLastError();
+}
+
+static int det_key_bits(uint32_t n_det)
+{
+    int end_bit = 1;
+    while (end_bit < 32 && (1u << end_bit) <= n_det) ++end_bit;
+    return end_bit;
+}
+
+hipError_t adjoint_reduce(const float4* d_hit, int64_t n_hit, uint32_t n_det, int32_t n_param, const float* d_S,
+                          uint32_t* d_key, uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp,
+                          size_t tmp_bytes, int64_t* d_seg, float* d_G, hipStream_t s)
+{
+    if (n_hit > 0) {
+        hipLaunchKernelGGL(k_hit_det, dim3((unsigned)((n_hit + 255) / 256)), dim3(256), 0, s, d_hit, n_hit, n_det, d_key, d_val);
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_key, d_key2, d_val, d_val2,
+                                                          (int)n_hit, 0, det_key_bits(n_det), s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_det_segments, dim3((n_det + 1 + 255) / 256), dim3(256), 0, s, d_key2, n_hit, n_det, d_seg);
+    hipLaunchKernelGGL(k_det_reduce, dim3(n_det), dim3(256), 0, s, d_S, d_val2, d_seg, n_param, d_G);
+    return hipGetLastError();
+}
+
+size_t adjoint_sort_bytes(int64_t n_hit, uint32_t n_det)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n_hit, 0, det_key_bits(n_det));
+    return bytes;
 }
 
 } // namespace oa_launch

@@ -23,6 +23,13 @@
 #define OA_STACK_DEPTH  32
 #define OA_CSG_MAX_HEIGHT 7
 
+/* Optional work counters (build with -DOA_STATS): traces, TLAS node visits,
+   instance tests, BLAS node visits, prim tests, tree prim tests, leaf
+   evaluations inside trees, propagation steps, wave loop iterations, active
+   lanes summed over wave loop iterations. Off in normal builds. */
+enum { OA_ST_TRACE, OA_ST_TLAS, OA_ST_INST, OA_ST_BLAS, OA_ST_PRIM, OA_ST_TREE, OA_ST_LEAF, OA_ST_STEP,
+       OA_ST_ITER, OA_ST_ACTIVE, OA_ST_N };
+
 struct BvhNode {
     float lo0[3], hi0[3];
     float lo1[3], hi1[3];

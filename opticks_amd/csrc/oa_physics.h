@@ -43,6 +43,15 @@ struct State {
     float  n2;
     float4 surface;       /* detect, absorb, specular, diffuse */
     int    ems;
+    int    m1_line;       /* bnd line of the material the photon travels in */
+};
+
+/* Segment events reported to a Tape during propagate_to_boundary. */
+enum : int { SEG_SAIL = 0, SEG_ABSORB = 1, SEG_SCATTER = 2 };
+
+/* The forward kernels record nothing. */
+struct NoTape {
+    __device__ inline __attribute__((always_inline)) void seg(int, float, float, float, float, int) {}
 };
 
 OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float cosTheta)
@@ -51,6 +60,7 @@ OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float 
     const int m1_line = cosTheta > 0.f ? line + 3 : line + 0;
     const int m2_line = cosTheta > 0.f ? line + 0 : line + 3;
     const int su_line = cosTheta > 0.f ? line + 2 : line + 1;
+    s.m1_line = m1_line;
     s.material1 = bnd_lookup(P, wl, m1_line, 0);
     s.group_velocity = bnd_lookup(P, wl, m1_line, 1).x;
     s.n2 = bnd_lookup(P, wl, m2_line, 0).x;
@@ -176,7 +186,8 @@ OA_D void rayleigh(Photon& p, Rng& rng)
 
 /* ----------------------------------------------------------- physics */
 
-OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, const State& s, float dist)
+template <class Tape>
+OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, const State& s, float dist, Tape& tape)
 {
     const float absorption_length = s.material1.y;
     const float scattering_length = s.material1.z;
@@ -188,6 +199,7 @@ OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, co
     const float absorption_distance = -absorption_length * oa_logf(u_absorption);
     if (absorption_distance <= scattering_distance) {
         if (absorption_distance <= dist) {
+            tape.seg(s.m1_line, p.wavelength, absorption_length, scattering_length, absorption_distance, SEG_ABSORB);
             p.time += absorption_distance / gv;
             p.pos = p.pos + absorption_distance * p.mom;
             float u_reemit = reemission_prob == 0.f ? 2.f : uni(rng);
@@ -203,12 +215,14 @@ OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, co
             return CMD_BREAK;
         }
     } else if (scattering_distance <= dist) {
+        tape.seg(s.m1_line, p.wavelength, absorption_length, scattering_length, scattering_distance, SEG_SCATTER);
         p.time += scattering_distance / gv;
         p.pos = p.pos + scattering_distance * p.mom;
         rayleigh(p, rng);
         flag = F_BULK_SCATTER;
         return CMD_CONTINUE;
     }
+    tape.seg(s.m1_line, p.wavelength, absorption_length, scattering_length, dist, SEG_SAIL);
     p.pos = p.pos + dist * p.mom;
     p.time += dist / gv;
     return CMD_BOUNDARY;
@@ -280,7 +294,8 @@ OA_D int at_surface(uint32_t& flag, Rng& rng, Photon& p, const State& s, v3 norm
     return action;
 }
 
-OA_D int propagate(const SimParams& P, Rng& rng, Photon& p, const Prd& prd)
+template <class Tape>
+OA_D int propagate(const SimParams& P, Rng& rng, Photon& p, const Prd& prd, Tape& tape)
 {
     const uint32_t boundary = prd.boundary;
     const uint32_t identity = prd.iindex_identity & 0xffffu;
@@ -295,7 +310,7 @@ OA_D int propagate(const SimParams& P, Rng& rng, Photon& p, const Prd& prd)
     State s;
     fill(P, s, boundary, p.wavelength, cosTheta);
     uint32_t flag = 0u;
-    int command = to_boundary(P, flag, rng, p, s, prd.t);
+    int command = to_boundary(P, flag, rng, p, s, prd.t, tape);
     if (command == CMD_BOUNDARY) {
         if (s.ems == EMS_NOSURFACE) command = at_boundary(flag, rng, p, s.material1.x, s.n2, normal);
         else if (s.ems == EMS_SURFACE) command = at_surface(flag, rng, p, s, normal);
@@ -498,10 +513,17 @@ OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
  * Nearest hit over instances x prims with t in (tmin, tmax); exact ties go to
  * the smaller (instance, prim) key so the answer does not depend on traversal
  * order. stk: this lane's LDS stack, entry k at stk[k*STRIDE].
+ *
+ * "while-while" traversal (Aila and Laine, "Understanding the efficiency of
+ * ray traversal on GPUs", HPG 2009): lanes walk TLAS nodes until each holds
+ * an instance leaf or is finished, then all pending leaves (instance ray
+ * transform, BLAS walk, CSG prim tests) run together, so the expensive CSG
+ * code executes with most lanes of the wave active instead of once per lane.
  */
 template <int STRIDE>
 OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float tmax, Prd& prd)
 {
+    OA_COUNT(OA_ST_TRACE);
     const v3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
     float best_t = tmax;
     uint64_t best_key = ~0ull;
@@ -509,20 +531,30 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     Isect best_is; best_is.x = best_is.y = best_is.z = best_is.w = 0.f;
     int sp = 0;
     uint32_t cur = 0u;                       /* TLAS root node */
-    for (;;) {
-        if (cur == OA_BVH_EMPTY) {
-            if (sp == 0) break;
-            sp--;
-            cur = stk[sp * STRIDE];
-            continue;
+    uint32_t leaf = OA_BVH_EMPTY;
+    bool done = false;
+    while (!done) {
+        /* phase 1: walk nodes until this lane has a leaf or nothing left */
+        while (!done && leaf == OA_BVH_EMPTY) {
+            if (cur == OA_BVH_EMPTY) {
+                if (sp == 0) { done = true; break; }
+                sp--;
+                cur = stk[sp * STRIDE];
+            } else if (cur & OA_BVH_LEAF_BIT) {
+                leaf = cur;
+                cur = OA_BVH_EMPTY;
+            } else {
+                OA_COUNT(OA_ST_TLAS);
+                cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
+            }
         }
-        if (!(cur & OA_BVH_LEAF_BIT)) {
-            cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
-            continue;
-        }
-        const uint32_t first = cur & 0x00ffffffu, count = (cur >> 24) & 0x7fu;
+        if (leaf == OA_BVH_EMPTY) continue;
+        /* phase 2: the pending instance leaf */
+        const uint32_t first = leaf & 0x00ffffffu, count = (leaf >> 24) & 0x7fu;
+        leaf = OA_BVH_EMPTY;
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t ii = P.tlas_items[first + k];
+            OA_COUNT(OA_ST_INST);
             const float4* m = reinterpret_cast<const float4*>(P.inst + ii);
             v3 oo, od;
             inst_ray(m, o, d, oo, od);
@@ -537,6 +569,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                     continue;
                 }
                 if (!(bc & OA_BVH_LEAF_BIT)) {
+                    OA_COUNT(OA_ST_BLAS);
                     bc = visit<STRIDE>(P.blas, bc, oo, inv2, tmin, best_t, stk, sp);
                     continue;
                 }
@@ -545,6 +578,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                     const uint32_t pi = P.blas_items[bf + j];
                     const PrimRec pr = P.prim[pi];
                     Isect is;
+                    OA_COUNT(OA_ST_PRIM);
                     if (!intersect_prim(is, P.node + 4u * pr.node_offset, P.itra, tmin, oo, od)) continue;
                     const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
                     if (is.w < best_t || (is.w == best_t && key < best_key)) {
@@ -554,7 +588,6 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 bc = OA_BVH_EMPTY;
             }
         }
-        cur = OA_BVH_EMPTY;
     }
     if (best_inst == 0xffffffffu) {
         prd.normal = mk(0.f, 0.f, 0.f); prd.t = 0.f; prd.lposcost = 0.f;

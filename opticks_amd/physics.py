@@ -117,7 +117,8 @@ class BoundaryLib:
                 else:
                     bnd[b, j] = -1.0
                     optical[b, j] = (0, EMS_NOSURFACE, 0, 0)
-        return dict(bnd=bnd, optical=optical, wl_low=WL_LOW, wl_step=WL_STEP, n_wl=N_WL)
+        return dict(bnd=bnd, optical=optical, wl_low=WL_LOW, wl_step=WL_STEP, n_wl=N_WL,
+                    material_names=list(self.mat_order))
 
 
 def gaussian_icdf(mean=430.0, sigma=20.0, lo=200.0, hi=800.0, n=4096, hd_factor=20):

@@ -234,6 +234,37 @@ class QSim:
         L.check(rc, "mi_sim_simulate_batch_device")
         return po, ho
 
+    def set_bnd(self, bnd):
+        """Replace the boundary table values (same shape as at create)."""
+        bnd = np.ascontiguousarray(bnd, np.float32)
+        if bnd.shape != self._bnd.shape:
+            raise ValueError(f"bnd shape {bnd.shape} != {self._bnd.shape}")
+        L.check(L.lib().mi_sim_set_bnd(self.handle, bnd.ctypes.data), "mi_sim_set_bnd")
+        self._bnd_current = bnd
+
+    @property
+    def base_bnd(self):
+        """The boundary table given at create."""
+        return self._bnd
+
+    def adjoint_device(self, gs_t, hit_t, n_hit, adj, grad_t, eventID=0, n_photon=-1, photon_offset=0,
+                       stream=None):
+        """d E[hits per detector]/d theta into grad_t (n_det, n_param) f32 on the GPU.
+
+        adj: MiAdjoint (see opticks_amd.adjoint.LengthParams.c_struct). The event must have
+        been simulated with the same gensteps, eventID, n_photon, photon_offset and bnd.
+        """
+        rc = L.lib().mi_sim_adjoint_device(
+            self.handle, eventID, gs_t.data_ptr(), gs_t.shape[0], n_photon, photon_offset,
+            None if n_hit == 0 else hit_t.data_ptr(), int(n_hit), C.byref(adj), grad_t.data_ptr(),
+            None if stream is None else C.c_void_p(stream.cuda_stream))
+        L.check(rc, "mi_sim_adjoint_device")
+
+    def last_adjoint_timing(self):
+        t = (C.c_double * 3)()
+        L.check(L.lib().mi_sim_last_adjoint_timing(self.handle, t), "mi_sim_last_adjoint_timing")
+        return dict(replay_ms=t[0], reduce_ms=t[1], total_ms=t[2])
+
     def trace_device(self, rays_t, isect_t):
         L.check(L.lib().mi_sim_trace_device(self.handle, rays_t.data_ptr(), rays_t.shape[0], isect_t.data_ptr(),
                                             None), "mi_sim_trace_device")

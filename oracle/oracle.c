@@ -798,7 +798,53 @@ typedef struct {
     f4 material1, m1group2, material2, surface;
     int32_t optical[4];
     int32_t index[4];
+    int m1_line;
 } ostate;
+
+/*
+ * Adjoint score accumulator (no counterpart in the reference). For the
+ * segment of a photon inside material m with absorption length La and
+ * Rayleigh length Ls the path-probability factor is exp(-ell/La) exp(-ell/Ls),
+ * times 1/La when the segment ends in absorption (reemission) or 1/Ls when it
+ * ends in a Rayleigh scatter. Its log-derivatives
+ *   d/dLa = ell/La^2 - [absorbed]/La,   d/dLs = ell/Ls^2 - [scattered]/Ls
+ * are spread onto the parameter bins of the two table samples the FP32 lerp
+ * of bnd_lookup reads, with weights (1-a) and a. row layout per parameter
+ * block: [n_bin absorption bins][n_bin Rayleigh bins].
+ */
+typedef struct {
+    const int32_t* mat_block;   /* (n_mat+1) block of each 1-based material index, <0 none */
+    int32_t n_mat;
+    const int32_t* sample_bin;  /* (n_wl) bin of each wavelength sample */
+    int32_t n_bin;
+    float* row;                 /* (n_block*2*n_bin) accumulated in path order */
+} oscore;
+
+enum { SEG_SAIL = 0, SEG_ABSORB = 1, SEG_SCATTER = 2 };
+
+static void score_seg(oscore* sc, const oracle_phys* ph, int m1_line, float wl, float La, float Ls, float ell, int ev)
+{
+    if (!sc) return;
+    int mat = ph->optical[4 * m1_line];
+    if (mat < 0 || mat > sc->n_mat) return;
+    int blk = sc->mat_block[mat];
+    if (blk < 0) return;
+    int nwl = ph->n_wl;
+    float fx = (wl - ph->wl_low) / ph->wl_step;
+    float fl = floorf(fx);
+    float a = fx - fl;
+    int i0 = (int)fl;
+    i0 = ((i0 % nwl) + nwl) % nwl;
+    int i1 = (i0 + 1) % nwl;
+    int b0 = sc->sample_bin[i0], b1 = sc->sample_bin[i1];
+    float sa = ell / (La * La) - (ev == SEG_ABSORB ? 1.f / La : 0.f);
+    float ss = ell / (Ls * Ls) - (ev == SEG_SCATTER ? 1.f / Ls : 0.f);
+    float* r = sc->row + blk * 2 * sc->n_bin;
+    r[b0] += sa * (1.f - a);
+    r[b1] += sa * a;
+    r[sc->n_bin + b0] += ss * (1.f - a);
+    r[sc->n_bin + b1] += ss * a;
+}
 
 /* Linear interpolation of one bnd "texture" row at fractional texel fx,
    wrap addressing (qudarap/QTex.cc:293-305, qudarap/qbnd.h:108-133). */
@@ -835,6 +881,7 @@ static void fill_state(const oracle_phys* ph, ostate* s, uint32_t boundary, floa
     s->index[1] = ph->optical[4 * m2_line];
     s->index[2] = ph->optical[4 * su_line];
     s->index[3] = 0;
+    s->m1_line = m1_line;
 }
 
 /* scintillation ICDF "texture": qscint::wavelength_hd0/10/20 (qudarap/qscint.h:160-230),
@@ -973,7 +1020,7 @@ static void rayleigh_scatter(oph* p, orng* rng)
 
 /* qsim::propagate_to_boundary (qudarap/qsim.h:721-866) */
 static int propagate_to_boundary(uint32_t* flag, orng* rng, oph* p, const ostate* s, float distance_to_boundary,
-                                 const oracle_phys* ph)
+                                 const oracle_phys* ph, oscore* sc)
 {
     float absorption_length = s->material1.y;
     float scattering_length = s->material1.z;
@@ -985,6 +1032,8 @@ static int propagate_to_boundary(uint32_t* flag, orng* rng, oph* p, const ostate
     float absorption_distance = -absorption_length * oa_logf(u_absorption);
     if (absorption_distance <= scattering_distance) {
         if (absorption_distance <= distance_to_boundary) {
+            score_seg(sc, ph, s->m1_line, p->wavelength, absorption_length, scattering_length, absorption_distance,
+                      SEG_ABSORB);
             p->time += absorption_distance / group_velocity;
             p->pos = add3(p->pos, smul3(absorption_distance, p->mom));
             float u_reemit = reemission_prob == 0.f ? 2.f : ou(rng);
@@ -1006,6 +1055,8 @@ static int propagate_to_boundary(uint32_t* flag, orng* rng, oph* p, const ostate
         }
     } else {
         if (scattering_distance <= distance_to_boundary) {
+            score_seg(sc, ph, s->m1_line, p->wavelength, absorption_length, scattering_length, scattering_distance,
+                      SEG_SCATTER);
             p->time += scattering_distance / group_velocity;
             p->pos = add3(p->pos, smul3(scattering_distance, p->mom));
             rayleigh_scatter(p, rng);
@@ -1013,6 +1064,7 @@ static int propagate_to_boundary(uint32_t* flag, orng* rng, oph* p, const ostate
             return C_CONTINUE;
         }
     }
+    score_seg(sc, ph, s->m1_line, p->wavelength, absorption_length, scattering_length, distance_to_boundary, SEG_SAIL);
     p->pos = add3(p->pos, smul3(distance_to_boundary, p->mom));
     p->time += distance_to_boundary / group_velocity;
     return C_BOUNDARY;
@@ -1110,7 +1162,7 @@ static int propagate_at_surface(uint32_t* flag, orng* rng, oph* p, const ostate*
 }
 
 /* qsim::propagate (qudarap/qsim.h:2229-2338) */
-static int propagate(orng* rng, oph* p, const oprd* prd, const oracle_phys* ph)
+static int propagate(orng* rng, oph* p, const oprd* prd, const oracle_phys* ph, oscore* sc)
 {
     uint32_t boundary = prd->gp_boundary & 0xffffu;
     uint32_t identity = prd->iindex_identity & 0xffffu;
@@ -1122,7 +1174,7 @@ static int propagate(orng* rng, oph* p, const oprd* prd, const oracle_phys* ph)
     ostate s;
     fill_state(ph, &s, boundary, p->wavelength, cosTheta);
     uint32_t flag = 0;
-    int command = propagate_to_boundary(&flag, rng, p, &s, prd->t, ph);
+    int command = propagate_to_boundary(&flag, rng, p, &s, prd->t, ph, sc);
     if (command == C_BOUNDARY) {
         int ems = s.optical[1];
         if (ems == EMS_NOSURFACE) {
@@ -1288,16 +1340,67 @@ static inline void seq_add_nibble(uint64_t* seqhis, uint64_t* seqbnd, unsigned s
     }
 }
 
+/* genstep of in-event photon idx: off[g] <= idx < off[g+1] (qudarap/QEvt.cu:181-226 seeding) */
+static int64_t genstep_of(const int64_t* off, int64_t n_gs, int64_t idx)
+{
+    int64_t lo = 0, hi = n_gs;
+    while (hi - lo > 1) { int64_t mid = (lo + hi) / 2; if (off[mid] <= idx) lo = mid; else hi = mid; }
+    while (lo < n_gs && off[lo + 1] <= idx) ++lo;
+    return lo;
+}
+
+/* one photon through the CSGOptiX7.cu:432-530 simulate body; returns error bits */
+static int simulate_one(const oworld* w, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
+                        const float* gs, uint64_t photon_idx, oph* p, uint64_t* seqhis, uint64_t* seqbnd, oscore* sc)
+{
+    int bad = 0;
+    orng rng;
+    rng.kind = cfg->rng_kind;
+    if (rng.kind == 1) {
+        rng.nk = cfg->precooked_nk;
+        rng.cursor = 0;
+        rng.pre = ((int64_t)photon_idx < cfg->precooked_ni) ? cfg->precooked + (int64_t)photon_idx * cfg->precooked_nk : NULL;
+        if (!rng.pre) { rng.nk = 0; }
+    } else {
+        oa_rng_init(&rng.ph, cfg->seed, cfg->rng_offset, cfg->event_skipahead, (uint64_t)event_idx, photon_idx);
+    }
+    int gencode = ((const int32_t*)gs)[0];
+    if (gencode == GS_TORCH) torch_generate(p, &rng, gs, photon_idx);
+    else if (gencode == GS_SCINTILLATION || gencode == GS_SCINT_R4695) scint_generate(p, &rng, gs, ph);
+    else { memset(p, 0, sizeof(*p)); bad |= 2; }
+    ph_set_index(p, photon_idx);
+    seqhis[0] = seqhis[1] = seqbnd[0] = seqbnd[1] = 0;
+    int bounce = 0;
+    seq_add_nibble(seqhis, seqbnd, (unsigned)bounce, ph_flag(p), ph_boundary(p));
+    oprd prd;
+    while (bounce < cfg->max_bounce && p->time < cfg->max_time) {
+        float tmin = (p->orient_boundary_flag & cfg->eps0_mask) ? cfg->tmin0 : cfg->tmin;
+        if (world_trace(w, p->pos, p->mom, tmin, cfg->tmax, &prd, cfg->traverse) != 0) { bad |= 1; break; }
+        if ((prd.gp_boundary & 0xffffu) == 0xffffu) break;
+        prd.normal = norm3(prd.normal);
+        int command = propagate(&rng, p, &prd, ph, sc);
+        bounce++;
+        seq_add_nibble(seqhis, seqbnd, (unsigned)bounce, ph_flag(p), ph_boundary(p));
+        if (command == C_BREAK) break;
+    }
+    return bad;
+}
+
+static int64_t* genstep_offsets(const float* gensteps, int64_t n_gs)
+{
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_gs + 1));
+    off[0] = 0;
+    for (int64_t g = 0; g < n_gs; ++g) off[g + 1] = off[g] + (int64_t)((const uint32_t*)gensteps)[24 * g + 3];
+    return off;
+}
+
 int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
                         const float* gensteps, int64_t n_gs, int64_t photon_offset,
                         int64_t p_begin, int64_t p_end, float* photon_out, uint64_t* seq_out)
 {
     oworld* w = (oworld*)wp;
     if (!w || !ph || !cfg || (n_gs > 0 && !gensteps)) return -1;
-    /* seeding: photon -> genstep by exclusive prefix sum (qudarap/QEvt.cu:181-226) */
-    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_gs + 1));
-    off[0] = 0;
-    for (int64_t g = 0; g < n_gs; ++g) off[g + 1] = off[g] + (int64_t)((const uint32_t*)gensteps)[24 * g + 3];
+    int64_t* off = genstep_offsets(gensteps, n_gs);
     int64_t total = off[n_gs];
     if (p_end > total) p_end = total;
     if (p_begin < 0) p_begin = 0;
@@ -1307,41 +1410,10 @@ int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
 #endif
 #pragma omp parallel for schedule(dynamic, 256) reduction(|:bad)
     for (int64_t idx = p_begin; idx < p_end; ++idx) {
-        int64_t lo = 0, hi = n_gs;               /* find g with off[g] <= idx < off[g+1] */
-        while (hi - lo > 1) { int64_t mid = (lo + hi) / 2; if (off[mid] <= idx) lo = mid; else hi = mid; }
-        while (lo < n_gs && off[lo + 1] <= idx) ++lo;
-        const float* gs = gensteps + 24 * lo;
-        uint64_t photon_idx = (uint64_t)(photon_offset + idx);
-        orng rng;
-        rng.kind = cfg->rng_kind;
-        if (rng.kind == 1) {
-            rng.nk = cfg->precooked_nk;
-            rng.cursor = 0;
-            rng.pre = ((int64_t)photon_idx < cfg->precooked_ni) ? cfg->precooked + (int64_t)photon_idx * cfg->precooked_nk : NULL;
-            if (!rng.pre) { rng.nk = 0; }
-        } else {
-            oa_rng_init(&rng.ph, cfg->seed, cfg->rng_offset, cfg->event_skipahead, (uint64_t)event_idx, photon_idx);
-        }
+        const float* gs = gensteps + 24 * genstep_of(off, n_gs, idx);
         oph p;
-        int gencode = ((const int32_t*)gs)[0];
-        if (gencode == GS_TORCH) torch_generate(&p, &rng, gs, photon_idx);
-        else if (gencode == GS_SCINTILLATION || gencode == GS_SCINT_R4695) scint_generate(&p, &rng, gs, ph);
-        else { memset(&p, 0, sizeof(p)); bad |= 2; }
-        ph_set_index(&p, photon_idx);
-        uint64_t seqhis[2] = { 0, 0 }, seqbnd[2] = { 0, 0 };
-        int bounce = 0;
-        seq_add_nibble(seqhis, seqbnd, (unsigned)bounce, ph_flag(&p), ph_boundary(&p));
-        oprd prd;
-        while (bounce < cfg->max_bounce && p.time < cfg->max_time) {
-            float tmin = (p.orient_boundary_flag & cfg->eps0_mask) ? cfg->tmin0 : cfg->tmin;
-            if (world_trace(w, p.pos, p.mom, tmin, cfg->tmax, &prd, cfg->traverse) != 0) { bad |= 1; break; }
-            if ((prd.gp_boundary & 0xffffu) == 0xffffu) break;
-            prd.normal = norm3(prd.normal);
-            int command = propagate(&rng, &p, &prd, ph);
-            bounce++;
-            seq_add_nibble(seqhis, seqbnd, (unsigned)bounce, ph_flag(&p), ph_boundary(&p));
-            if (command == C_BREAK) break;
-        }
+        uint64_t seqhis[2], seqbnd[2];
+        bad |= simulate_one(w, ph, cfg, event_idx, gs, (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, NULL);
         int64_t o = idx - p_begin;
         if (photon_out) memcpy(photon_out + 16 * o, &p, 64);
         if (seq_out) {
@@ -1351,6 +1423,36 @@ int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
     }
     free(off);
     return bad ? -(int64_t)bad - 1 : total;
+}
+
+int oracle_adjoint_rows(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
+                        const float* gensteps, int64_t n_gs, int64_t photon_offset,
+                        const int64_t* photons, int64_t n, const int32_t* mat_block, int32_t n_mat,
+                        const int32_t* sample_bin, int32_t n_bin, int32_t n_param, float* rows)
+{
+    oworld* w = (oworld*)wp;
+    if (!w || !ph || !cfg || !gensteps || n_gs <= 0 || (n > 0 && (!photons || !rows))) return -1;
+    int64_t* off = genstep_offsets(gensteps, n_gs);
+    const int64_t total = off[n_gs];
+    int bad = 0;
+#ifdef _OPENMP
+    if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64) reduction(|:bad)
+    for (int64_t i = 0; i < n; ++i) {
+        float* row = rows + i * (int64_t)n_param;
+        for (int32_t k = 0; k < n_param; ++k) row[k] = 0.f;
+        const int64_t idx = photons[i];
+        if (idx < 0 || idx >= total) { bad |= 4; continue; }
+        oscore sc;
+        sc.mat_block = mat_block; sc.n_mat = n_mat; sc.sample_bin = sample_bin; sc.n_bin = n_bin; sc.row = row;
+        oph p;
+        uint64_t seqhis[2], seqbnd[2];
+        bad |= simulate_one(w, ph, cfg, event_idx, gensteps + 24 * genstep_of(off, n_gs, idx),
+                            (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, &sc);
+    }
+    free(off);
+    return bad ? -bad : 0;
 }
 
 /* ------------------------------------------------------- test helpers */

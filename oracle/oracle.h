@@ -80,6 +80,19 @@ int64_t oracle_simulate(void* world, const oracle_phys* ph, const oracle_cfg* cf
                         int64_t photon_offset, int64_t p_begin, int64_t p_end,
                         float* photon_out, uint64_t* seq_out);
 
+/*
+ * Adjoint score rows (no counterpart in the reference): replays the in-event
+ * photons photons[0..n) and writes, per photon, d log p(path)/d theta for the
+ * binned absorption and Rayleigh lengths (rows (n, n_param) f32, n_param =
+ * n_block*2*n_bin, block order [absorption bins, Rayleigh bins]).
+ * mat_block (n_mat+1) maps 1-based material index to a block (<0 none);
+ * sample_bin (n_wl) maps each table sample to its bin. Returns 0 or <0.
+ */
+int oracle_adjoint_rows(void* world, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
+                        const float* gensteps, int64_t n_gs, int64_t photon_offset,
+                        const int64_t* photons, int64_t n, const int32_t* mat_block, int32_t n_mat,
+                        const int32_t* sample_bin, int32_t n_bin, int32_t n_param, float* rows);
+
 /* Nearest hit for n rays; same record layout as mi_sim_trace_device. */
 int oracle_trace(void* world, const float* rays, int64_t n, float* isect, int32_t traverse);
 

@@ -65,6 +65,10 @@ def lib():
                                       C.c_void_p, C.c_void_p]
         L.oracle_trace.restype = C.c_int
         L.oracle_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]
+        L.oracle_adjoint_rows.argtypes = [C.c_void_p, C.POINTER(OraclePhys), C.POINTER(OracleCfg), C.c_int32,
+                                          C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                                          C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.oracle_adjoint_rows.restype = C.c_int
         L.oracle_intersect_prim.restype = C.c_int
         L.oracle_intersect_prim.argtypes = [C.POINTER(OracleGeom), C.c_int32, C.c_float,
                                             C.c_void_p, C.c_void_p, C.c_void_p]
@@ -169,6 +173,23 @@ class World:
         if rc < 0:
             raise RuntimeError(f"oracle_simulate failed rc={rc}")
         return photon, seq
+
+    def adjoint_rows(self, gensteps, photons, mat_block, sample_bin, n_bin, event_idx=0, photon_offset=0,
+                     **cfgkw):
+        """Score rows (n, n_param) f32 of the in-event photons listed in `photons`."""
+        gs = np.ascontiguousarray(gensteps, np.float32).reshape(-1, 6, 4)
+        ph = np.ascontiguousarray(photons, np.int64)
+        mb = np.ascontiguousarray(mat_block, np.int32)
+        sb = np.ascontiguousarray(sample_bin, np.int32)
+        n_param = int(max(mb.max(), 0) + 1) * 2 * int(n_bin)
+        rows = np.zeros((len(ph), n_param), np.float32)
+        cfg = self.cfg(**cfgkw)
+        rc = lib().oracle_adjoint_rows(self.handle, C.byref(self.phys), C.byref(cfg), event_idx, gs.ctypes.data,
+                                       len(gs), photon_offset, ph.ctypes.data, len(ph), mb.ctypes.data,
+                                       len(mb) - 1, sb.ctypes.data, int(n_bin), n_param, rows.ctypes.data)
+        if rc < 0:
+            raise RuntimeError(f"oracle_adjoint_rows failed rc={rc}")
+        return rows
 
     def trace(self, rays, traverse=1):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 2, 4)

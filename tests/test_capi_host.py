@@ -95,3 +95,31 @@ def test_physics_tables():
     icdf = ph["icdf"]
     assert icdf.shape == (3, 4096) and np.all(np.diff(icdf[0]) >= 0)
     assert abs(np.median(icdf[0]) - 430) < 0.1
+
+
+def test_cpp_adaptor_compiles_and_reports_errors(tmp_path):
+    """MI355XSimulator.hh builds with g++ against the library; a bad geometry
+    surfaces as a C++ exception carrying mi_last_error()."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "adaptor.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "MI355XSimulator.hh"
+int main() {
+    mi_geometry g = {}; mi_physics p = {};
+    std::printf("%s\n", mi_version());
+    try { MI355XSimulator s(g, p); std::printf("unexpected\n"); return 1; }
+    catch (const std::exception& e) { std::printf("caught: %s\n", e.what()); }
+    return 0;
+}
+''')
+    exe = tmp_path / "adaptor"
+    lib = os.path.join(ROOT, "opticks_amd")
+    cmd = ["g++", "-std=c++17", "-I", os.path.join(ROOT, "opticks_amd", "csrc"), str(src), "-o", str(exe),
+           "-L", lib, "-lopticks_amd", f"-Wl,-rpath,{lib}", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    assert out.startswith("opticks_amd") and "caught: mi_sim_create:" in out
