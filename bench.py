@@ -12,8 +12,13 @@ One step = one simulate call with gensteps already in HBM: genstep prefix
 (N,4,4), seq (N,2,2) and hits into preallocated device buffers.
 
 Also reported: the dominant kernel's roofline (HIP events on the kernel's
-stream), the CPU checker's rate on a bounded sample on this host, and the
-batched 64-event workload when --batched is given.
+stream), the CPU checker's rate on a bounded sample on this host, the
+adjoint/forward device-time ratio on the same geometry with 256 binned
+LS/water absorption and Rayleigh lengths (10M photons, 2,000 detectors), and
+the batched 64-event workload when --batched is given.
+
+--copy-calibration-mib M appends one device copy of M MiB after the timed
+region: a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE counters.
 """
 from __future__ import annotations
 
@@ -29,7 +34,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+# VALU issue peak: 256 CUs x 2.4 GHz x one wave64 VALU instruction per CU per
+# clock (4 SIMDs, 16 lanes, 4 cycles per wave64 FP32 instruction)
+VALU_PEAK_GINST = 256 * 2.4
 
 
 def algorithmic_bytes(n_photon, n_gs, steps_per_photon, hit_frac):
@@ -66,6 +73,48 @@ def cpu_baseline(w, sample_photons, threads):
     return sample_photons / dt, dt
 
 
+def profile_json(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def bench_adjoint(torch, dev, steps):
+    """Forward then path-replay adjoint on the binned-length workload: config1
+    geometry and photons, LS and water absorption/Rayleigh lengths as 4 x 64
+    bins (x(1+U(-0.2,0.2)), seed 42); gradient (2000 detectors, 256) f32."""
+    from opticks_amd import configs
+    from opticks_amd.adjoint import yield_jacobian
+    from opticks_amd.qsim import QSim, SEventConfig
+    w = configs.config3()
+    params = w.meta["params"]
+    N = w.num_photon
+    n_det = w.meta["num_det"]
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs = torch.from_numpy(w.gensteps).to(dev)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+    n_hit = sim.simulate_device(gs, photon, None, hit, n_photon=N)
+    G = yield_jacobian(sim, params, gs, hit, n_hit, n_det, n_photon=N)
+    fwd, adj, rep, red = [], [], [], []
+    for _ in range(steps):
+        n_hit = sim.simulate_device(gs, photon, None, hit, n_photon=N)
+        fwd.append(sim.last_timing()["total_ms"])
+        sim.adjoint_device(gs, hit, n_hit, params.c_struct(n_det), G, n_photon=N)
+        t = sim.last_adjoint_timing()
+        adj.append(t["total_ms"]); rep.append(t["replay_ms"]); red.append(t["reduce_ms"])
+    torch.cuda.synchronize()
+    sim.close()
+    f, a = float(np.mean(fwd)), float(np.mean(adj))
+    return {"workload": "csg-boolean-10M with LS/water absorption+Rayleigh lengths as 4x64 bins "
+                        "(256 parameters), 2000 detectors",
+            "n_photon": int(N), "hits": int(n_hit), "n_param": int(params.n_param), "n_det": int(n_det),
+            "forward_ms": f, "adjoint_ms": a, "replay_ms": float(np.mean(rep)), "reduce_ms": float(np.mean(red)),
+            "adjoint_forward_ratio": a / f}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,8 +125,10 @@ def main():
     ap.add_argument("--refill", type=int, default=1)
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="photons for the CPU baseline (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0)
     ap.add_argument("--batched", action="store_true", help="also time the 64-event batched workload")
     args = ap.parse_args()
 
@@ -122,18 +173,19 @@ def main():
     kernel_ms = float(np.mean(prop_ms))
     B = algorithmic_bytes(N, G, S, n_hit / N)
     achieved = B / (kernel_ms * 1e-3) / 1e9
-    # VALU side: counted flops per propagation step of this workload (counting rule:
-    # add/sub/mul 1, div/sqrt/log/sin/cos as one each), per photon:
-    flops_per_step = 520.0
-    valu_tflops = N * S * flops_per_step / (kernel_ms * 1e-3) / 1e12
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_config1.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            t = json.load(f)
-        if t.get("n_photon") == N and t.get("kernel") == args.kernel:
-            traffic = t.get("hbm_bytes_per_launch")
+    kname = "k_sim_wave" if kernel == L.MI_KERNEL_WAVEFRONT else "k_sim_plain"
+    # HBM bytes and VALU instructions per launch of this kernel on this workload,
+    # from rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE scaled by a copy kernel;
+    # SQ_INSTS_VALU, SQ_THREAD_CYCLES_VALU, SQ_ACTIVE_INST_VALU)
+    traffic, valu = None, None
+    pj = profile_json("pmc_k_sim_wave_csg10M.json")
+    if pj and pj.get("n_photon") == N and pj.get("kernel") == kname:
+        traffic = pj.get("hbm_bytes_per_launch")
+        if pj.get("valu_insts_per_launch"):
+            rate = pj["valu_insts_per_launch"] / (kernel_ms * 1e-3) / 1e9
+            valu = {"achieved_ginst_s": rate, "peak_ginst_s": VALU_PEAK_GINST, "frac": rate / VALU_PEAK_GINST,
+                    "insts_per_launch": pj["valu_insts_per_launch"],
+                    "lane_utilisation": pj.get("valu_lane_utilisation")}
 
     line = {
         "metric": "photons propagated/sec on 1 MI355X vs CPU qsim; adjoint/forward time ratio",
@@ -155,21 +207,30 @@ def main():
                    "outputs": "photon+seq+hit", "steps_per_photon": S, "hits": int(n_hit)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sim_wave" if kernel == L.MI_KERNEL_WAVEFRONT else "k_sim_plain",
-                     "kernel_ms": kernel_ms, "alg_bytes_per_launch": B,
-                     "valu": {"achieved_tflops": valu_tflops, "peak_tflops": FP32_PEAK_TFLOPS,
-                              "frac": valu_tflops / FP32_PEAK_TFLOPS, "flops_per_step": flops_per_step}},
+                     "kernel": kname, "kernel_ms": kernel_ms, "alg_bytes_per_launch": B, "valu": valu},
         "cpu_baseline": None,
     }
+    if args.copy_calibration_mib > 0:
+        src = torch.ones(args.copy_calibration_mib * 1024 * 256, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if not args.no_adjoint:
+        a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)))
+        line["adjoint"] = a
+        line["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        rate, cdt = cpu_baseline(w, args.cpu_sample, threads)
+        sample = args.cpu_sample if args.cpu_sample > 0 else N
+        rate, cdt = cpu_baseline(w, sample, threads)
         line["cpu_baseline"] = {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
-                                "sample": f"first {args.cpu_sample} photons of the same workload "
-                                          f"({cdt:.1f} s, C restatement of qsim/CSG, OpenMP)"}
+                                "sample": f"first {sample} photons of the same workload "
+                                          f"({cdt:.1f} s wall on {threads} threads; C restatement of "
+                                          f"qsim/CSG, OpenMP over photons, own median BVH)"}
     if args.batched:
         line["batched"] = bench_batched(sim, torch, dev)
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
     sim.close()
 
 

@@ -152,6 +152,7 @@ def _autograd_function():
             counts = hits_per_detector(hit_t, n_hit, n_det)
             ctx.state = (sim, params, gs_t, hit_t, int(n_hit), n_det, event_id, n_photon, bnd)
             ctx.theta_shape = theta.shape
+            ctx.theta_device, ctx.theta_dtype = theta.device, theta.dtype
             return counts
 
         @staticmethod
@@ -162,6 +163,7 @@ def _autograd_function():
             sim.adjoint_device(gs_t, hit_t, n_hit, params.c_struct(n_det), G, eventID=event_id,
                                n_photon=n_photon)
             grad_theta = (grad_counts.to(torch.float32).reshape(1, n_det) @ G).reshape(ctx.theta_shape)
+            grad_theta = grad_theta.to(device=ctx.theta_device, dtype=ctx.theta_dtype)
             return grad_theta, None, None, None, None, None, None
 
     return ExpectedHits

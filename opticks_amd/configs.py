@@ -122,7 +122,9 @@ def detector_tree():
     return Node.op(CSG_DIFFERENCE, Node.op(CSG_UNION, ell, neck), bore)
 
 
-def config0(num_photon=1_000_000, seed=42, water_absorption=20000.0, water_rayleigh=30000.0):
+def config0(num_photon=1_000_000, seed=42, water_absorption=20000.0, water_rayleigh=30000.0, sensor=False):
+    """sensor=True gives the world instance sensor_identifier 0, so its detector
+    surface counts as detector 0 for per-detector yields."""
     lib = BoundaryLib()
     lib.add_material(Material("Vacuum", 1.0, 1e9, 1e9))
     lib.add_material(Material("Water", 1.333, water_absorption, water_rayleigh))
@@ -133,7 +135,10 @@ def config0(num_photon=1_000_000, seed=42, water_absorption=20000.0, water_rayle
     p0 = fd.addPrim(Node.box3(4000.0, 4000.0, 4000.0), b_world)
     p1 = fd.addPrim(Node.sphere(1000.0), b_sphere)
     s0 = fd.addSolid("world", [p0, p1])
-    fd.addInstance(np.eye(4), s0)
+    if sensor:
+        fd.addInstance(np.eye(4), s0, sensor_identifier=0, sensor_index=0)
+    else:
+        fd.addInstance(np.eye(4), s0)
     phys = lib.arrays()
     phys["icdf"], phys["icdf_hd_factor"] = gaussian_icdf()
     phys["boundary_names"] = lib.boundaries

@@ -1,0 +1,204 @@
+"""Path-replay adjoint on the MI355X (mi_sim_adjoint_device) against
+
+* the oracle's score rows for the same hit photons, summed per detector in
+  float64 (the replay reproduces the forward path bit for bit, so only the
+  FP32 Kahan summation differs);
+* the closed-form yield derivative of the water sphere without scattering;
+* central finite differences, step 1%, with common random numbers (same seed,
+  same photon streams) on the 2,000-detector CSG workload with the binned
+  LS/water lengths, along several parameter directions summed over detectors,
+  with the FD error bar taken from the photons whose outcome changed and the
+  adjoint's from independent photon subsets;
+* the torch.autograd.Function wrapper (backward = Jacobian contraction).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from opticks_amd import configs
+from opticks_amd.adjoint import LengthParams, expected_hits, yield_jacobian
+from opticks_amd.qsim import QSim, SEventConfig
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+N_DET = 2000
+
+
+def run_forward(sim, gs_t, n_photon, event_id=0):
+    photon = torch.empty((n_photon, 4, 4), dtype=torch.float32, device=DEV)
+    hit = torch.empty((n_photon, 4, 4), dtype=torch.float32, device=DEV)
+    n_hit = sim.simulate_device(gs_t, photon, None, hit, eventID=event_id, n_photon=n_photon)
+    del photon
+    return hit, n_hit
+
+
+def hit_photon_index(hit, n_hit):
+    return hit[:n_hit, 3, 2].contiguous().view(torch.int32).long()
+
+
+def test_adjoint_matches_oracle_rows():
+    w = configs.config3(num_gs=20, per_gs=1000)
+    params = w.meta["params"]
+    N = w.num_photon
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    hit, n_hit = run_forward(sim, gs_t, N)
+    assert n_hit > 100
+    idx_all = hit_photon_index(hit, n_hit).cpu().numpy()
+    world = po.World(w.geometry, w.physics)
+    ref_photon, _ = world.simulate(w.gensteps, seed=w.seed)
+    ref_hit_idx = np.nonzero(ref_photon[:, 3, 3].view(np.uint32) & 0x40)[0]
+    common = np.intersect1d(idx_all, ref_hit_idx)
+    assert len(common) >= 0.999 * max(len(idx_all), len(ref_hit_idx))
+    keep = torch.from_numpy(np.isin(idx_all, common)).to(DEV)
+    hit = hit[:n_hit][keep].contiguous()
+    n_hit = hit.shape[0]
+    G = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, n_photon=N).cpu().numpy().astype(np.float64)
+    h = hit.cpu().numpy()
+    idx = h[:, 3, 2].view(np.uint32).astype(np.int64)
+    det = (h[:, 3, 1].view(np.uint32) & 0xffff).astype(np.int64) - 1
+    rows = world.adjoint_rows(w.gensteps, idx, params.mat_block, params.sample_bin, params.n_bin,
+                              seed=w.seed).astype(np.float64)
+    G64 = np.zeros((N_DET, params.n_param))
+    A64 = np.zeros((N_DET, params.n_param))
+    np.add.at(G64, det, rows)
+    np.add.at(A64, det, np.abs(rows))
+    err = np.abs(G - G64)
+    assert np.all(err <= 1e-5 * A64 + 1e-30), err.max()
+    assert np.count_nonzero(G) > 1000
+    # the 256 parameters in order: LS absorption, LS Rayleigh, water absorption, water Rayleigh
+    assert np.abs(G).sum(axis=0).reshape(2, 2, 64).sum(axis=2).min() > 0
+    sim.close()
+
+
+def test_adjoint_closed_form_water_sphere():
+    La, R, n = 20000.0, 1000.0, 1.333
+    w = configs.config0(num_photon=4_000_000, water_absorption=La, water_rayleigh=1e12, sensor=True)
+    w.gensteps[0, 3, 3] = 443.25   # samples 443 (bin 25, weight 0.75) and 444 nm (bin 26, 0.25)
+    params = LengthParams(w.physics, ("Water",), n_bin=64)
+    N = w.num_photon
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    hit, n_hit = run_forward(sim, gs_t, N)
+    G = yield_jacobian(sim, params, gs_t, hit, n_hit, 1, n_photon=N).cpu().numpy()[0] / N
+    rf = ((n - 1) / (n + 1)) ** 2
+    x = np.exp(-R / La)
+    q = rf * x * x
+    dP = (1 - rf) * x * (1 + q) / (1 - q) ** 2 * R / La ** 2
+    assert abs(G[25] / (0.75 * dP) - 1) < 2e-3
+    assert abs(G[26] / (0.25 * dP) - 1) < 2e-3
+    sim.close()
+
+
+@pytest.fixture(scope="module")
+def fd_setup():
+    w = configs.config3(num_gs=4000, per_gs=10_000)
+    params = w.meta["params"]
+    theta = w.meta["theta"].astype(np.float64)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    yield w, params, theta, sim, gs_t
+    sim.close()
+
+
+def test_adjoint_vs_central_fd(fd_setup):
+    w, params, theta, sim, gs_t = fd_setup
+    N = w.num_photon
+    base_bnd = sim.base_bnd
+    h = 0.01
+    hit, n_hit = run_forward(sim, gs_t, N)
+    G = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, n_photon=N)
+    Gtot = G.sum(dim=0).double().cpu().numpy().reshape(params.shape)
+    # adjoint error bar: 8 disjoint photon subsets of the hits
+    idx = hit_photon_index(hit, n_hit)
+    parts = []
+    for k in range(8):
+        sel = hit[:n_hit][(idx % 8) == k].contiguous()
+        Gk = yield_jacobian(sim, params, gs_t, sel, sel.shape[0], N_DET, n_photon=N)
+        parts.append(Gk.sum(dim=0).double().cpu().numpy().reshape(params.shape))
+    parts = np.stack(parts)
+    assert np.allclose(parts.sum(axis=0), Gtot, rtol=1e-4, atol=1e-6 * np.abs(Gtot).max())
+    del hit
+
+    rng = np.random.default_rng(7)
+    directions = {
+        "all": np.ones(params.shape),
+        "LS abs": np.zeros(params.shape), "LS ray": np.zeros(params.shape),
+        "water abs": np.zeros(params.shape), "water ray": np.zeros(params.shape),
+        "random signs": rng.choice([-1.0, 1.0], size=params.shape),
+    }
+    directions["LS abs"][0, 0] = 1
+    directions["LS ray"][0, 1] = 1
+    directions["water abs"][1, 0] = 1
+    directions["water ray"][1, 1] = 1
+    report = {}
+    for name, v in directions.items():
+        dtheta = theta * v                      # d theta / d s along s -> theta (1 + s v)
+        adj = float((Gtot * dtheta).sum())
+        adj_parts = (parts * dtheta).sum(axis=(1, 2, 3))
+        sig_adj = float(adj_parts.std(ddof=1) * np.sqrt(len(adj_parts)))
+        masks = []
+        for sgn in (+1, -1):
+            sim.set_bnd(params.apply(base_bnd, (theta * (1.0 + sgn * h * v)).astype(np.float32)))
+            hp, nh = run_forward(sim, gs_t, N)
+            m = torch.zeros(N, dtype=torch.bool, device=DEV)
+            m[hit_photon_index(hp, nh)] = True
+            masks.append(m)
+            del hp
+        fd = (masks[0].sum().item() - masks[1].sum().item()) / (2 * h)
+        n_changed = (masks[0] ^ masks[1]).sum().item()
+        sig_fd = np.sqrt(n_changed) / (2 * h)
+        sig = np.hypot(sig_fd, sig_adj)
+        report[name] = (adj, fd, sig_adj, sig_fd)
+        print(f"FD check {name:13s} adjoint {adj:12.1f} +- {sig_adj:8.1f}  fd {fd:12.1f} +- {sig_fd:8.1f}"
+              f"  rel diff {(adj - fd) / fd:+.4f}", flush=True)
+        assert abs(adj - fd) < 4 * sig + 1e-9, (name, adj, fd, sig)
+    sim.set_bnd(base_bnd)
+    # along the directions that carry the yield's sensitivity, agreement within 1%
+    for name in ("all", "LS abs", "water abs"):
+        adj, fd, sa, sf = report[name]
+        assert abs(adj - fd) <= 0.01 * abs(fd) + 3 * np.hypot(sa, sf), (name, adj, fd)
+
+
+def test_autograd_wrapper_matches_jacobian():
+    w = configs.config3(num_gs=50, per_gs=2000)
+    params = w.meta["params"]
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    theta = torch.tensor(w.meta["theta"], dtype=torch.float32, requires_grad=True)
+    counts = expected_hits(theta, sim, params, gs_t, N_DET)
+    assert counts.shape == (N_DET,)
+    weights = torch.linspace(-1.0, 1.0, N_DET, device=counts.device)
+    (counts * weights).sum().backward()
+    hit, n_hit = run_forward(sim, gs_t, w.num_photon)
+    assert int(counts.sum().item()) == n_hit
+    G = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, n_photon=w.num_photon)
+    ref = (weights[None, :] @ G).reshape(params.shape).cpu()
+    assert torch.allclose(theta.grad, ref, rtol=1e-5, atol=1e-7 * float(ref.abs().max()))
+    sim.close()
+
+
+def test_adjoint_rejects_bad_arguments():
+    from opticks_amd import _lib as L
+    w = configs.config3(num_gs=2, per_gs=100)
+    params = w.meta["params"]
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    hit, n_hit = run_forward(sim, gs_t, w.num_photon)
+    G = torch.empty((N_DET, params.n_param), dtype=torch.float32, device=DEV)
+    bad = LengthParams(w.physics, ("LS",), n_bin=64)
+    bad.sample_bin = bad.sample_bin.copy()
+    bad.sample_bin[0] = 64
+    with pytest.raises(RuntimeError):
+        sim.adjoint_device(gs_t, hit, n_hit, bad.c_struct(N_DET), G, n_photon=w.num_photon)
+    with pytest.raises(RuntimeError):
+        sim.adjoint_device(gs_t, hit, n_hit, params.c_struct(70000), G, n_photon=w.num_photon)
+    # no hits: zero gradient
+    sim.adjoint_device(gs_t, hit, 0, params.c_struct(N_DET), G, n_photon=w.num_photon)
+    assert float(G.abs().sum()) == 0.0
+    sim.close()
