@@ -22,6 +22,8 @@
  *   QBnd upload of bnd (qudarap/QBnd.cc:130-175)         mi_sim_set_bnd (new values,
  *                                                         same shape, no rebuild)
  *   (no counterpart: differentiation)                   mi_sim_adjoint_device
+ *   (no counterpart: replayable captured batches)       mi_batch_* (hipGraph)
+ *   (no counterpart: launch tuning)                     mi_sim_autotune
  *
  * Array layouts are the reference's, so arrays exported by the reference
  * (CSGFoundry / SSim folders) can be passed unchanged:
@@ -252,6 +254,46 @@ int  mi_sim_adjoint_device(mi_sim* sim, int32_t event_id,
                            const void* d_hit, int64_t n_hit,
                            const mi_adjoint* adj, void* d_grad, void* stream);
 int  mi_sim_last_adjoint_timing(const mi_sim* sim, double* ms3);
+
+/*
+ * Captured batch (no counterpart in the reference): the batched form's
+ * launch sequence -- genstep prefix, persistent propagation, hit compaction,
+ * per-event hit offsets -- recorded once into a hipGraph and replayed by
+ * mi_batch_launch. Arguments as mi_sim_simulate_batch_device; the device
+ * buffers must stay valid while the batch exists. Between launches the
+ * gensteps' contents may change but not their per-genstep photon counts
+ * (checked on the device; mi_batch_wait then fails with MI_ERR_ARG).
+ * Launch settings (kernel, block, refill, waves) are those of the handle at
+ * create; a later change of TLAS leaf size or precooked randoms invalidates
+ * the batch (launch fails; recreate it). bnd values set with mi_sim_set_bnd
+ * are seen by later launches. One launch in flight per batch.
+ */
+typedef struct mi_batch mi_batch;
+int  mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids,
+                     const int64_t* gs_offsets, const void* d_gensteps,
+                     void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                     int64_t* photon_offsets, mi_batch** out);
+int  mi_batch_launch(mi_batch* b, void* stream);                 /* asynchronous */
+int  mi_batch_wait(mi_batch* b, int64_t* hit_offsets, double* device_ms);
+int64_t mi_batch_num_photon(const mi_batch* b);
+int  mi_batch_destroy(mi_batch* b);
+
+/*
+ * Autotuner (no counterpart in the reference): times the batched form on the
+ * given events for every block size {64,128,256} x refill (compaction) period
+ * {1,4,16} x TLAS leaf size {1,2,4}, reps timed runs each after one warm run,
+ * and leaves the handle configured with the fastest. results (optional,
+ * max_results entries) receives each candidate's mean device ms.
+ */
+typedef struct mi_tune_result {
+    int32_t block_size, refill_period, bvh_leaf_size, kernel;
+    double ms;
+} mi_tune_result;
+int  mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids,
+                     const int64_t* gs_offsets, const void* d_gensteps,
+                     void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                     int32_t reps, mi_tune_result* results, int32_t max_results,
+                     int32_t* n_results);
 
 #ifdef __cplusplus
 }

@@ -55,11 +55,17 @@ class MiAdjoint(C.Structure):
                 ("n_bin", C.c_int32), ("n_block", C.c_int32), ("n_det", C.c_int32)]
 
 
+class MiTuneResult(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("refill_period", C.c_int32), ("bvh_leaf_size", C.c_int32),
+                ("kernel", C.c_int32), ("ms", C.c_double)]
+
+
 SYMBOLS = ["mi_config_default", "mi_last_error", "mi_version", "mi_sim_create", "mi_sim_destroy",
            "mi_sim_set_config", "mi_sim_get_config", "mi_sim_simulate", "mi_sim_reset",
            "mi_sim_simulate_device", "mi_sim_simulate_batch_device", "mi_sim_trace_device",
            "mi_sim_refit_instances", "mi_sim_last_timing", "mi_sim_set_bnd", "mi_sim_adjoint_device",
-           "mi_sim_last_adjoint_timing"]
+           "mi_sim_last_adjoint_timing", "mi_batch_create", "mi_batch_launch", "mi_batch_wait",
+           "mi_batch_num_photon", "mi_batch_destroy", "mi_sim_autotune"]
 
 _lib = None
 
@@ -94,6 +100,16 @@ def lib():
     L.mi_sim_adjoint_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                         C.c_void_p, C.c_int64, C.POINTER(MiAdjoint), C.c_void_p, C.c_void_p]
     L.mi_sim_last_adjoint_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.mi_batch_create.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_void_p)]
+    L.mi_batch_launch.argtypes = [C.c_void_p, C.c_void_p]
+    L.mi_batch_wait.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+    L.mi_batch_num_photon.argtypes = [C.c_void_p]
+    L.mi_batch_num_photon.restype = C.c_int64
+    L.mi_batch_destroy.argtypes = [C.c_void_p]
+    L.mi_sim_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.POINTER(MiTuneResult),
+                                  C.c_int32, C.POINTER(C.c_int32)]
     for name in SYMBOLS:
         if name not in ("mi_last_error", "mi_version", "mi_config_default", "mi_sim_destroy"):
             getattr(L, name).restype = C.c_int

@@ -119,6 +119,9 @@ struct mi_sim {
     uint32_t tlas_depth = 0, blas_depth = 0;
     std::vector<uint32_t> solid_blas_root;
     std::vector<uint32_t> h_inst_blas_root;
+    std::vector<std::array<double, 6>> solid_box;   /* padded solid-frame prim box per solid */
+    std::vector<float> h_o2w;                       /* (n_inst,16) current instance transforms */
+    uint64_t generation = 0;                        /* bumped when a device buffer may move */
     DBuf<float4> node, itra;
     DBuf<InstRec> inst;
     DBuf<PrimRec> prim;
@@ -242,6 +245,54 @@ static int validate_geometry(const mi_geometry* g, const mi_physics* ph)
     return MI_OK;
 }
 
+/* padded world boxes of the instances from the current transforms */
+static void instance_items(const mi_sim* sim, std::vector<oa_bvh::Item>& titems)
+{
+    const int64_t ni = sim->n_inst;
+    titems.resize(ni);
+    for (int64_t i = 0; i < ni; ++i) {
+        const float* m = sim->h_o2w.data() + 16 * i;
+        int32_t gas;
+        std::memcpy(&gas, m + 7, 4);
+        const std::array<double, 6>& bx = sim->solid_box[gas];
+        oa_bvh::Item& it = titems[i];
+        for (int k = 0; k < 3; ++k) { it.lo[k] = 1e300; it.hi[k] = -1e300; }
+        for (int c = 0; c < 8; ++c) {
+            double v[3] = { (c & 1) ? bx[3] : bx[0], (c & 2) ? bx[4] : bx[1], (c & 4) ? bx[5] : bx[2] };
+            for (int k = 0; k < 3; ++k) {
+                double p = v[0] * m[k] + v[1] * m[4 + k] + v[2] * m[8 + k] + m[12 + k];
+                it.lo[k] = std::min(it.lo[k], p); it.hi[k] = std::max(it.hi[k], p);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            double pad = 1e-3 * (it.hi[k] - it.lo[k]) + 1e-2;
+            it.lo[k] -= pad; it.hi[k] += pad;
+        }
+        it.id = (uint32_t)i;
+    }
+}
+
+/* host SAH build of the TLAS over instance boxes with the given leaf size, then upload */
+static int build_tlas(mi_sim* sim, int leaf)
+{
+    std::vector<oa_bvh::Item> titems;
+    instance_items(sim, titems);
+    std::vector<BvhNode> tlas;
+    std::vector<uint32_t> tlas_items;
+    oa_bvh::Builder tb(tlas, tlas_items, leaf);
+    uint32_t troot = tb.build(titems);
+    if (troot != 0) return fail(MI_ERR_GEOMETRY, "TLAS root is not node 0");
+    if (tb.max_depth + sim->blas_depth + 2 > OA_STACK_DEPTH)
+        return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
+    HIPCHECK(hipStreamSynchronize(sim->stream));
+    HIPCHECK(upload(sim->tlas, tlas.data(), tlas.size(), sim->stream));
+    HIPCHECK(upload(sim->tlas_items, tlas_items.data(), tlas_items.size(), sim->stream));
+    sim->tlas_depth = tb.max_depth;
+    sim->n_tlas = (int64_t)tlas.size();
+    sim->generation++;
+    return MI_OK;
+}
+
 static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
 {
     hipStream_t s = sim->stream;
@@ -275,10 +326,11 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
         bdepth = std::max(bdepth, b.max_depth);
         solid_box[so] = bx;
     }
-    /* instances: world->object transforms and padded world boxes */
+    /* instances: world->object transforms */
     std::vector<InstRec> inst(ni);
-    std::vector<oa_bvh::Item> titems(ni);
     sim->h_inst_blas_root.resize(ni);
+    sim->solid_box = solid_box;
+    sim->h_o2w.assign(g->inst, g->inst + 16 * ni);
     for (int64_t i = 0; i < ni; ++i) {
         const float* m = g->inst + 16 * i;
         int32_t gas, ident;
@@ -290,34 +342,12 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
         r.m[7] = f32_of((uint32_t)ident);
         r.m[11] = f32_of(sim->solid_blas_root[gas]);
         sim->h_inst_blas_root[i] = sim->solid_blas_root[gas];
-        const std::array<double, 6>& bx = solid_box[gas];
-        oa_bvh::Item& it = titems[i];
-        for (int k = 0; k < 3; ++k) { it.lo[k] = 1e300; it.hi[k] = -1e300; }
-        for (int c = 0; c < 8; ++c) {
-            double v[3] = { (c & 1) ? bx[3] : bx[0], (c & 2) ? bx[4] : bx[1], (c & 4) ? bx[5] : bx[2] };
-            for (int k = 0; k < 3; ++k) {
-                double p = v[0] * m[k] + v[1] * m[4 + k] + v[2] * m[8 + k] + m[12 + k];
-                it.lo[k] = std::min(it.lo[k], p); it.hi[k] = std::max(it.hi[k], p);
-            }
-        }
-        for (int k = 0; k < 3; ++k) {
-            double pad = 1e-3 * (it.hi[k] - it.lo[k]) + 1e-2;
-            it.lo[k] -= pad; it.hi[k] += pad;
-        }
-        it.id = (uint32_t)i;
     }
-    std::vector<BvhNode> tlas;
-    std::vector<uint32_t> tlas_items;
-    oa_bvh::Builder tb(tlas, tlas_items, sim->cfg.bvh_leaf_size);
-    uint32_t troot = tb.build(titems);
-    if (troot != 0) return fail(MI_ERR_GEOMETRY, "TLAS root is not node 0");
-    if (tb.max_depth + bdepth + 2 > OA_STACK_DEPTH)
-        return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
-    sim->tlas_depth = tb.max_depth;
     sim->blas_depth = bdepth;
-    sim->n_tlas = (int64_t)tlas.size();
     sim->n_inst = ni;
     sim->n_prim = g->n_prim;
+    int trc = build_tlas(sim, sim->cfg.bvh_leaf_size);
+    if (trc) return trc;
 
     std::vector<PrimRec> prim(g->n_prim);
     for (int64_t p = 0; p < g->n_prim; ++p) {
@@ -333,8 +363,6 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     else HIPCHECK(sim->itra.reserve(4));
     HIPCHECK(upload(sim->inst, inst.data(), inst.size(), s));
     HIPCHECK(upload(sim->prim, prim.data(), prim.size(), s));
-    HIPCHECK(upload(sim->tlas, tlas.data(), tlas.size(), s));
-    HIPCHECK(upload(sim->tlas_items, tlas_items.data(), tlas_items.size(), s));
     HIPCHECK(upload(sim->blas, blas.data(), blas.size(), s));
     HIPCHECK(upload(sim->blas_items, blas_items.data(), blas_items.size(), s));
     HIPCHECK(upload(sim->o2w, g->inst, (size_t)ni * 16, s));
@@ -377,6 +405,7 @@ static int apply_config(mi_sim* sim, const mi_config* cfg)
         HIPCHECK(upload(sim->precooked, cfg->precooked, (size_t)(cfg->precooked_ni * cfg->precooked_nk), sim->stream));
     }
     sim->cfg.precooked = nullptr;   /* the handle keeps its own device copy */
+    if (cfg->rng_kind == MI_RNG_PRECOOKED) sim->generation++;
     return MI_OK;
 }
 
@@ -429,9 +458,11 @@ void mi_sim_destroy(mi_sim* sim)
 int mi_sim_set_config(mi_sim* sim, const mi_config* cfg)
 {
     if (!sim || !cfg) return fail(MI_ERR_ARG, "null");
-    if (cfg->bvh_leaf_size != sim->cfg.bvh_leaf_size)
-        return fail(MI_ERR_ARG, "bvh_leaf_size is fixed at create time");
-    return apply_config(sim, cfg);
+    const int old_leaf = sim->cfg.bvh_leaf_size;
+    int rc = apply_config(sim, cfg);
+    if (rc) return rc;
+    if (cfg->bvh_leaf_size != old_leaf) rc = build_tlas(sim, cfg->bvh_leaf_size);
+    return rc;
 }
 
 int mi_sim_get_config(const mi_sim* sim, mi_config* cfg)
@@ -757,6 +788,7 @@ int mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst)
     }
     HIPCHECK(upload(sim->inst, rec.data(), rec.size(), s));
     HIPCHECK(upload(sim->o2w, inst, (size_t)n_inst * 16, s));
+    sim->h_o2w.assign(inst, inst + 16 * n_inst);
     HIPCHECK(oa_launch::refit(sim->o2w.p, sim->blas.p, sim->inst_blas_root.p, n_inst, sim->inst_box.p,
                               sim->tlas.p, sim->n_tlas, sim->tlas_depth, sim->tlas_items.p, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -849,6 +881,232 @@ int mi_sim_last_adjoint_timing(const mi_sim* sim, double* ms3)
     if (!sim || !ms3) return fail(MI_ERR_ARG, "null argument");
     for (int i = 0; i < 3; ++i) ms3[i] = sim->adj_ms[i];
     return MI_OK;
+}
+
+/* ------------------------------------------------------ captured batches */
+
+struct mi_batch {
+    mi_sim* sim = nullptr;
+    uint64_t generation = 0;
+    int32_t n_event = 0;
+    int64_t n_gs = 0, n_photon = 0, n_words = 0, hit_capacity = 0;
+    bool want_hits = false;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = { nullptr, nullptr };
+    DBuf<int64_t> prefix, saved_prefix, evbuf, block_total, ho;
+    DBuf<int32_t> ev_id, word_prefix;
+    DBuf<unsigned long long> words, next, flag;
+    int64_t* h_out = nullptr;   /* pinned: hit offsets (n_event+1), then the prefix-mismatch flag */
+    bool launched = false;
+    double last_ms = 0.0;
+};
+
+int mi_batch_destroy(mi_batch* b)
+{
+    if (!b) return MI_OK;
+    if (b->launched && b->ev[1]) (void)hipEventSynchronize(b->ev[1]);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    b->prefix.release(); b->saved_prefix.release(); b->evbuf.release(); b->block_total.release(); b->ho.release();
+    b->ev_id.release(); b->word_prefix.release(); b->words.release(); b->next.release(); b->flag.release();
+    delete b;
+    return MI_OK;
+}
+
+/* the work of one batch launch, issued on stream s (recorded when s is capturing) */
+static hipError_t batch_work(mi_batch* b, const SimParams& P, const void* d_gensteps, const void* d_photon, void* d_hit,
+                             hipStream_t s)
+{
+    const mi_sim* sim = b->sim;
+    hipError_t e;
+    if ((e = oa_launch::gs_prefix((const float*)d_gensteps, b->n_gs, b->prefix.p, s)) != hipSuccess) return e;
+    if ((e = oa_launch::zero_u64(b->flag.p, 1, s)) != hipSuccess) return e;
+    if ((e = oa_launch::prefix_check(b->prefix.p, b->saved_prefix.p, b->n_gs + 1, b->flag.p, s)) != hipSuccess) return e;
+    if (sim->cfg.kernel == MI_KERNEL_WAVEFRONT) {
+        if ((e = oa_launch::zero_u64(b->words.p, b->n_words, s)) != hipSuccess) return e;
+        if ((e = oa_launch::zero_u64(b->next.p, 1, s)) != hipSuccess) return e;
+    }
+    if ((e = oa_launch::simulate(P, sim->cfg.kernel, sim->cfg.block_size, wave_grid(sim), s)) != hipSuccess) return e;
+    if ((e = oa_launch::compact_hits(b->words.p, b->n_words, b->word_prefix.p, b->block_total.p,
+                                     (const float4*)d_photon, (float4*)d_hit, d_hit ? b->hit_capacity : 0, s)) != hipSuccess)
+        return e;
+    if ((e = oa_launch::event_hit_offsets(b->words.p, b->word_prefix.p, b->block_total.p, P.ev_ph_begin, b->n_event,
+                                          b->n_words, b->ho.p, s)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(b->h_out, b->ho.p, sizeof(int64_t) * (b->n_event + 1), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return e;
+    return hipMemcpyAsync(b->h_out + b->n_event + 1, b->flag.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+}
+
+int mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
+                    const void* d_gensteps, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                    int64_t* photon_offsets, mi_batch** out)
+{
+    if (!out) return fail(MI_ERR_ARG, "null out");
+    *out = nullptr;
+    if (!sim || n_event <= 0 || !event_ids || !gs_offsets || !photon_offsets || !d_gensteps)
+        return fail(MI_ERR_ARG, "bad batch arguments");
+    if (d_hit && !d_photon) return fail(MI_ERR_ARG, "hits are gathered from the photon array: pass d_photon");
+    for (int32_t e = 0; e < n_event; ++e)
+        if (gs_offsets[e + 1] < gs_offsets[e] || gs_offsets[0] != 0) return fail(MI_ERR_ARG, "gs_offsets not ascending from 0");
+    const int64_t n_gs = gs_offsets[n_event];
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    mi_batch* b = new mi_batch();
+    b->sim = sim;
+    b->n_event = n_event;
+    b->n_gs = n_gs;
+    b->hit_capacity = hit_capacity;
+    auto bail = [&](int rc) { mi_batch_destroy(b); return rc; };
+#define BCHECK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return bail(fail(MI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); } while (0)
+    hipStream_t s = sim->stream;
+    BCHECK(b->prefix.reserve((size_t)n_gs + 1));
+    BCHECK(b->saved_prefix.reserve((size_t)n_gs + 1));
+    BCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, b->prefix.p, s));
+    std::vector<int64_t> hp((size_t)n_gs + 1);
+    BCHECK(hipMemcpyAsync(hp.data(), b->prefix.p, sizeof(int64_t) * (n_gs + 1), hipMemcpyDeviceToHost, s));
+    BCHECK(hipMemcpyAsync(b->saved_prefix.p, b->prefix.p, sizeof(int64_t) * (n_gs + 1), hipMemcpyDeviceToDevice, s));
+    BCHECK(hipStreamSynchronize(s));
+    std::vector<int64_t> evh(2 * ((size_t)n_event + 1));
+    for (int e = 0; e <= n_event; ++e) {
+        evh[e] = gs_offsets[e];
+        evh[n_event + 1 + e] = hp[gs_offsets[e]];
+        photon_offsets[e] = hp[gs_offsets[e]];
+    }
+    b->n_photon = hp[n_gs];
+    if (b->n_photon == 0) return bail(fail(MI_NOTHING_TO_DO, "gensteps carry no photons"));
+    b->n_words = (b->n_photon + 63) / 64;
+    const int64_t nb = (b->n_words + 255) / 256;
+    BCHECK(upload(b->evbuf, evh.data(), evh.size(), s));
+    BCHECK(upload(b->ev_id, event_ids, (size_t)n_event, s));
+    BCHECK(b->words.reserve((size_t)b->n_words + 1));
+    BCHECK(b->next.reserve(1));
+    BCHECK(b->flag.reserve(1));
+    BCHECK(b->word_prefix.reserve((size_t)b->n_words + 1));
+    BCHECK(b->block_total.reserve((size_t)nb + 1));
+    BCHECK(b->ho.reserve((size_t)n_event + 1));
+    BCHECK(hipHostMalloc((void**)&b->h_out, sizeof(int64_t) * ((size_t)n_event + 2), hipHostMallocDefault));
+    for (auto& e : b->ev) BCHECK(hipEventCreate(&e));
+    BCHECK(hipStreamCreate(&b->cap));
+    SimParams P = base_params(sim);
+    P.gensteps = (const float*)d_gensteps;
+    P.gs_prefix = b->prefix.p;
+    P.n_gs = n_gs;
+    P.n_event = n_event;
+    P.ev_gs_begin = b->evbuf.p;
+    P.ev_ph_begin = b->evbuf.p + (n_event + 1);
+    P.ev_id = b->ev_id.p;
+    P.n_photon = b->n_photon;
+    P.photon_offset = 0;
+    P.photon = (float4*)d_photon;
+    P.seq = (uint64_t*)d_seq;
+    P.hitwords = b->words.p;
+    P.next = b->next.p;
+    BCHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    hipError_t we = batch_work(b, P, d_gensteps, d_photon, d_hit, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->cap, &g);
+    b->graph = g;
+    BCHECK(we);
+    BCHECK(ce);
+    BCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+#undef BCHECK
+    b->generation = sim->generation;
+    *out = b;
+    return MI_OK;
+}
+
+int mi_batch_launch(mi_batch* b, void* stream)
+{
+    if (!b) return fail(MI_ERR_ARG, "null batch");
+    if (b->generation != b->sim->generation)
+        return fail(MI_ERR_ARG, "the handle's device buffers moved since capture (leaf size or precooked change): recreate the batch");
+    hipStream_t s = stream ? (hipStream_t)stream : b->sim->stream;
+    if (b->launched) HIPCHECK(hipEventSynchronize(b->ev[1]));   /* one replay in flight per batch */
+    HIPCHECK(hipEventRecord(b->ev[0], s));
+    HIPCHECK(hipGraphLaunch(b->exec, s));
+    HIPCHECK(hipEventRecord(b->ev[1], s));
+    b->launched = true;
+    return MI_OK;
+}
+
+int mi_batch_wait(mi_batch* b, int64_t* hit_offsets, double* device_ms)
+{
+    if (!b) return fail(MI_ERR_ARG, "null batch");
+    if (!b->launched) return fail(MI_ERR_ARG, "batch not launched");
+    HIPCHECK(hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->last_ms = ms;
+    if (device_ms) *device_ms = ms;
+    if (hit_offsets)
+        for (int32_t e = 0; e <= b->n_event; ++e) hit_offsets[e] = b->h_out[e];
+    if (b->h_out[b->n_event + 1] != 0)
+        return fail(MI_ERR_ARG, "the gensteps' photon counts differ from those captured: results invalid");
+    if (b->hit_capacity > 0 && b->h_out[b->n_event] > b->hit_capacity)
+        return fail(MI_ERR_CAPACITY, "hit_capacity too small: " + std::to_string(b->h_out[b->n_event]) + " hits");
+    return MI_OK;
+}
+
+int64_t mi_batch_num_photon(const mi_batch* b) { return b ? b->n_photon : -1; }
+
+/* ----------------------------------------------------------- autotuner */
+
+int mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
+                    const void* d_gensteps, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                    int32_t reps, mi_tune_result* results, int32_t max_results, int32_t* n_results)
+{
+    if (!sim || !n_results) return fail(MI_ERR_ARG, "null argument");
+    *n_results = 0;
+    if (reps < 1) reps = 1;
+    static const int blocks[] = { 64, 128, 256 };
+    static const int refills[] = { 1, 4, 16 };
+    static const int leaves[] = { 1, 2, 4 };
+    const mi_config base = sim->cfg;
+    mi_config best = base;
+    double best_ms = 1e300;
+    std::vector<int64_t> po((size_t)n_event + 1), ho((size_t)n_event + 1);
+    int rc = MI_OK;
+    for (int leaf : leaves) {
+        for (int blk : blocks) {
+            for (int rf : refills) {
+                mi_config c = base;
+                c.bvh_leaf_size = leaf; c.block_size = blk; c.refill_period = rf;
+                rc = mi_sim_set_config(sim, &c);
+                if (rc) break;
+                rc = mi_sim_simulate_batch_device(sim, n_event, event_ids, gs_offsets, d_gensteps, po.data(), d_photon,
+                                                  d_seq, d_hit, hit_capacity, ho.data(), nullptr);
+                if (rc) break;
+                double tot = 0.0;
+                for (int r = 0; r < reps && !rc; ++r) {
+                    rc = mi_sim_simulate_batch_device(sim, n_event, event_ids, gs_offsets, d_gensteps, po.data(),
+                                                      d_photon, d_seq, d_hit, hit_capacity, ho.data(), nullptr);
+                    tot += sim->last_ms[3];
+                }
+                if (rc) break;
+                const double ms = tot / reps;
+                if (results && *n_results < max_results) {
+                    mi_tune_result& t = results[*n_results];
+                    t.block_size = blk; t.refill_period = rf; t.bvh_leaf_size = leaf; t.kernel = c.kernel; t.ms = ms;
+                }
+                ++*n_results;
+                if (ms < best_ms) { best_ms = ms; best = c; }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    if (rc) {
+        const std::string err = g_err;
+        (void)mi_sim_set_config(sim, &base);
+        g_err = err;
+        return rc;
+    }
+    return mi_sim_set_config(sim, &best);
 }
 
 int mi_sim_last_timing(const mi_sim* sim, double* ms4)

@@ -7,6 +7,7 @@ namespace oa_launch {
 hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s);
 hipError_t simulate(const SimParams& P, int kernel, int block, int grid, hipStream_t s);
 int wave_blocks_per_cu(int block);   /* resident blocks of the persistent kernel per CU */
+hipError_t prefix_check(const int64_t* a, const int64_t* b, int64_t n, unsigned long long* flag, hipStream_t s);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 hipError_t compact_hits(const unsigned long long* d_words, int64_t n_words, int32_t* d_word_prefix,
                         int64_t* d_block_total, const float4* d_photon, float4* d_hit, int64_t hit_cap,

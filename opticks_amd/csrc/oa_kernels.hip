@@ -477,6 +477,18 @@ __global__ void k_zero_u64(unsigned long long* p, int64_t n)
     if (i < n) p[i] = 0ull;
 }
 
+__global__ void k_prefix_check(const int64_t* a, const int64_t* b, int64_t n, unsigned long long* flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && a[i] != b[i]) flag[0] = 1ull;
+}
+
+hipError_t prefix_check(const int64_t* a, const int64_t* b, int64_t n, unsigned long long* flag, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_prefix_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n, flag);
+    return hipGetLastError();
+}
+
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);

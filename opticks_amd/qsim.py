@@ -265,6 +265,29 @@ class QSim:
         L.check(L.lib().mi_sim_last_adjoint_timing(self.handle, t), "mi_sim_last_adjoint_timing")
         return dict(replay_ms=t[0], reduce_ms=t[1], total_ms=t[2])
 
+    def batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None):
+        """Captured batch (hipGraph) over fixed buffers; see Batch."""
+        return Batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t, hit_t)
+
+    def autotune(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, reps=2):
+        """Pick block size, refill period and TLAS leaf size on these events; returns all candidates."""
+        ne = len(event_ids)
+        ev = np.ascontiguousarray(event_ids, np.int32)
+        go = np.ascontiguousarray(gs_offsets, np.int64)
+        res = (L.MiTuneResult * 64)()
+        n = C.c_int32(0)
+        cap = 0 if hit_t is None else hit_t.shape[0]
+        L.check(L.lib().mi_sim_autotune(self.handle, ne, ev.ctypes.data, go.ctypes.data, gs_t.data_ptr(),
+                                        photon_t.data_ptr(), None if seq_t is None else seq_t.data_ptr(),
+                                        None if hit_t is None else hit_t.data_ptr(), cap, reps, res, 64,
+                                        C.byref(n)), "mi_sim_autotune")
+        cfg = L.MiConfig()
+        L.check(L.lib().mi_sim_get_config(self.handle, C.byref(cfg)), "mi_sim_get_config")
+        self.config.block_size, self.config.refill_period = cfg.block_size, cfg.refill_period
+        self.config.bvh_leaf_size = cfg.bvh_leaf_size
+        return [dict(block_size=r.block_size, refill_period=r.refill_period, bvh_leaf_size=r.bvh_leaf_size,
+                     ms=r.ms) for r in res[:min(n.value, 64)]]
+
     def trace_device(self, rays_t, isect_t):
         L.check(L.lib().mi_sim_trace_device(self.handle, rays_t.data_ptr(), rays_t.shape[0], isect_t.data_ptr(),
                                             None), "mi_sim_trace_device")
@@ -272,3 +295,50 @@ class QSim:
     def refit_instances(self, inst):
         inst = np.ascontiguousarray(inst, np.float32).reshape(-1, 4, 4)
         L.check(L.lib().mi_sim_refit_instances(self.handle, inst.ctypes.data, len(inst)), "mi_sim_refit_instances")
+
+
+class Batch:
+    """Many events' launch sequence captured once into a hipGraph, replayed by launch().
+
+    The tensors passed at creation are the batch's inputs/outputs for every
+    replay; gensteps may change between replays as long as each genstep keeps
+    its photon count.
+    """
+
+    def __init__(self, sim: QSim, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None):
+        self.sim = sim
+        self.n_event = len(event_ids)
+        ev = np.ascontiguousarray(event_ids, np.int32)
+        go = np.ascontiguousarray(gs_offsets, np.int64)
+        self.photon_offsets = np.zeros(self.n_event + 1, np.int64)
+        self._keep = (gs_t, photon_t, seq_t, hit_t)
+        h = C.c_void_p()
+        cap = 0 if hit_t is None else hit_t.shape[0]
+        L.check(L.lib().mi_batch_create(sim.handle, self.n_event, ev.ctypes.data, go.ctypes.data, gs_t.data_ptr(),
+                                        photon_t.data_ptr(), None if seq_t is None else seq_t.data_ptr(),
+                                        None if hit_t is None else hit_t.data_ptr(), cap,
+                                        self.photon_offsets.ctypes.data, C.byref(h)), "mi_batch_create")
+        self.handle = h
+        self.num_photon = int(L.lib().mi_batch_num_photon(h))
+
+    def launch(self, stream=None):
+        L.check(L.lib().mi_batch_launch(self.handle, None if stream is None else C.c_void_p(stream.cuda_stream)),
+                "mi_batch_launch")
+
+    def wait(self):
+        """-> (hit_offsets (n_event+1,), device ms of the replay)"""
+        ho = np.zeros(self.n_event + 1, np.int64)
+        ms = C.c_double(0)
+        L.check(L.lib().mi_batch_wait(self.handle, ho.ctypes.data, C.byref(ms)), "mi_batch_wait")
+        return ho, ms.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().mi_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
