@@ -15,7 +15,7 @@ Also reported: the dominant kernel's roofline (HIP events on the kernel's
 stream), the CPU checker's rate on a bounded sample on this host, the
 adjoint/forward device-time ratio on the same geometry with 256 binned
 LS/water absorption and Rayleigh lengths (10M photons, 2,000 detectors), and
-the batched 64-event workload when --batched is given.
+the batched 64-event workload (autotuned, replayed from a captured hipGraph).
 
 --copy-calibration-mib M appends one device copy of M MiB after the timed
 region: a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE counters.
@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
-    ap.add_argument("--batched", action="store_true", help="also time the 64-event batched workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-event batched workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -228,40 +228,70 @@ def main():
                                 "sample": f"first {sample} photons of the same workload "
                                           f"({cdt:.1f} s wall on {threads} threads; C restatement of "
                                           f"qsim/CSG, OpenMP over photons, own median BVH)"}
-    if args.batched:
-        line["batched"] = bench_batched(sim, torch, dev)
+    if not args.no_batched:
+        line["batched"] = bench_batched(torch, dev, max(1, min(args.steps, 3)))
     print(json.dumps(line), flush=True)
     sim.close()
 
 
-def bench_batched(sim_unused, torch, dev):
+def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
     """64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons in
-    the 20,000-detector geometry, one persistent launch per step."""
+    the 20,000-detector geometry: autotune (block, refill period, TLAS leaf) on
+    the first events, capture the whole batch into a hipGraph, time replays.
+    History agreement: seq of the first events against the plain kernel run
+    per event (one launch per event, one lane per photon)."""
     from opticks_amd import configs
+    from opticks_amd import _lib as L
     from opticks_amd.qsim import QSim, SEventConfig
     w = configs.config2()
     N = w.num_photon
     gs_all = np.concatenate(w.events)
     offs = np.cumsum([0] + [len(e) for e in w.events])
+    ev = list(range(len(w.events)))
     sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
     gs = torch.from_numpy(gs_all).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
     hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
-    ev = list(range(len(w.events)))
-    sim.simulate_batch(gs, offs, ev, photon, seq, hit)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    steps = 2
-    for _ in range(steps):
-        po_, ho = sim.simulate_batch(gs, offs, ev, photon, seq, hit)
+    tune = sim.autotune(gs, offs[:tune_events + 1], ev[:tune_events], photon, seq, hit, reps=1)
+    tune_s = time.perf_counter() - t0
+    best = min(tune, key=lambda r: r["ms"])
+    b = sim.batch(gs, offs, ev, photon, seq, hit)
+    b.launch(); b.wait()
     torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+        ho, ms = b.wait()
+        dev_ms.append(ms)
     dt = (time.perf_counter() - t0) / steps
-    t = sim.last_timing()
+    # agreement with the plain per-event form on the first events
+    S = seq[: int(b.photon_offsets[check_events])].cpu().numpy()
+    plain = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=L.MI_KERNEL_PLAIN))
+    same = total = 0
+    for e in range(check_events):
+        a0, a1 = int(b.photon_offsets[e]), int(b.photon_offsets[e + 1])
+        ge = torch.from_numpy(w.events[e]).to(dev)
+        sq = torch.empty((a1 - a0, 2, 2), dtype=torch.int64, device=dev)
+        ph = torch.empty((a1 - a0, 4, 4), dtype=torch.float32, device=dev)
+        plain.simulate_device(ge, ph, sq, None, eventID=e, n_photon=a1 - a0)
+        same += int(np.all(sq.cpu().numpy() == S[a0:a1], axis=(1, 2)).sum())
+        total += a1 - a0
+    plain.close()
+    b.close()
     sim.close()
-    return {"workload": "64 events, 20000 detectors at 19.4 m, LS r 17.7 m", "n_photon": int(N),
-            "photons_per_s": N / dt, "ms_per_step": dt * 1e3, "propagate_ms": t["propagate_ms"],
-            "hits": int(ho[-1])}
+    mean_ms = float(np.mean(dev_ms))
+    return {"workload": "64 events (Poisson(200) scintillation gensteps x Poisson(5000) photons), 20000 CSG "
+                        "detectors at 19.4 m, LS r 17.7 m; one captured hipGraph replay per step",
+            "n_photon": int(N), "n_event": len(ev), "photons_per_s": N / (mean_ms * 1e-3),
+            "photons_per_s_wall": N / dt, "device_ms_per_step": mean_ms, "wall_ms_per_step": dt * 1e3,
+            "hits": int(ho[-1]),
+            "autotune": {"events": tune_events, "seconds": tune_s, "candidates": len(tune),
+                         "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size")},
+                         "chosen_ms": best["ms"], "slowest_ms": max(r["ms"] for r in tune)},
+            "seqhis_agreement_vs_plain_per_event": same / max(total, 1), "agreement_photons": total}
 
 
 if __name__ == "__main__":

@@ -226,6 +226,8 @@ int  mi_sim_set_bnd(mi_sim* sim, const float* bnd);
  * mat_block (host, n_mat+1): block of each 1-based material index, -1 = none.
  * sample_bin (host, n_wl): bin of each wavelength sample, in [0, n_bin).
  * n_det: detectors, hits carry sensor identifiers 1..n_det (others ignored).
+ * max_row_floats: device scratch for score rows per chunk of hits, in floats
+ * (0 = 2^30, i.e. 4 GiB); hits are processed in chunks that fit it.
  */
 typedef struct mi_adjoint {
     const int32_t* mat_block;
@@ -234,6 +236,7 @@ typedef struct mi_adjoint {
     int32_t n_bin;
     int32_t n_block;
     int32_t n_det;
+    int64_t max_row_floats;
 } mi_adjoint;
 
 /*
@@ -245,7 +248,8 @@ typedef struct mi_adjoint {
  * is spread over the bins of the two table samples each lookup interpolates.
  * d_grad (n_det, n_block*2*n_bin) f32 receives d E[hits of detector d]/d theta,
  * summed per detector in photon order with Kahan compensation.
- * Device scratch: n_hit*n_param*4 bytes of score rows plus sort buffers.
+ * Device scratch: min(n_hit*n_param, max_row_floats) floats of score rows,
+ * n_det*n_param floats of Kahan compensation, sort buffers.
  * Synchronous on return; mi_sim_last_adjoint_timing gives [replay, reduce, total] ms.
  */
 int  mi_sim_adjoint_device(mi_sim* sim, int32_t event_id,

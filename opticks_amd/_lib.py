@@ -52,7 +52,8 @@ class MiEventOut(C.Structure):
 
 class MiAdjoint(C.Structure):
     _fields_ = [("mat_block", C.c_void_p), ("n_mat", C.c_int32), ("sample_bin", C.c_void_p),
-                ("n_bin", C.c_int32), ("n_block", C.c_int32), ("n_det", C.c_int32)]
+                ("n_bin", C.c_int32), ("n_block", C.c_int32), ("n_det", C.c_int32),
+                ("max_row_floats", C.c_int64)]
 
 
 class MiTuneResult(C.Structure):

@@ -98,9 +98,9 @@ class LengthParams:
                 out[b, j, 0, :, col] = theta[blk, p, self.sample_bin]
         return out
 
-    def c_struct(self, n_det: int) -> L.MiAdjoint:
+    def c_struct(self, n_det: int, max_row_floats: int = 0) -> L.MiAdjoint:
         a = L.MiAdjoint(self.mat_block.ctypes.data, self.n_mat, self.sample_bin.ctypes.data, self.n_bin,
-                        self.n_block, int(n_det))
+                        self.n_block, int(n_det), int(max_row_floats))
         a._keep = (self.mat_block, self.sample_bin)
         return a
 
@@ -182,10 +182,11 @@ def expected_hits(theta, sim, params: LengthParams, gs_t, n_det: int, event_id: 
     return _EH.apply(theta, sim, params, gs_t, int(n_det), int(event_id), int(n_photon))
 
 
-def yield_jacobian(sim, params: LengthParams, gs_t, hit_t, n_hit, n_det, event_id=0, n_photon=-1, stream=None):
+def yield_jacobian(sim, params: LengthParams, gs_t, hit_t, n_hit, n_det, event_id=0, n_photon=-1, stream=None,
+                   max_row_floats=0):
     """(n_det, n_param) d E[hits_d]/d theta for an event already simulated with sim."""
     import torch
     G = torch.empty((n_det, params.n_param), dtype=torch.float32, device=gs_t.device)
-    sim.adjoint_device(gs_t, hit_t, n_hit, params.c_struct(n_det), G, eventID=event_id, n_photon=n_photon,
-                       stream=stream)
+    sim.adjoint_device(gs_t, hit_t, n_hit, params.c_struct(n_det, max_row_floats), G, eventID=event_id,
+                       n_photon=n_photon, stream=stream)
     return G

@@ -149,7 +149,7 @@ struct mi_sim {
     std::vector<uint64_t> h_seq;
     /* adjoint */
     DBuf<int32_t> adj_mat_block, adj_sample_bin;
-    DBuf<float> adj_rows;
+    DBuf<float> adj_rows, adj_comp;
     DBuf<uint32_t> adj_key, adj_val, adj_key2, adj_val2;
     DBuf<unsigned char> adj_tmp;
     DBuf<int64_t> adj_seg;
@@ -795,6 +795,9 @@ int mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst)
     return MI_OK;
 }
 
+/* score-row scratch limit of one adjoint chunk, in floats (4 GiB) */
+static const int64_t OA_ADJ_ROWS_MAX = (int64_t)1 << 30;
+
 int mi_sim_set_bnd(mi_sim* sim, const float* bnd)
 {
     if (!sim || !bnd) return fail(MI_ERR_ARG, "null sim or bnd");
@@ -851,28 +854,42 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     P.n_photon = n_photon;
     P.photon_offset = photon_offset;
     const uint32_t n_det = (uint32_t)adj->n_det;
-    const size_t tmp_bytes = std::max<size_t>(1, oa_launch::adjoint_sort_bytes(n_hit, n_det));
-    if (sim->adj_rows.reserve((size_t)(n_hit * n_param)) != hipSuccess)
-        return fail(MI_ERR_CAPACITY, "adjoint: cannot allocate " + std::to_string(n_hit * n_param * 4) + " bytes of score rows");
-    HIPCHECK(sim->adj_key.reserve((size_t)n_hit));
-    HIPCHECK(sim->adj_val.reserve((size_t)n_hit));
-    HIPCHECK(sim->adj_key2.reserve((size_t)n_hit));
-    HIPCHECK(sim->adj_val2.reserve((size_t)n_hit));
+    /* hits are processed in chunks of consecutive hits (photon order) whose
+       score rows fit OA_ADJ_ROWS_MAX floats; the per-detector Kahan sums
+       carry their compensation from chunk to chunk */
+    const int64_t rows_max = adj->max_row_floats > 0 ? adj->max_row_floats : OA_ADJ_ROWS_MAX;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(n_hit, 1), rows_max / n_param));
+    const size_t tmp_bytes = std::max<size_t>(1, oa_launch::adjoint_sort_bytes(chunk, n_det));
+    if (sim->adj_rows.reserve((size_t)(chunk * n_param)) != hipSuccess)
+        return fail(MI_ERR_CAPACITY, "adjoint: cannot allocate " + std::to_string(chunk * n_param * 4) + " bytes of score rows");
+    HIPCHECK(sim->adj_comp.reserve((size_t)n_det * n_param));
+    HIPCHECK(sim->adj_key.reserve((size_t)chunk));
+    HIPCHECK(sim->adj_val.reserve((size_t)chunk));
+    HIPCHECK(sim->adj_key2.reserve((size_t)chunk));
+    HIPCHECK(sim->adj_val2.reserve((size_t)chunk));
     HIPCHECK(sim->adj_tmp.reserve(tmp_bytes));
     HIPCHECK(sim->adj_seg.reserve((size_t)n_det + 1));
-    HIPCHECK(hipEventRecord(sim->ev[0], s));
-    HIPCHECK(oa_launch::adjoint_replay(P, sim->adj_mat_block.p, adj->n_mat, sim->adj_sample_bin.p, adj->n_bin, n_param,
-                                       (const float4*)d_hit, n_hit, sim->adj_rows.p, s));
-    HIPCHECK(hipEventRecord(sim->ev[1], s));
-    HIPCHECK(oa_launch::adjoint_reduce((const float4*)d_hit, n_hit, n_det, n_param, sim->adj_rows.p, sim->adj_key.p,
-                                       sim->adj_val.p, sim->adj_key2.p, sim->adj_val2.p, sim->adj_tmp.p, tmp_bytes,
-                                       sim->adj_seg.p, (float*)d_grad, s));
-    HIPCHECK(hipEventRecord(sim->ev[2], s));
-    HIPCHECK(hipEventSynchronize(sim->ev[2]));
-    float a = 0.f, b = 0.f;
-    HIPCHECK(hipEventElapsedTime(&a, sim->ev[0], sim->ev[1]));
-    HIPCHECK(hipEventElapsedTime(&b, sim->ev[1], sim->ev[2]));
-    sim->adj_ms[0] = a; sim->adj_ms[1] = b; sim->adj_ms[2] = (double)a + b;
+    double replay_ms = 0.0, reduce_ms = 0.0;
+    int64_t h0 = 0;
+    do {
+        const int64_t nh = std::min<int64_t>(chunk, n_hit - h0);
+        const float4* hits = n_hit > 0 ? (const float4*)d_hit + 4 * h0 : nullptr;
+        HIPCHECK(hipEventRecord(sim->ev[0], s));
+        HIPCHECK(oa_launch::adjoint_replay(P, sim->adj_mat_block.p, adj->n_mat, sim->adj_sample_bin.p, adj->n_bin,
+                                           n_param, hits, nh, sim->adj_rows.p, s));
+        HIPCHECK(hipEventRecord(sim->ev[1], s));
+        HIPCHECK(oa_launch::adjoint_reduce(hits, nh, n_det, n_param, sim->adj_rows.p, sim->adj_key.p, sim->adj_val.p,
+                                           sim->adj_key2.p, sim->adj_val2.p, sim->adj_tmp.p, tmp_bytes, sim->adj_seg.p,
+                                           (float*)d_grad, sim->adj_comp.p, h0 == 0 ? 1 : 0, s));
+        HIPCHECK(hipEventRecord(sim->ev[2], s));
+        HIPCHECK(hipEventSynchronize(sim->ev[2]));
+        float a = 0.f, b = 0.f;
+        HIPCHECK(hipEventElapsedTime(&a, sim->ev[0], sim->ev[1]));
+        HIPCHECK(hipEventElapsedTime(&b, sim->ev[1], sim->ev[2]));
+        replay_ms += a; reduce_ms += b;
+        h0 += nh;
+    } while (h0 < n_hit);
+    sim->adj_ms[0] = replay_ms; sim->adj_ms[1] = reduce_ms; sim->adj_ms[2] = replay_ms + reduce_ms;
     return MI_OK;
 }
 

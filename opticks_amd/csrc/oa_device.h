@@ -27,7 +27,16 @@
 __device__ unsigned long long oa_stats[OA_ST_N];
 #define OA_COUNT(i) atomicAdd(&oa_stats[(i)], 1ull)
 #define OA_COUNT_N(i, n) atomicAdd(&oa_stats[(i)], (unsigned long long)(n))
+#define OA_WAVE_COUNT(w, l)                                                                   \
+    do {                                                                                      \
+        const unsigned long long m_ = __ballot(1);                                            \
+        if ((int)__lane_id() == __ffsll((long long)m_) - 1) {                                 \
+            atomicAdd(&oa_stats[(w)], 1ull);                                                  \
+            atomicAdd(&oa_stats[(l)], (unsigned long long)__popcll(m_));                      \
+        }                                                                                     \
+    } while (0)
 #else
+#define OA_WAVE_COUNT(w, l) ((void)0)
 #define OA_COUNT(i) ((void)0)
 #define OA_COUNT_N(i, n) ((void)0)
 #endif

@@ -23,6 +23,6 @@ hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_
                           int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t n_hit, float* d_S, hipStream_t s);
 hipError_t adjoint_reduce(const float4* d_hit, int64_t n_hit, uint32_t n_det, int32_t n_param, const float* d_S,
                           uint32_t* d_key, uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp,
-                          size_t tmp_bytes, int64_t* d_seg, float* d_G, hipStream_t s);
+                          size_t tmp_bytes, int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s);
 size_t adjoint_sort_bytes(int64_t n_hit, uint32_t n_det);
 }

@@ -129,6 +129,7 @@ OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk, Tape& tape)
     if (prd.boundary == 0xffffu) return true;
     prd.normal = normalize(prd.normal);
     OA_COUNT(OA_ST_STEP);
+    OA_WAVE_COUNT(OA_ST_W_PHYS, OA_ST_L_PHYS);
     const int command = propagate(P, L.rng, L.p, prd, tape);
     L.bounce++;
     seq_add(L, (unsigned)L.bounce, ph_flag(L.p), ph_boundary(L.p));
@@ -615,21 +616,25 @@ __global__ void k_det_segments(const uint32_t* key, int64_t n, uint32_t n_det, i
     seg[d] = lo;
 }
 
-/* G[d,k] = Kahan sum over the detector's hits (in photon order) of S[h,k] */
+/* G[d,k] = Kahan sum over the detector's hits (in photon order) of S[h,k];
+   with first == 0 the sum and its compensation C continue from an earlier
+   chunk of hits (chunks are consecutive in photon order) */
 __global__ void __launch_bounds__(256) k_det_reduce(const float* S, const uint32_t* order, const int64_t* seg,
-                                                    int32_t n_param, float* G)
+                                                    int32_t n_param, float* G, float* C, int first)
 {
     const int d = blockIdx.x;
     const int64_t b = seg[d], e = seg[d + 1];
     for (int k = threadIdx.x; k < n_param; k += blockDim.x) {
-        float sum = 0.f, c = 0.f;
+        const int64_t o = (int64_t)d * n_param + k;
+        float sum = first ? 0.f : G[o], c = first ? 0.f : C[o];
         for (int64_t i = b; i < e; ++i) {
             const float y = S[(int64_t)order[i] * n_param + k] - c;
             const float t = sum + y;
             c = (t - sum) - y;
             sum = t;
         }
-        G[(int64_t)d * n_param + k] = sum;
+        G[o] = sum;
+        C[o] = c;
     }
 }
 
@@ -658,7 +663,7 @@ static int det_key_bits(uint32_t n_det)
 
 hipError_t adjoint_reduce(const float4* d_hit, int64_t n_hit, uint32_t n_det, int32_t n_param, const float* d_S,
                           uint32_t* d_key, uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp,
-                          size_t tmp_bytes, int64_t* d_seg, float* d_G, hipStream_t s)
+                          size_t tmp_bytes, int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s)
 {
     if (n_hit > 0) {
         hipLaunchKernelGGL(k_hit_det, dim3((unsigned)((n_hit + 255) / 256)), dim3(256), 0, s, d_hit, n_hit, n_det, d_key, d_val);
@@ -667,7 +672,7 @@ hipError_t adjoint_reduce(const float4* d_hit, int64_t n_hit, uint32_t n_det, in
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_det_segments, dim3((n_det + 1 + 255) / 256), dim3(256), 0, s, d_key2, n_hit, n_det, d_seg);
-    hipLaunchKernelGGL(k_det_reduce, dim3(n_det), dim3(256), 0, s, d_S, d_val2, d_seg, n_param, d_G);
+    hipLaunchKernelGGL(k_det_reduce, dim3(n_det), dim3(256), 0, s, d_S, d_val2, d_seg, n_param, d_G, d_C, first);
     return hipGetLastError();
 }
 

@@ -26,9 +26,14 @@
 /* Optional work counters (build with -DOA_STATS): traces, TLAS node visits,
    instance tests, BLAS node visits, prim tests, tree prim tests, leaf
    evaluations inside trees, propagation steps, wave loop iterations, active
-   lanes summed over wave loop iterations. Off in normal builds. */
+   lanes summed over wave loop iterations, and for the trace entry, the TLAS
+   walk, the leaf (instance/BLAS) walk, prim tests and physics the number of
+   wave executions and the lanes active in them. Off in normal builds. */
 enum { OA_ST_TRACE, OA_ST_TLAS, OA_ST_INST, OA_ST_BLAS, OA_ST_PRIM, OA_ST_TREE, OA_ST_LEAF, OA_ST_STEP,
-       OA_ST_ITER, OA_ST_ACTIVE, OA_ST_N };
+       OA_ST_ITER, OA_ST_ACTIVE,
+       /* wave-level pairs: loop iterations executed by the wave, and lanes active in them */
+       OA_ST_W_TRACE, OA_ST_L_TRACE, OA_ST_W_P1, OA_ST_L_P1, OA_ST_W_P2, OA_ST_L_P2,
+       OA_ST_W_PRIM, OA_ST_L_PRIM, OA_ST_W_PHYS, OA_ST_L_PHYS, OA_ST_N };
 
 struct BvhNode {
     float lo0[3], hi0[3];

@@ -524,6 +524,7 @@ template <int STRIDE>
 OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float tmax, Prd& prd)
 {
     OA_COUNT(OA_ST_TRACE);
+    OA_WAVE_COUNT(OA_ST_W_TRACE, OA_ST_L_TRACE);
     const v3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
     float best_t = tmax;
     uint64_t best_key = ~0ull;
@@ -536,6 +537,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     while (!done) {
         /* phase 1: walk nodes until this lane has a leaf or nothing left */
         while (!done && leaf == OA_BVH_EMPTY) {
+            OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
             if (cur == OA_BVH_EMPTY) {
                 if (sp == 0) { done = true; break; }
                 sp--;
@@ -562,6 +564,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             const int sp_base = sp;
             uint32_t bc = __float_as_uint(m[2].w);     /* BLAS root node of this solid */
             for (;;) {
+                OA_WAVE_COUNT(OA_ST_W_P2, OA_ST_L_P2);
                 if (bc == OA_BVH_EMPTY) {
                     if (sp == sp_base) break;
                     sp--;
@@ -579,6 +582,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                     const PrimRec pr = P.prim[pi];
                     Isect is;
                     OA_COUNT(OA_ST_PRIM);
+                    OA_WAVE_COUNT(OA_ST_W_PRIM, OA_ST_L_PRIM);
                     if (!intersect_prim(is, P.node + 4u * pr.node_offset, P.itra, tmin, oo, od)) continue;
                     const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
                     if (is.w < best_t || (is.w == best_t && key < best_key)) {

@@ -70,6 +70,10 @@ def test_adjoint_matches_oracle_rows():
     np.add.at(A64, det, np.abs(rows))
     err = np.abs(G - G64)
     assert np.all(err <= 1e-5 * A64 + 1e-30), err.max()
+    # chunks of 37 hits (score rows 37 x 256 floats) give the same sums up to rounding
+    Gc = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, n_photon=N, max_row_floats=37 * 256)
+    errc = np.abs(Gc.cpu().numpy().astype(np.float64) - G64)
+    assert np.all(errc <= 1e-5 * A64 + 1e-30), errc.max()
     assert np.count_nonzero(G) > 1000
     # the 256 parameters in order: LS absorption, LS Rayleigh, water absorption, water Rayleigh
     assert np.abs(G).sum(axis=0).reshape(2, 2, 64).sum(axis=2).min() > 0
@@ -95,6 +99,9 @@ def test_adjoint_closed_form_water_sphere():
     sim.close()
 
 
+FD_EVENTS = 10          # 10 events x 40M photons = 400M photons per FD evaluation
+
+
 @pytest.fixture(scope="module")
 def fd_setup():
     w = configs.config3(num_gs=4000, per_gs=10_000)
@@ -107,23 +114,26 @@ def fd_setup():
 
 
 def test_adjoint_vs_central_fd(fd_setup):
+    """Central differences of total hits along parameter directions v (theta -> theta(1 + s v), s = +-1%),
+    same seeds and photon streams on both sides; 400M photons as 10 events with their own streams."""
     w, params, theta, sim, gs_t = fd_setup
     N = w.num_photon
     base_bnd = sim.base_bnd
     h = 0.01
-    hit, n_hit = run_forward(sim, gs_t, N)
-    G = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, n_photon=N)
-    Gtot = G.sum(dim=0).double().cpu().numpy().reshape(params.shape)
-    # adjoint error bar: 8 disjoint photon subsets of the hits
-    idx = hit_photon_index(hit, n_hit)
-    parts = []
-    for k in range(8):
-        sel = hit[:n_hit][(idx % 8) == k].contiguous()
-        Gk = yield_jacobian(sim, params, gs_t, sel, sel.shape[0], N_DET, n_photon=N)
-        parts.append(Gk.sum(dim=0).double().cpu().numpy().reshape(params.shape))
-    parts = np.stack(parts)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    Gtot = np.zeros(params.shape)
+    parts = np.zeros((8,) + params.shape)     # adjoint error bar: 8 disjoint photon subsets
+    for ev in range(FD_EVENTS):
+        n_hit = sim.simulate_device(gs_t, photon, None, hit, eventID=ev, n_photon=N)
+        G = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, event_id=ev, n_photon=N)
+        Gtot += G.sum(dim=0).double().cpu().numpy().reshape(params.shape)
+        idx = hit_photon_index(hit, n_hit)
+        for k in range(8):
+            sel = hit[:n_hit][(idx % 8) == k].contiguous()
+            Gk = yield_jacobian(sim, params, gs_t, sel, sel.shape[0], N_DET, event_id=ev, n_photon=N)
+            parts[k] += Gk.sum(dim=0).double().cpu().numpy().reshape(params.shape)
     assert np.allclose(parts.sum(axis=0), Gtot, rtol=1e-4, atol=1e-6 * np.abs(Gtot).max())
-    del hit
 
     rng = np.random.default_rng(7)
     directions = {
@@ -142,27 +152,30 @@ def test_adjoint_vs_central_fd(fd_setup):
         adj = float((Gtot * dtheta).sum())
         adj_parts = (parts * dtheta).sum(axis=(1, 2, 3))
         sig_adj = float(adj_parts.std(ddof=1) * np.sqrt(len(adj_parts)))
-        masks = []
-        for sgn in (+1, -1):
-            sim.set_bnd(params.apply(base_bnd, (theta * (1.0 + sgn * h * v)).astype(np.float32)))
-            hp, nh = run_forward(sim, gs_t, N)
-            m = torch.zeros(N, dtype=torch.bool, device=DEV)
-            m[hit_photon_index(hp, nh)] = True
-            masks.append(m)
-            del hp
-        fd = (masks[0].sum().item() - masks[1].sum().item()) / (2 * h)
-        n_changed = (masks[0] ^ masks[1]).sum().item()
+        dH, n_changed = 0, 0
+        bnds = [params.apply(base_bnd, (theta * (1.0 + sgn * h * v)).astype(np.float32)) for sgn in (+1, -1)]
+        for ev in range(FD_EVENTS):
+            masks = []
+            for bnd in bnds:
+                sim.set_bnd(bnd)
+                nh = sim.simulate_device(gs_t, photon, None, hit, eventID=ev, n_photon=N)
+                m = torch.zeros(N, dtype=torch.bool, device=DEV)
+                m[hit_photon_index(hit, nh)] = True
+                masks.append(m)
+            dH += masks[0].sum().item() - masks[1].sum().item()
+            n_changed += (masks[0] ^ masks[1]).sum().item()
+        fd = dH / (2 * h)
         sig_fd = np.sqrt(n_changed) / (2 * h)
         sig = np.hypot(sig_fd, sig_adj)
         report[name] = (adj, fd, sig_adj, sig_fd)
         print(f"FD check {name:13s} adjoint {adj:12.1f} +- {sig_adj:8.1f}  fd {fd:12.1f} +- {sig_fd:8.1f}"
-              f"  rel diff {(adj - fd) / fd:+.4f}", flush=True)
+              f"  rel diff {(adj - fd) / fd:+.4f}  ({FD_EVENTS * N / 1e6:.0f}M photons)", flush=True)
         assert abs(adj - fd) < 4 * sig + 1e-9, (name, adj, fd, sig)
     sim.set_bnd(base_bnd)
-    # along the directions that carry the yield's sensitivity, agreement within 1%
+    # the directions that carry the yield's sensitivity: within 1% (FD error there is 0.3-0.6%)
     for name in ("all", "LS abs", "water abs"):
         adj, fd, sa, sf = report[name]
-        assert abs(adj - fd) <= 0.01 * abs(fd) + 3 * np.hypot(sa, sf), (name, adj, fd)
+        assert abs(adj - fd) <= 0.01 * abs(fd), (name, adj, fd, sa, sf)
 
 
 def test_autograd_wrapper_matches_jacobian():
