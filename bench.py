@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-event batched workload")
+    ap.add_argument("--no-full", action="store_true", help="skip the 256-event forward+adjoint graph")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -230,6 +231,8 @@ def main():
                                           f"qsim/CSG, OpenMP over photons, own median BVH)"}
     if not args.no_batched:
         line["batched"] = bench_batched(torch, dev, max(1, min(args.steps, 3)))
+    if not args.no_full:
+        line["full_batched_adjoint"] = bench_full(torch, dev, max(1, min(args.steps, 2)))
     print(json.dumps(line), flush=True)
     sim.close()
 
@@ -292,6 +295,48 @@ def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
                          "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size")},
                          "chosen_ms": best["ms"], "slowest_ms": max(r["ms"] for r in tune)},
             "seqhis_agreement_vs_plain_per_event": same / max(total, 1), "agreement_photons": total}
+
+
+def bench_full(torch, dev, steps):
+    """256 events (Poisson(200) scintillation gensteps x Poisson(5000) photons, about
+    256M photons) in the 20,000-detector geometry with the 256 binned lengths:
+    forward hits and the yield adjoint of every hit in one captured graph."""
+    from opticks_amd import configs
+    from opticks_amd.qsim import QSim, SEventConfig
+    w = configs.config4()
+    params = w.meta["params"]
+    n_det = w.meta["num_det"]
+    N = w.num_photon
+    gs_all = np.concatenate(w.events)
+    offs = np.cumsum([0] + [len(e) for e in w.events])
+    ev = list(range(len(w.events)))
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, block_size=64))
+    gs = torch.from_numpy(gs_all).to(dev)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
+    cap = int(0.2 * N)
+    hit = torch.empty((cap, 4, 4), dtype=torch.float32, device=dev)
+    G = torch.empty((n_det, params.n_param), dtype=torch.float32, device=dev)
+    b = sim.batch(gs, offs, ev, photon, seq, hit, adjoint=params.c_struct(n_det), grad_t=G)
+    b.launch(); b.wait()
+    fwd, adj, tot = [], [], []
+    for _ in range(steps):
+        b.launch()
+        ho, ms = b.wait()
+        t = b.last_timing()
+        fwd.append(t["forward_ms"]); adj.append(t["adjoint_ms"]); tot.append(ms)
+    steps_per_photon = mean_steps(seq[: min(N, 20_000_000)].cpu().numpy().view(np.uint64))
+    b.close()
+    sim.close()
+    f, a, T = float(np.mean(fwd)), float(np.mean(adj)), float(np.mean(tot))
+    B = algorithmic_bytes(N, len(gs_all), steps_per_photon, 0.0)
+    return {"workload": "256 events (Poisson(200) scintillation gensteps x Poisson(5000) photons), 20000 CSG "
+                        "detectors, LS/water lengths as 4x64 bins; forward + adjoint in one captured graph",
+            "n_photon": int(N), "n_event": len(ev), "hits": int(ho[-1]), "n_param": int(params.n_param),
+            "n_det": int(n_det), "forward_ms": f, "adjoint_ms": a, "graph_ms": T,
+            "photons_per_s_forward": N / (f * 1e-3), "photons_per_s_forward_plus_adjoint": N / (T * 1e-3),
+            "adjoint_forward_ratio": a / f, "steps_per_photon": steps_per_photon,
+            "forward_alg_hbm_gb_s": B / (f * 1e-3) / 1e9}
 
 
 if __name__ == "__main__":

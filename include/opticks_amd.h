@@ -277,9 +277,24 @@ int  mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids,
                      const int64_t* gs_offsets, const void* d_gensteps,
                      void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
                      int64_t* photon_offsets, mi_batch** out);
+/*
+ * Same, with the path-replay adjoint of every hit of the batch captured into
+ * the same graph after the forward part (hipGraph child graphs joined by an
+ * event node): d_grad (adj->n_det, n_param) receives d E[hits per detector]/d theta
+ * summed over all events of the batch. hit_capacity also bounds the adjoint's
+ * hit slots; score rows use min(hit_capacity, max_row_floats/n_param) hits per
+ * chunk, and the chunk loop is fixed at capture, reading the actual hit count on
+ * the device. mi_batch_last_timing gives [forward, adjoint] ms of the last replay.
+ */
+int  mi_batch_create_adjoint(mi_sim* sim, int32_t n_event, const int32_t* event_ids,
+                             const int64_t* gs_offsets, const void* d_gensteps,
+                             void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                             const mi_adjoint* adj, void* d_grad,
+                             int64_t* photon_offsets, mi_batch** out);
 int  mi_batch_launch(mi_batch* b, void* stream);                 /* asynchronous */
 int  mi_batch_wait(mi_batch* b, int64_t* hit_offsets, double* device_ms);
 int64_t mi_batch_num_photon(const mi_batch* b);
+int  mi_batch_last_timing(const mi_batch* b, double* ms2);
 int  mi_batch_destroy(mi_batch* b);
 
 /*

@@ -18,6 +18,7 @@ The benchmark workloads, built in code (the reference ships no data fixtures).
            around a 17.7 m LS sphere.
   config3  config1 with LS and water absorption/Rayleigh lengths as 64 bins over
            200-800 nm each (256 parameters), each bin x(1+U(-0.2,0.2)) (seed 42).
+  config4  256 events as config2 (about 256M photons) with the config3 parameterisation.
 
 Choices the workload statements leave open (documented, fixed here):
   * world box half-side 6500 mm (config1) / 21000 mm (config2), inner surface absorbs;
@@ -222,6 +223,20 @@ def config3(num_gs=1000, per_gs=10_000, seed=42, n_bin=64):
     theta = perturbed_theta(params, CONFIG3_BASE_LENGTHS, seed=seed, frac=0.2)
     w.physics["bnd"] = params.apply(w.physics["bnd"], theta)
     w.name = "config3"
+    w.meta.update(params=params, theta=theta)
+    return w
+
+
+def config4(num_events=256, mean_gs=200, mean_photons=5000, seed=42, n_bin=64):
+    """config2 geometry (20,000 detectors at 19.4 m, LS r 17.7 m) with num_events
+    Poisson(mean_gs) x Poisson(mean_photons) scintillation events and the LS/water
+    absorption and Rayleigh lengths as n_bin bins each (x(1+U(-0.2,0.2)), seed)."""
+    from .adjoint import LengthParams, perturbed_theta
+    w = config2(num_events=num_events, mean_gs=mean_gs, mean_photons=mean_photons, seed=seed)
+    params = LengthParams(w.physics, ("LS", "Water"), n_bin=n_bin, lo=200.0, hi=800.0)
+    theta = perturbed_theta(params, CONFIG3_BASE_LENGTHS, seed=seed, frac=0.2)
+    w.physics["bnd"] = params.apply(w.physics["bnd"], theta)
+    w.name = "config4"
     w.meta.update(params=params, theta=theta)
     return w
 

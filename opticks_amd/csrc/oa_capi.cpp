@@ -807,15 +807,8 @@ int mi_sim_set_bnd(mi_sim* sim, const float* bnd)
     return MI_OK;
 }
 
-int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
-                          int64_t photon_offset, const void* d_hit, int64_t n_hit, const mi_adjoint* adj,
-                          void* d_grad, void* stream)
+static int check_adjoint(const mi_sim* sim, const mi_adjoint* adj)
 {
-    if (!sim || !adj) return fail(MI_ERR_ARG, "null sim or adjoint description");
-    if (!d_grad) return fail(MI_ERR_ARG, "null gradient buffer");
-    if (n_gs <= 0 || !d_gensteps) return fail(MI_ERR_ARG, "the event's gensteps are required");
-    if (n_hit < 0 || (n_hit > 0 && !d_hit)) return fail(MI_ERR_ARG, "bad hit array");
-    if (n_hit > (int64_t)INT32_MAX) return fail(MI_ERR_CAPACITY, "more than 2^31-1 hits in one adjoint call");
     if (adj->n_bin <= 0 || adj->n_block <= 0 || adj->n_det <= 0 || adj->n_det > 0xffff || adj->n_mat < 0)
         return fail(MI_ERR_ARG, "adjoint: n_bin, n_block must be > 0, n_det in [1, 65535], n_mat >= 0");
     if (!adj->mat_block || !adj->sample_bin) return fail(MI_ERR_ARG, "adjoint: null mat_block or sample_bin");
@@ -827,7 +820,21 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
             return fail(MI_ERR_ARG, "adjoint: sample_bin entry out of [0, n_bin)");
     const int64_t n_param64 = (int64_t)adj->n_block * 2 * adj->n_bin;
     if (n_param64 > (1 << 20)) return fail(MI_ERR_ARG, "adjoint: too many parameters");
-    const int32_t n_param = (int32_t)n_param64;
+    return MI_OK;
+}
+
+int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
+                          int64_t photon_offset, const void* d_hit, int64_t n_hit, const mi_adjoint* adj,
+                          void* d_grad, void* stream)
+{
+    if (!sim || !adj) return fail(MI_ERR_ARG, "null sim or adjoint description");
+    if (!d_grad) return fail(MI_ERR_ARG, "null gradient buffer");
+    if (n_gs <= 0 || !d_gensteps) return fail(MI_ERR_ARG, "the event's gensteps are required");
+    if (n_hit < 0 || (n_hit > 0 && !d_hit)) return fail(MI_ERR_ARG, "bad hit array");
+    if (n_hit > (int64_t)INT32_MAX) return fail(MI_ERR_CAPACITY, "more than 2^31-1 hits in one adjoint call");
+    int arc = check_adjoint(sim, adj);
+    if (arc) return arc;
+    const int32_t n_param = adj->n_block * 2 * adj->n_bin;
     hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
     HIPCHECK(upload(sim->adj_mat_block, adj->mat_block, (size_t)adj->n_mat + 1, s));
     HIPCHECK(upload(sim->adj_sample_bin, adj->sample_bin, (size_t)sim->n_wl, s));
@@ -869,18 +876,21 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     HIPCHECK(sim->adj_val2.reserve((size_t)chunk));
     HIPCHECK(sim->adj_tmp.reserve(tmp_bytes));
     HIPCHECK(sim->adj_seg.reserve((size_t)n_det + 1));
+    /* persistent replay over every resident block slot */
+    const int replay_grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu());
     double replay_ms = 0.0, reduce_ms = 0.0;
     int64_t h0 = 0;
     do {
         const int64_t nh = std::min<int64_t>(chunk, n_hit - h0);
-        const float4* hits = n_hit > 0 ? (const float4*)d_hit + 4 * h0 : nullptr;
+        const float4* hits = (const float4*)d_hit;
         HIPCHECK(hipEventRecord(sim->ev[0], s));
         HIPCHECK(oa_launch::adjoint_replay(P, sim->adj_mat_block.p, adj->n_mat, sim->adj_sample_bin.p, adj->n_bin,
-                                           n_param, hits, nh, sim->adj_rows.p, s));
+                                           n_param, hits, h0, nh, nullptr, 0, 0, sim->adj_rows.p, replay_grid, s));
         HIPCHECK(hipEventRecord(sim->ev[1], s));
-        HIPCHECK(oa_launch::adjoint_reduce(hits, nh, n_det, n_param, sim->adj_rows.p, sim->adj_key.p, sim->adj_val.p,
-                                           sim->adj_key2.p, sim->adj_val2.p, sim->adj_tmp.p, tmp_bytes, sim->adj_seg.p,
-                                           (float*)d_grad, sim->adj_comp.p, h0 == 0 ? 1 : 0, s));
+        HIPCHECK(oa_launch::adjoint_reduce(hits, h0, nh, nullptr, 0, 0, n_det, n_param, sim->adj_rows.p,
+                                           sim->adj_key.p, sim->adj_val.p, sim->adj_key2.p, sim->adj_val2.p,
+                                           sim->adj_tmp.p, tmp_bytes, sim->adj_seg.p, (float*)d_grad,
+                                           sim->adj_comp.p, h0 == 0 ? 1 : 0, s));
         HIPCHECK(hipEventRecord(sim->ev[2], s));
         HIPCHECK(hipEventSynchronize(sim->ev[2]));
         float a = 0.f, b = 0.f;
@@ -918,6 +928,19 @@ struct mi_batch {
     int64_t* h_out = nullptr;   /* pinned: hit offsets (n_event+1), then the prefix-mismatch flag */
     bool launched = false;
     double last_ms = 0.0;
+    /* adjoint part (optional): replay + per-detector reduction of every hit, chunked */
+    bool has_adj = false;
+    int32_t n_mat = 0, n_bin = 0, n_param = 0, n_det = 0;
+    int64_t chunk = 0, n_chunk = 0;
+    size_t tmp_bytes = 0;
+    float* d_grad = nullptr;
+    DBuf<int32_t> mat_block, sample_bin;
+    DBuf<float> rows, comp;
+    DBuf<uint32_t> key, val, key2, val2;
+    DBuf<unsigned char> tmp;
+    DBuf<int64_t> seg;
+    hipEvent_t ev_mid = nullptr;
+    double last_fwd_ms = 0.0, last_adj_ms = 0.0;
 };
 
 int mi_batch_destroy(mi_batch* b)
@@ -927,10 +950,13 @@ int mi_batch_destroy(mi_batch* b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    if (b->ev_mid) (void)hipEventDestroy(b->ev_mid);
     if (b->cap) (void)hipStreamDestroy(b->cap);
     if (b->h_out) (void)hipHostFree(b->h_out);
     b->prefix.release(); b->saved_prefix.release(); b->evbuf.release(); b->block_total.release(); b->ho.release();
     b->ev_id.release(); b->word_prefix.release(); b->words.release(); b->next.release(); b->flag.release();
+    b->mat_block.release(); b->sample_bin.release(); b->rows.release(); b->comp.release();
+    b->key.release(); b->val.release(); b->key2.release(); b->val2.release(); b->tmp.release(); b->seg.release();
     delete b;
     return MI_OK;
 }
@@ -960,9 +986,32 @@ static hipError_t batch_work(mi_batch* b, const SimParams& P, const void* d_gens
     return hipMemcpyAsync(b->h_out + b->n_event + 1, b->flag.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
 }
 
-int mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
-                    const void* d_gensteps, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
-                    int64_t* photon_offsets, mi_batch** out)
+/* adjoint of every hit of the batch: fixed chunks of b->chunk hit slots; the
+   kernels read the hit count from the device offsets, so no host round trip */
+static hipError_t batch_adjoint_work(mi_batch* b, SimParams P, const void* d_hit, hipStream_t s)
+{
+    const mi_sim* sim = b->sim;
+    const int grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu());
+    hipError_t e;
+    for (int64_t c = 0; c < b->n_chunk; ++c) {
+        const int64_t begin = c * b->chunk;
+        const int64_t n_max = std::min(b->chunk, b->hit_capacity - begin);
+        if ((e = oa_launch::adjoint_replay(P, b->mat_block.p, b->n_mat, b->sample_bin.p, b->n_bin, b->n_param,
+                                           (const float4*)d_hit, begin, n_max, b->ho.p, b->n_event, b->hit_capacity,
+                                           b->rows.p, grid, s)) != hipSuccess)
+            return e;
+        if ((e = oa_launch::adjoint_reduce((const float4*)d_hit, begin, n_max, b->ho.p, b->n_event, b->hit_capacity,
+                                           (uint32_t)b->n_det, b->n_param, b->rows.p, b->key.p, b->val.p, b->key2.p,
+                                           b->val2.p, b->tmp.p, b->tmp_bytes, b->seg.p, b->d_grad, b->comp.p,
+                                           c == 0 ? 1 : 0, s)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
+                        const void* d_gensteps, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                        const mi_adjoint* adj, void* d_grad, int64_t* photon_offsets, mi_batch** out)
 {
     if (!out) return fail(MI_ERR_ARG, "null out");
     *out = nullptr;
@@ -1023,18 +1072,89 @@ int mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, cons
     P.seq = (uint64_t*)d_seq;
     P.hitwords = b->words.p;
     P.next = b->next.p;
+    if (adj) {
+        if (!d_hit || hit_capacity <= 0) return bail(fail(MI_ERR_ARG, "the batch adjoint needs the hit buffer"));
+        if (!d_grad) return bail(fail(MI_ERR_ARG, "null gradient buffer"));
+        int rc = check_adjoint(sim, adj);
+        if (rc) return bail(rc);
+        b->has_adj = true;
+        b->n_mat = adj->n_mat; b->n_bin = adj->n_bin; b->n_det = adj->n_det;
+        b->n_param = adj->n_block * 2 * adj->n_bin;
+        b->d_grad = (float*)d_grad;
+        const int64_t rows_max = adj->max_row_floats > 0 ? adj->max_row_floats : OA_ADJ_ROWS_MAX;
+        b->chunk = std::max<int64_t>(1, std::min<int64_t>(hit_capacity, rows_max / b->n_param));
+        b->n_chunk = (hit_capacity + b->chunk - 1) / b->chunk;
+        b->tmp_bytes = std::max<size_t>(1, oa_launch::adjoint_sort_bytes(b->chunk, (uint32_t)b->n_det));
+        BCHECK(upload(b->mat_block, adj->mat_block, (size_t)adj->n_mat + 1, s));
+        BCHECK(upload(b->sample_bin, adj->sample_bin, (size_t)sim->n_wl, s));
+        if (b->rows.reserve((size_t)(b->chunk * b->n_param)) != hipSuccess)
+            return bail(fail(MI_ERR_CAPACITY, "batch adjoint: cannot allocate " +
+                                                  std::to_string(b->chunk * b->n_param * 4) + " bytes of score rows"));
+        BCHECK(b->comp.reserve((size_t)b->n_det * b->n_param));
+        BCHECK(b->key.reserve((size_t)b->chunk));
+        BCHECK(b->val.reserve((size_t)b->chunk));
+        BCHECK(b->key2.reserve((size_t)b->chunk));
+        BCHECK(b->val2.reserve((size_t)b->chunk));
+        BCHECK(b->tmp.reserve(b->tmp_bytes));
+        BCHECK(b->seg.reserve((size_t)b->n_det + 1));
+        BCHECK(hipEventCreate(&b->ev_mid));
+    }
+    /* one graph: forward child graph, an event record node (splits the
+       device time), adjoint child graph */
+    hipGraph_t gf = nullptr, ga = nullptr;
     BCHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
     hipError_t we = batch_work(b, P, d_gensteps, d_photon, d_hit, b->cap);
-    hipGraph_t g = nullptr;
-    hipError_t ce = hipStreamEndCapture(b->cap, &g);
-    b->graph = g;
-    BCHECK(we);
-    BCHECK(ce);
+    hipError_t ce = hipStreamEndCapture(b->cap, &gf);
+    if (we != hipSuccess || ce != hipSuccess) {
+        if (gf) (void)hipGraphDestroy(gf);
+        BCHECK(we);
+        BCHECK(ce);
+    }
+    if (!b->has_adj) {
+        b->graph = gf;
+    } else {
+        BCHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        we = batch_adjoint_work(b, P, d_hit, b->cap);
+        ce = hipStreamEndCapture(b->cap, &ga);
+        if (we != hipSuccess || ce != hipSuccess) {
+            (void)hipGraphDestroy(gf);
+            if (ga) (void)hipGraphDestroy(ga);
+            BCHECK(we);
+            BCHECK(ce);
+        }
+        hipGraph_t parent = nullptr;
+        hipGraphNode_t nf = nullptr, nm = nullptr, na = nullptr;
+        hipError_t e1 = hipGraphCreate(&parent, 0);
+        if (e1 == hipSuccess) e1 = hipGraphAddChildGraphNode(&nf, parent, nullptr, 0, gf);
+        if (e1 == hipSuccess) e1 = hipGraphAddEventRecordNode(&nm, parent, &nf, 1, b->ev_mid);
+        if (e1 == hipSuccess) e1 = hipGraphAddChildGraphNode(&na, parent, &nm, 1, ga);
+        (void)hipGraphDestroy(gf);
+        (void)hipGraphDestroy(ga);
+        b->graph = parent;
+        BCHECK(e1);
+    }
     BCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 #undef BCHECK
     b->generation = sim->generation;
     *out = b;
     return MI_OK;
+}
+
+int mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
+                    const void* d_gensteps, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                    int64_t* photon_offsets, mi_batch** out)
+{
+    return batch_create(sim, n_event, event_ids, gs_offsets, d_gensteps, d_photon, d_seq, d_hit, hit_capacity,
+                        nullptr, nullptr, photon_offsets, out);
+}
+
+int mi_batch_create_adjoint(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,
+                            const void* d_gensteps, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                            const mi_adjoint* adj, void* d_grad, int64_t* photon_offsets, mi_batch** out)
+{
+    if (!adj) return fail(MI_ERR_ARG, "null adjoint description");
+    return batch_create(sim, n_event, event_ids, gs_offsets, d_gensteps, d_photon, d_seq, d_hit, hit_capacity, adj,
+                        d_grad, photon_offsets, out);
 }
 
 int mi_batch_launch(mi_batch* b, void* stream)
@@ -1060,6 +1180,14 @@ int mi_batch_wait(mi_batch* b, int64_t* hit_offsets, double* device_ms)
     HIPCHECK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     b->last_ms = ms;
     if (device_ms) *device_ms = ms;
+    if (b->has_adj) {
+        float f = 0.f, a = 0.f;
+        HIPCHECK(hipEventElapsedTime(&f, b->ev[0], b->ev_mid));
+        HIPCHECK(hipEventElapsedTime(&a, b->ev_mid, b->ev[1]));
+        b->last_fwd_ms = f; b->last_adj_ms = a;
+    } else {
+        b->last_fwd_ms = ms; b->last_adj_ms = 0.0;
+    }
     if (hit_offsets)
         for (int32_t e = 0; e <= b->n_event; ++e) hit_offsets[e] = b->h_out[e];
     if (b->h_out[b->n_event + 1] != 0)
@@ -1070,6 +1198,14 @@ int mi_batch_wait(mi_batch* b, int64_t* hit_offsets, double* device_ms)
 }
 
 int64_t mi_batch_num_photon(const mi_batch* b) { return b ? b->n_photon : -1; }
+
+int mi_batch_last_timing(const mi_batch* b, double* ms2)
+{
+    if (!b || !ms2) return fail(MI_ERR_ARG, "null argument");
+    ms2[0] = b->last_fwd_ms;
+    ms2[1] = b->last_adj_ms;
+    return MI_OK;
+}
 
 /* ----------------------------------------------------------- autotuner */
 

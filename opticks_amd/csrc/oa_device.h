@@ -35,6 +35,14 @@ __device__ unsigned long long oa_stats[OA_ST_N];
             atomicAdd(&oa_stats[(l)], (unsigned long long)__popcll(m_));                      \
         }                                                                                     \
     } while (0)
+/* per-trace iteration histograms (log2 bins) of the TLAS walk [0] and the
+   leaf walk [1], and the first OA_SLOW_MAX traces whose TLAS walk took more
+   than OA_SLOW_ITER iterations: (o, tmin), (d, tmax), (best_t, n1, n2, inst) */
+#define OA_SLOW_MAX 16384
+#define OA_SLOW_ITER 100
+__device__ unsigned long long oa_hist[2][32];
+__device__ unsigned long long oa_slow_n;
+__device__ float4 oa_slow[OA_SLOW_MAX * 3];
 #else
 #define OA_WAVE_COUNT(w, l) ((void)0)
 #define OA_COUNT(i) ((void)0)

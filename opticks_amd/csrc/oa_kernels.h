@@ -19,10 +19,15 @@ hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out,
 hipError_t refit(const float* d_o2w, const BvhNode* d_blas, const uint32_t* d_inst_blas_root, int64_t n_inst,
                  float* d_inst_box, BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth,
                  const uint32_t* d_tlas_items, hipStream_t s);
+/* adjoint over hits [begin, begin+n) of d_hit, n = n_max or, with d_hit_off (per-event
+   hit offsets on the device, batched events), the device count clipped to hit_cap */
 hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_t n_mat, const int32_t* d_sample_bin,
-                          int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t n_hit, float* d_S, hipStream_t s);
-hipError_t adjoint_reduce(const float4* d_hit, int64_t n_hit, uint32_t n_det, int32_t n_param, const float* d_S,
-                          uint32_t* d_key, uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp,
-                          size_t tmp_bytes, int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s);
+                          int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t begin, int64_t n_max,
+                          const int64_t* d_hit_off, int32_t n_ev, int64_t hit_cap, float* d_S, int grid, hipStream_t s);
+hipError_t adjoint_reduce(const float4* d_hit, int64_t begin, int64_t n_max, const int64_t* d_hit_off, int32_t n_ev,
+                          int64_t hit_cap, uint32_t n_det, int32_t n_param, const float* d_S, uint32_t* d_key,
+                          uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp, size_t tmp_bytes,
+                          int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s);
+int adjoint_blocks_per_cu();
 size_t adjoint_sort_bytes(int64_t n_hit, uint32_t n_det);
 }

@@ -386,6 +386,28 @@ __global__ void k_refit_level(BvhNode* nodes, int64_t n_nodes, uint32_t depth, c
 
 namespace oa_launch {
 
+/* histograms (2 x 32) and slow-trace records (up to OA_SLOW_MAX x 3 float4)
+   of an OA_STATS build; returns the number of slow traces seen, or -1 */
+extern "C" long long oa_stats_dump(unsigned long long* hist64, float* slow, int reset)
+{
+#ifdef OA_STATS
+    unsigned long long n = 0;
+    if (hipMemcpyFromSymbol(hist64, HIP_SYMBOL(oa_hist), sizeof(unsigned long long) * 64) != hipSuccess) return -1;
+    if (hipMemcpyFromSymbol(&n, HIP_SYMBOL(oa_slow_n), sizeof(n)) != hipSuccess) return -1;
+    if (slow && hipMemcpyFromSymbol(slow, HIP_SYMBOL(oa_slow), sizeof(float4) * OA_SLOW_MAX * 3) != hipSuccess) return -1;
+    if (reset) {
+        static const unsigned long long zh[64] = {};
+        const unsigned long long z = 0;
+        if (hipMemcpyToSymbol(HIP_SYMBOL(oa_hist), zh, sizeof(zh)) != hipSuccess) return -1;
+        if (hipMemcpyToSymbol(HIP_SYMBOL(oa_slow_n), &z, sizeof(z)) != hipSuccess) return -1;
+    }
+    return (long long)n;
+#else
+    (void)hist64; (void)slow; (void)reset;
+    return -1;
+#endif
+}
+
 /* counters of an OA_STATS build (zeros otherwise) */
 extern "C" int oa_stats_read(unsigned long long* out, int reset)
 {
@@ -568,42 +590,112 @@ struct ScoreTape {
     }
 };
 
-/* one lane replays one hit photon (single-event launches) */
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint(SimParams P, AdjParams A, const float4* hits, int64_t n_hit,
-                                                          float* S)
+/*
+ * Hits processed by one adjoint chunk: hits [begin, begin + n) with
+ * n = n_max, or -- batched events, hit count known only on the device --
+ * n = min(n_max, min(hit_off[n_ev], hit_cap) - begin), where hit_off (n_ev+1)
+ * are the per-event hit offsets of the compaction.
+ */
+OA_D int64_t chunk_hits(int64_t begin, int64_t n_max, const int64_t* hit_off, int32_t n_ev, int64_t hit_cap)
 {
-    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
-    const int64_t h = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (h >= n_hit) return;
+    if (!hit_off) return n_max;
+    int64_t n = hit_off[n_ev];
+    n = n < hit_cap ? n : hit_cap;
+    n -= begin;
+    n = n < n_max ? n : n_max;
+    return n > 0 ? n : 0;
+}
+
+/* launch-local photon index of hit h, or -1: single event (hit_off null) from
+   the photon index minus photon_offset; batched events through the hit's event */
+OA_D int64_t hit_photon(const SimParams& P, const float4* hits, int64_t h, const int64_t* hit_off, int32_t n_ev)
+{
     const float4 q3 = hits[4 * h + 3];
     const uint32_t ident = __float_as_uint(q3.y), index = __float_as_uint(q3.z);
-    const int64_t photon_idx = ((int64_t)(ident >> 24) << 32) | (int64_t)index;
-    const int64_t g = photon_idx - P.photon_offset;
-    if (g < 0 || g >= P.n_photon) return;
+    const int64_t idx = ((int64_t)(ident >> 24) << 32) | (int64_t)index;
+    if (!hit_off) {
+        const int64_t g = idx - P.photon_offset;
+        return (g >= 0 && g < P.n_photon) ? g : -1;
+    }
+    const int e = (int)upper_index(hit_off, 0, n_ev, h);
+    const int64_t g = P.ev_ph_begin[e] + idx;
+    return (g >= P.ev_ph_begin[e] && g < P.ev_ph_begin[e + 1]) ? g : -1;
+}
+
+/* persistent replay: lanes take the next hit of the chunk from a global
+   counter (zeroed before the launch) as soon as their photon is done, as
+   k_sim_wave does; the score row of hit h is S + (h - begin) * n_param */
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjParams A, const float4* hits,
+                                                               int64_t begin, int64_t n_max, const int64_t* hit_off,
+                                                               int32_t n_ev, int64_t hit_cap, float* S)
+{
+    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
+    uint32_t* stk = s_stack + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t n = chunk_hits(begin, n_max, hit_off, n_ev, hit_cap);
     ScoreTape tape;
     tape.optical = P.optical; tape.n_wl = P.n_wl; tape.wl_low = P.wl_low; tape.wl_step = P.wl_step;
     tape.A = A;
-    tape.row = S + h * (int64_t)A.n_param;
+    tape.row = S;
     Lane L;
-    lane_start(P, L, g);
-    while (!lane_step<BLOCK>(P, L, s_stack + threadIdx.x, tape)) {}
+    bool active = false;
+    bool exhausted = n <= 0;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (!exhausted && idle != 0ull) {
+            const int n_idle = __popcll(idle);
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0ull;
+            if (lane == leader) base = atomicAdd(P.next, (unsigned long long)n_idle);
+            base = __shfl(base, leader);
+            if (!active) {
+                const unsigned long long below = idle & ((1ull << lane) - 1ull);
+                const int64_t i = (int64_t)(base + (unsigned long long)__popcll(below));
+                if (i < n) {
+                    const int64_t g = hit_photon(P, hits, begin + i, hit_off, n_ev);
+                    if (g >= 0) {
+                        lane_start(P, L, g);
+                        tape.row = S + i * (int64_t)A.n_param;
+                        active = true;
+                    }
+                }
+            }
+            if ((int64_t)(base + (unsigned long long)n_idle) >= n) exhausted = true;
+        }
+        const unsigned long long act = __ballot(active);
+        if (act == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && lane_step<BLOCK>(P, L, stk, tape)) active = false;
+    }
 }
 
-__global__ void k_zero_f32(float* p, int64_t n)
+/* zero the score rows of the chunk's hits (grid-stride) */
+__global__ void k_zero_rows(float* S, int32_t n_param, int64_t begin, int64_t n_max, const int64_t* hit_off,
+                            int32_t n_ev, int64_t hit_cap)
+{
+    const int64_t n = chunk_hits(begin, n_max, hit_off, n_ev, hit_cap) * (int64_t)n_param;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        S[i] = 0.f;
+}
+
+/* detector key (sensor_identifier - 1) of each of the chunk's n_max slots;
+   non-sensors and slots past the chunk's hits get n_det */
+__global__ void k_hit_det(const float4* hits, int64_t begin, int64_t n_max, const int64_t* hit_off, int32_t n_ev,
+                          int64_t hit_cap, uint32_t n_det, uint32_t* key, uint32_t* val)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.f;
-}
-
-/* detector key (sensor_identifier) of each hit; non-sensors get n_det */
-__global__ void k_hit_det(const float4* hits, int64_t n_hit, uint32_t n_det, uint32_t* key, uint32_t* val)
-{
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= n_hit) return;
-    const uint32_t id = __float_as_uint(hits[4 * h + 3].y) & 0x00ffffffu;
-    key[h] = (id >= 1u && id <= n_det) ? id - 1u : n_det;
-    val[h] = (uint32_t)h;
+    if (i >= n_max) return;
+    const int64_t n = chunk_hits(begin, n_max, hit_off, n_ev, hit_cap);
+    uint32_t k = n_det;
+    if (i < n) {
+        const uint32_t id = __float_as_uint(hits[4 * (begin + i) + 3].y) & 0x00ffffffu;
+        k = (id >= 1u && id <= n_det) ? id - 1u : n_det;
+    }
+    key[i] = k;
+    val[i] = (uint32_t)i;
 }
 
 /* start of each detector's run in the sorted keys */
@@ -642,15 +734,25 @@ __global__ void __launch_bounds__(256) k_det_reduce(const float* S, const uint32
 
 namespace oa_launch {
 
+int adjoint_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint_wave<128>, 128, 0) != hipSuccess) return 0;
+    return nb;
+}
+
 hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_t n_mat, const int32_t* d_sample_bin,
-                          int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t n_hit, float* d_S, hipStream_t s)
+                          int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t begin, int64_t n_max,
+                          const int64_t* d_hit_off, int32_t n_ev, int64_t hit_cap, float* d_S, int grid, hipStream_t s)
 {
     AdjParams A;
     A.mat_block = d_mat_block; A.n_mat = n_mat; A.sample_bin = d_sample_bin; A.n_bin = n_bin; A.n_param = n_param;
-    const int64_t ns = n_hit * (int64_t)n_param;
-    if (ns > 0) hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, d_S, ns);
-    if (n_hit > 0)
-        hipLaunchKernelGGL(k_adjoint<128>, dim3((unsigned)((n_hit + 127) / 128)), dim3(128), 0, s, P, A, d_hit, n_hit, d_S);
+    if (n_max <= 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)std::max(1, grid * 4)), dim3(256), 0, s, d_S, n_param, begin, n_max,
+                       d_hit_off, n_ev, hit_cap);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, P.next, (int64_t)1);
+    hipLaunchKernelGGL(k_adjoint_wave<128>, dim3((unsigned)std::max(1, grid)), dim3(128), 0, s, P, A, d_hit, begin, n_max,
+                       d_hit_off, n_ev, hit_cap, d_S);
     return hipGetLastError();
 }
 
@@ -661,17 +763,19 @@ static int det_key_bits(uint32_t n_det)
     return end_bit;
 }
 
-hipError_t adjoint_reduce(const float4* d_hit, int64_t n_hit, uint32_t n_det, int32_t n_param, const float* d_S,
-                          uint32_t* d_key, uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp,
-                          size_t tmp_bytes, int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s)
+hipError_t adjoint_reduce(const float4* d_hit, int64_t begin, int64_t n_max, const int64_t* d_hit_off, int32_t n_ev,
+                          int64_t hit_cap, uint32_t n_det, int32_t n_param, const float* d_S, uint32_t* d_key,
+                          uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp, size_t tmp_bytes,
+                          int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s)
 {
-    if (n_hit > 0) {
-        hipLaunchKernelGGL(k_hit_det, dim3((unsigned)((n_hit + 255) / 256)), dim3(256), 0, s, d_hit, n_hit, n_det, d_key, d_val);
+    if (n_max > 0) {
+        hipLaunchKernelGGL(k_hit_det, dim3((unsigned)((n_max + 255) / 256)), dim3(256), 0, s, d_hit, begin, n_max,
+                           d_hit_off, n_ev, hit_cap, n_det, d_key, d_val);
         hipError_t e = hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_key, d_key2, d_val, d_val2,
-                                                          (int)n_hit, 0, det_key_bits(n_det), s);
+                                                          (int)n_max, 0, det_key_bits(n_det), s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_det_segments, dim3((n_det + 1 + 255) / 256), dim3(256), 0, s, d_key2, n_hit, n_det, d_seg);
+    hipLaunchKernelGGL(k_det_segments, dim3((n_det + 1 + 255) / 256), dim3(256), 0, s, d_key2, n_max, n_det, d_seg);
     hipLaunchKernelGGL(k_det_reduce, dim3(n_det), dim3(256), 0, s, d_S, d_val2, d_seg, n_param, d_G, d_C, first);
     return hipGetLastError();
 }

@@ -534,10 +534,17 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     uint32_t cur = 0u;                       /* TLAS root node */
     uint32_t leaf = OA_BVH_EMPTY;
     bool done = false;
+#ifdef OA_STATS
+    uint32_t n1 = 0, n2 = 0;
+#endif
     while (!done) {
+#ifndef OA_SPECULATIVE
         /* phase 1: walk nodes until this lane has a leaf or nothing left */
         while (!done && leaf == OA_BVH_EMPTY) {
             OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
+#ifdef OA_STATS
+            n1++;
+#endif
             if (cur == OA_BVH_EMPTY) {
                 if (sp == 0) { done = true; break; }
                 sp--;
@@ -550,6 +557,36 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
             }
         }
+#else
+        /* phase 1, speculative: a lane that already holds a leaf keeps walking
+           internal nodes until every lane of the wave holds a leaf or is done */
+        for (;;) {
+            const bool walk = !done && (leaf == OA_BVH_EMPTY || (cur != OA_BVH_EMPTY && !(cur & OA_BVH_LEAF_BIT)));
+            if (walk) {
+                OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
+#ifdef OA_STATS
+                n1++;
+#endif
+                if (cur == OA_BVH_EMPTY) {
+                    if (sp == 0) done = true;
+                    else { sp--; cur = stk[sp * STRIDE]; }
+                } else if (cur & OA_BVH_LEAF_BIT) {
+                    leaf = cur;
+                    cur = OA_BVH_EMPTY;
+                } else {
+                    OA_COUNT(OA_ST_TLAS);
+                    cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
+                }
+            }
+#ifndef OA_SPEC_READY_QUARTERS
+            if (__all(done || leaf != OA_BVH_EMPTY)) break;
+#else
+            /* start the leaf phase once this many quarters of the lanes hold a leaf or are done */
+            const unsigned long long ready = __ballot(done || leaf != OA_BVH_EMPTY), act = __ballot(1);
+            if (ready == act || 4 * __popcll(ready) >= OA_SPEC_READY_QUARTERS * __popcll(act)) break;
+#endif
+        }
+#endif
         if (leaf == OA_BVH_EMPTY) continue;
         /* phase 2: the pending instance leaf */
         const uint32_t first = leaf & 0x00ffffffu, count = (leaf >> 24) & 0x7fu;
@@ -565,6 +602,9 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             uint32_t bc = __float_as_uint(m[2].w);     /* BLAS root node of this solid */
             for (;;) {
                 OA_WAVE_COUNT(OA_ST_W_P2, OA_ST_L_P2);
+#ifdef OA_STATS
+                n2++;
+#endif
                 if (bc == OA_BVH_EMPTY) {
                     if (sp == sp_base) break;
                     sp--;
@@ -593,6 +633,18 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             }
         }
     }
+#ifdef OA_STATS
+    atomicAdd(&oa_hist[0][n1 ? 32 - __clz(n1) : 0], 1ull);
+    atomicAdd(&oa_hist[1][n2 ? 32 - __clz(n2) : 0], 1ull);
+    if (n1 > OA_SLOW_ITER) {
+        const unsigned long long k = atomicAdd(&oa_slow_n, 1ull);
+        if (k < OA_SLOW_MAX) {
+            oa_slow[3 * k + 0] = make_float4(o.x, o.y, o.z, tmin);
+            oa_slow[3 * k + 1] = make_float4(d.x, d.y, d.z, tmax);
+            oa_slow[3 * k + 2] = make_float4(best_t, (float)n1, (float)n2, __uint_as_float(best_inst));
+        }
+    }
+#endif
     if (best_inst == 0xffffffffu) {
         prd.normal = mk(0.f, 0.f, 0.f); prd.t = 0.f; prd.lposcost = 0.f;
         prd.iindex_identity = 0xffffffffu; prd.boundary = 0xffffu;

@@ -265,9 +265,10 @@ class QSim:
         L.check(L.lib().mi_sim_last_adjoint_timing(self.handle, t), "mi_sim_last_adjoint_timing")
         return dict(replay_ms=t[0], reduce_ms=t[1], total_ms=t[2])
 
-    def batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None):
-        """Captured batch (hipGraph) over fixed buffers; see Batch."""
-        return Batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t, hit_t)
+    def batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, adjoint=None, grad_t=None):
+        """Captured batch (hipGraph) over fixed buffers; with adjoint (MiAdjoint) and grad_t
+        (n_det, n_param) f32 the replay also runs the yield adjoint of all hits. See Batch."""
+        return Batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t, hit_t, adjoint, grad_t)
 
     def autotune(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, reps=2):
         """Pick block size, refill period and TLAS leaf size on these events; returns all candidates."""
@@ -305,19 +306,28 @@ class Batch:
     its photon count.
     """
 
-    def __init__(self, sim: QSim, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None):
+    def __init__(self, sim: QSim, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, adjoint=None,
+                 grad_t=None):
         self.sim = sim
         self.n_event = len(event_ids)
         ev = np.ascontiguousarray(event_ids, np.int32)
         go = np.ascontiguousarray(gs_offsets, np.int64)
         self.photon_offsets = np.zeros(self.n_event + 1, np.int64)
-        self._keep = (gs_t, photon_t, seq_t, hit_t)
+        self._keep = (gs_t, photon_t, seq_t, hit_t, adjoint, grad_t)
         h = C.c_void_p()
         cap = 0 if hit_t is None else hit_t.shape[0]
-        L.check(L.lib().mi_batch_create(sim.handle, self.n_event, ev.ctypes.data, go.ctypes.data, gs_t.data_ptr(),
-                                        photon_t.data_ptr(), None if seq_t is None else seq_t.data_ptr(),
-                                        None if hit_t is None else hit_t.data_ptr(), cap,
-                                        self.photon_offsets.ctypes.data, C.byref(h)), "mi_batch_create")
+        seq_p = None if seq_t is None else seq_t.data_ptr()
+        hit_p = None if hit_t is None else hit_t.data_ptr()
+        if adjoint is None:
+            L.check(L.lib().mi_batch_create(sim.handle, self.n_event, ev.ctypes.data, go.ctypes.data, gs_t.data_ptr(),
+                                            photon_t.data_ptr(), seq_p, hit_p, cap, self.photon_offsets.ctypes.data,
+                                            C.byref(h)), "mi_batch_create")
+        else:
+            L.check(L.lib().mi_batch_create_adjoint(sim.handle, self.n_event, ev.ctypes.data, go.ctypes.data,
+                                                    gs_t.data_ptr(), photon_t.data_ptr(), seq_p, hit_p, cap,
+                                                    C.byref(adjoint), grad_t.data_ptr(),
+                                                    self.photon_offsets.ctypes.data, C.byref(h)),
+                    "mi_batch_create_adjoint")
         self.handle = h
         self.num_photon = int(L.lib().mi_batch_num_photon(h))
 
@@ -331,6 +341,12 @@ class Batch:
         ms = C.c_double(0)
         L.check(L.lib().mi_batch_wait(self.handle, ho.ctypes.data, C.byref(ms)), "mi_batch_wait")
         return ho, ms.value
+
+    def last_timing(self):
+        """[forward ms, adjoint ms] of the last replay (adjoint 0 without one)"""
+        t = (C.c_double * 2)()
+        L.check(L.lib().mi_batch_last_timing(self.handle, t), "mi_batch_last_timing")
+        return dict(forward_ms=t[0], adjoint_ms=t[1])
 
     def close(self):
         if getattr(self, "handle", None):

@@ -123,3 +123,43 @@ def test_autotune_picks_a_candidate_and_keeps_results(w2):
     assert np.array_equal(P2.view(np.uint32), P.view(np.uint32))
     assert np.array_equal(S2, S)
     sim.close()
+
+
+def test_batch_adjoint_graph_equals_per_event_adjoints():
+    from opticks_amd.adjoint import yield_jacobian
+    w = configs.config3(num_gs=30, per_gs=1000)
+    params = w.meta["params"]
+    n_det = 2000
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    offs = np.array([0, 10, 20, 30])
+    ev_ids = [4, 5, 6]
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    N = w.num_photon
+    # per-event reference: forward + adjoint of each event on its own
+    ref = np.zeros((n_det, params.n_param))
+    for e in range(3):
+        ge = gs_t[offs[e]:offs[e + 1]].contiguous()
+        ne = int(w.gensteps[offs[e]:offs[e + 1]].view(np.uint32)[:, 0, 3].sum())
+        ph = torch.empty((ne, 4, 4), dtype=torch.float32, device=DEV)
+        ht = torch.empty((ne, 4, 4), dtype=torch.float32, device=DEV)
+        nh = sim.simulate_device(ge, ph, None, ht, eventID=ev_ids[e], n_photon=ne)
+        ref += yield_jacobian(sim, params, ge, ht, nh, n_det, event_id=ev_ids[e], n_photon=ne).double().cpu().numpy()
+    scale = np.abs(ref).max()
+    assert scale > 0
+    photon, seq, hit = buffers(N)
+    for max_rows in (0, 53 * params.n_param):          # one chunk, then chunks of 53 hit slots
+        G = torch.full((n_det, params.n_param), 7.0, dtype=torch.float32, device=DEV)
+        b = sim.batch(gs_t, offs, ev_ids, photon, seq, hit, adjoint=params.c_struct(n_det, max_rows), grad_t=G)
+        outs = []
+        for rep in range(2):
+            b.launch()
+            ho, ms = b.wait()
+            t = b.last_timing()
+            assert t["forward_ms"] > 0 and t["adjoint_ms"] > 0
+            assert abs(t["forward_ms"] + t["adjoint_ms"] - ms) < 1e-3 * ms + 1e-3
+            outs.append(G.double().cpu().numpy())
+            torch.cuda.synchronize()
+        assert np.array_equal(outs[0], outs[1])            # replays are deterministic
+        assert np.allclose(outs[0], ref, rtol=1e-4, atol=1e-6 * scale)
+        b.close()
+    sim.close()
