@@ -73,6 +73,20 @@ def cpu_baseline(w, sample_photons, threads):
     return sample_photons / dt, dt
 
 
+def cpu_model():
+    """model name and logical CPU count of this host"""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for l in f:
+                if l.startswith("model name"):
+                    name = l.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs visible, {len(os.sched_getaffinity(0))} usable by this process"
+
+
 def profile_json(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -225,10 +239,15 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sample = args.cpu_sample if args.cpu_sample > 0 else N
         rate, cdt = cpu_baseline(w, sample, threads)
+        sample1 = min(sample, 1_000_000)
+        rate1, cdt1 = cpu_baseline(w, sample1, 1)
         line["cpu_baseline"] = {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
                                 "sample": f"first {sample} photons of the same workload "
                                           f"({cdt:.1f} s wall on {threads} threads; C restatement of "
-                                          f"qsim/CSG, OpenMP over photons, own median BVH)"}
+                                          f"qsim/CSG, OpenMP over photons, own median BVH)",
+                                "value_1_thread": rate1,
+                                "sample_1_thread": f"first {sample1} photons, {cdt1:.1f} s on 1 thread",
+                                "host_cpu": cpu_model()}
     if not args.no_batched:
         line["batched"] = bench_batched(torch, dev, max(1, min(args.steps, 3)))
     if not args.no_full:

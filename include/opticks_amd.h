@@ -24,6 +24,7 @@
  *   (no counterpart: differentiation)                   mi_sim_adjoint_device
  *   (no counterpart: replayable captured batches)       mi_batch_* (hipGraph)
  *   (no counterpart: launch tuning)                     mi_sim_autotune
+ *   SPM::merge_partial_select (sysrap/SPM.cu:135-240)   mi_sim_merge_hits_device
  *
  * Array layouts are the reference's, so arrays exported by the reference
  * (CSGFoundry / SSim folders) can be passed unchanged:
@@ -258,6 +259,26 @@ int  mi_sim_adjoint_device(mi_sim* sim, int32_t event_id,
                            const void* d_hit, int64_t n_hit,
                            const mi_adjoint* adj, void* d_grad, void* stream);
 int  mi_sim_last_adjoint_timing(const mi_sim* sim, double* ms3);
+
+/*
+ * Hit merging, SPM::merge_partial_select (sysrap/SPM.cu:135-240, sysrap/SPM.hh):
+ * records d_in (n_in) of type MI_RECORD_PHOTON (sphoton, 64 B) or
+ * MI_RECORD_PHOTONLITE (sphotonlite, 16 B, sysrap/sphotonlite.h:41-60) are
+ * selected by (flagmask & select_mask) != 0 (MI_MERGE_ALREADY_SELECTED skips
+ * the selection), then records with equal key (identity << 48 | (unsigned)(time /
+ * time_window)) are merged with the type's reduce_op (sphoton.h:292-320,
+ * sphotonlite.h:68-97): earliest time (first in input order on ties), OR of
+ * flagmasks, sum of hitcounts. Output in key order into d_out (caller-owned,
+ * out_capacity records); time_window 0 returns the selection in input order
+ * unmerged. *n_out gets the record count; MI_ERR_CAPACITY if it exceeds
+ * out_capacity. Synchronous on return.
+ */
+#define MI_RECORD_PHOTON          0
+#define MI_RECORD_PHOTONLITE      1
+#define MI_MERGE_ALREADY_SELECTED 0xffffffffu
+int  mi_sim_merge_hits_device(mi_sim* sim, int32_t record, const void* d_in, int64_t n_in,
+                              uint32_t select_mask, float time_window,
+                              void* d_out, int64_t out_capacity, int64_t* n_out, void* stream);
 
 /*
  * Captured batch (no counterpart in the reference): the batched form's

@@ -13,7 +13,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("oa_kernels.hip", "oa_capi.cpp")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("oa_kernels.hip", "oa_merge.hip", "oa_capi.cpp")]
 OUT = os.path.join(HERE, "libopticks_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -23,6 +23,7 @@
 #include "oa_layout.h"
 #include "oa_bvh.h"
 #include "oa_kernels.h"
+#include "oa_merge.h"
 
 namespace {
 
@@ -153,6 +154,12 @@ struct mi_sim {
     DBuf<uint32_t> adj_key, adj_val, adj_key2, adj_val2;
     DBuf<unsigned char> adj_tmp;
     DBuf<int64_t> adj_seg;
+    /* hit merging */
+    DBuf<uint8_t> m_flags;
+    DBuf<uint32_t> m_sel, m_vals, m_vals2;
+    DBuf<uint64_t> m_keys, m_keys2;
+    DBuf<int32_t> m_len, m_off, m_counts;
+    DBuf<unsigned char> m_tmp;
 };
 
 extern "C" {
@@ -450,6 +457,12 @@ void mi_sim_destroy(mi_sim* sim)
     sim->gs.release(); sim->prefix.release(); sim->block_total.release(); sim->evbuf.release();
     sim->word_prefix.release(); sim->ev_id.release(); sim->words.release(); sim->next.release();
     sim->photon.release(); sim->hit.release(); sim->seq.release();
+    sim->adj_mat_block.release(); sim->adj_sample_bin.release(); sim->adj_rows.release(); sim->adj_comp.release();
+    sim->adj_key.release(); sim->adj_val.release(); sim->adj_key2.release(); sim->adj_val2.release();
+    sim->adj_tmp.release(); sim->adj_seg.release();
+    sim->m_flags.release(); sim->m_sel.release(); sim->m_vals.release(); sim->m_vals2.release();
+    sim->m_keys.release(); sim->m_keys2.release(); sim->m_len.release(); sim->m_off.release();
+    sim->m_counts.release(); sim->m_tmp.release();
     for (auto& e : sim->ev) if (e) (void)hipEventDestroy(e);
     if (sim->stream) (void)hipStreamDestroy(sim->stream);
     delete sim;
@@ -900,6 +913,58 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
         h0 += nh;
     } while (h0 < n_hit);
     sim->adj_ms[0] = replay_ms; sim->adj_ms[1] = reduce_ms; sim->adj_ms[2] = replay_ms + reduce_ms;
+    return MI_OK;
+}
+
+int mi_sim_merge_hits_device(mi_sim* sim, int32_t record, const void* d_in, int64_t n_in, uint32_t select_mask,
+                             float time_window, void* d_out, int64_t out_capacity, int64_t* n_out, void* stream)
+{
+    if (!sim || !n_out) return fail(MI_ERR_ARG, "null argument");
+    *n_out = 0;
+    if (record != MI_RECORD_PHOTON && record != MI_RECORD_PHOTONLITE) return fail(MI_ERR_ARG, "unknown record type");
+    if (n_in < 0 || (n_in > 0 && !d_in)) return fail(MI_ERR_ARG, "bad input records");
+    if (n_in > (int64_t)INT32_MAX) return fail(MI_ERR_CAPACITY, "more than 2^31-1 records in one merge");
+    if (!(time_window >= 0.f)) return fail(MI_ERR_ARG, "time_window must be >= 0");
+    if (out_capacity < 0 || (out_capacity > 0 && !d_out)) return fail(MI_ERR_ARG, "bad output buffer");
+    if (n_in == 0) return MI_OK;
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    const size_t n = (size_t)n_in;
+    HIPCHECK(sim->m_flags.reserve(n));
+    HIPCHECK(sim->m_sel.reserve(n));
+    HIPCHECK(sim->m_vals.reserve(n));
+    HIPCHECK(sim->m_vals2.reserve(n));
+    HIPCHECK(sim->m_keys.reserve(n));
+    HIPCHECK(sim->m_keys2.reserve(n));
+    HIPCHECK(sim->m_len.reserve(n));
+    HIPCHECK(sim->m_off.reserve(n));
+    HIPCHECK(sim->m_counts.reserve(2));
+    const size_t tmp_bytes = std::max<size_t>(1, oa_merge::scratch_bytes(n_in));
+    HIPCHECK(sim->m_tmp.reserve(tmp_bytes));
+    oa_merge::Scratch S{ sim->m_flags.p, sim->m_sel.p, sim->m_vals.p, sim->m_vals2.p, sim->m_keys.p, sim->m_keys2.p,
+                         sim->m_len.p, sim->m_off.p, sim->m_counts.p, sim->m_tmp.p, tmp_bytes };
+    const int rec = record == MI_RECORD_PHOTON ? OA_MERGE_PHOTON : OA_MERGE_PHOTONLITE;
+    HIPCHECK(oa_merge::select(S, rec, d_in, n_in, select_mask, s));
+    int32_t cnt[2] = { 0, 0 };
+    HIPCHECK(hipMemcpyAsync(cnt, sim->m_counts.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    const int64_t n_sel = cnt[0];
+    if (n_sel == 0) return MI_OK;
+    if (time_window == 0.f) {
+        /* selection only, input order (SPM::NOMERGE_TIME_WINDOW) */
+        *n_out = n_sel;
+        if (n_sel > out_capacity) return fail(MI_ERR_CAPACITY, "out_capacity too small: " + std::to_string(n_sel) + " records");
+        HIPCHECK(oa_merge::gather_selected(S, rec, d_in, d_out, n_sel, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return MI_OK;
+    }
+    HIPCHECK(oa_merge::sort_runs(S, rec, d_in, n_sel, time_window, s));
+    HIPCHECK(hipMemcpyAsync(cnt + 1, sim->m_counts.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    const int64_t n_runs = cnt[1];
+    *n_out = n_runs;
+    if (n_runs > out_capacity) return fail(MI_ERR_CAPACITY, "out_capacity too small: " + std::to_string(n_runs) + " records");
+    HIPCHECK(oa_merge::reduce_runs(S, rec, d_in, d_out, n_runs, s));
+    HIPCHECK(hipStreamSynchronize(s));
     return MI_OK;
 }
 

@@ -289,6 +289,25 @@ class QSim:
         return [dict(block_size=r.block_size, refill_period=r.refill_period, bvh_leaf_size=r.bvh_leaf_size,
                      ms=r.ms) for r in res[:min(n.value, 64)]]
 
+    def merge_hits_device(self, in_t, select_mask=None, time_window=1.0, out_t=None, stream=None):
+        """SPM::merge_partial_select on the GPU. in_t: (n,4,4) f32 sphoton records or (n,4)
+        sphotonlite records (int32/float32 words); select_mask None = the hit mask, or
+        L.MI_MERGE_ALREADY_SELECTED; time_window in ns (0 = select only). Returns the
+        merged records as a view of out_t (allocated like in_t when None)."""
+        import torch
+        record = L.MI_RECORD_PHOTON if in_t.dim() == 3 else L.MI_RECORD_PHOTONLITE
+        mask = flag_mask(self.config.hit_mask) if select_mask is None else select_mask
+        n = in_t.shape[0]
+        if out_t is None:
+            out_t = torch.empty_like(in_t)
+        n_out = C.c_int64(0)
+        rc = L.lib().mi_sim_merge_hits_device(self.handle, record, in_t.data_ptr() if n else None, n,
+                                              C.c_uint32(mask & 0xFFFFFFFF), float(time_window),
+                                              out_t.data_ptr() if out_t.shape[0] else None, out_t.shape[0],
+                                              C.byref(n_out), None if stream is None else C.c_void_p(stream.cuda_stream))
+        L.check(rc, "mi_sim_merge_hits_device")
+        return out_t[:n_out.value]
+
     def trace_device(self, rays_t, isect_t):
         L.check(L.lib().mi_sim_trace_device(self.handle, rays_t.data_ptr(), rays_t.shape[0], isect_t.data_ptr(),
                                             None), "mi_sim_trace_device")
