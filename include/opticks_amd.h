@@ -16,6 +16,7 @@
  *     (qudarap/QSim.cc:720-755)
  *   QSim::simulate(eventID, reset) per-slice launches   mi_sim_simulate_device
  *     (qudarap/QSim.cc:436-536, CSGOptiX.cc:1091-1189)    (device-resident arrays)
+ *   evt->photonlite output (CSGOptiX7.cu:520-526)       mi_sim_simulate_lite_device
  *   SSimulator::reset(eventID) (G4CXOpticks.cc:491-511)  mi_sim_reset
  *   CSGOptiX simtrace (CSGOptiX7.cu:563-604)            mi_sim_trace_device
  *   (no counterpart: batched events in one launch)      mi_sim_simulate_batch_device
@@ -177,6 +178,21 @@ int  mi_sim_simulate_device(mi_sim* sim, int32_t event_id,
                             void* d_photon, void* d_seq,
                             void* d_hit, int64_t hit_capacity, int64_t* n_hit_out,
                             void* stream);
+
+/*
+ * Same, also writing d_photonlite (n_photon) sphotonlite records
+ * (sysrap/sphotonlite.h:41-60), as CSGOptiX7.cu:520-526 does for
+ * evt->photonlite: hitcount 1 | identity, time, 16-bit lposcost / lposfphi of
+ * the last trace's local hit position, flagmask. d_photon may be NULL (the lite
+ * records are the output), in which case d_hit must be NULL too; lite hits are
+ * selected and merged with mi_sim_merge_hits_device.
+ */
+int  mi_sim_simulate_lite_device(mi_sim* sim, int32_t event_id,
+                                 const void* d_gensteps, int64_t n_gs,
+                                 int64_t n_photon, int64_t photon_offset,
+                                 void* d_photonlite, void* d_photon, void* d_seq,
+                                 void* d_hit, int64_t hit_capacity, int64_t* n_hit_out,
+                                 void* stream);
 
 /*
  * Batched form: n_event events, event e owns gensteps

@@ -549,9 +549,9 @@ static int compact(mi_sim* sim, int64_t n_photon, const float4* d_photon, float4
     return MI_OK;
 }
 
-int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
-                           int64_t photon_offset, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
-                           int64_t* n_hit_out, void* stream)
+static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
+                                int64_t photon_offset, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                                int64_t* n_hit_out, void* d_lite, void* stream)
 {
     if (!sim) return fail(MI_ERR_ARG, "null sim");
     if (n_hit_out) *n_hit_out = 0;
@@ -615,6 +615,7 @@ int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps
         P.photon_offset = photon_offset + p0;
         P.photon = d_photon ? (float4*)d_photon + 4 * p0 : nullptr;
         P.seq = d_seq ? (uint64_t*)d_seq + 4 * p0 : nullptr;
+        P.photonlite = d_lite ? (uint4*)d_lite + p0 : nullptr;
         int rc = launch(sim, P, s);
         if (rc) return rc;
         HIPCHECK(hipEventRecord(sim->ev[2], s));
@@ -642,6 +643,23 @@ int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps
     if (n_hit_out) *n_hit_out = hits;
     if (d_hit && hits > hit_capacity) return fail(MI_ERR_CAPACITY, "hit_capacity too small: " + std::to_string(hits) + " hits");
     return MI_OK;
+}
+
+int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
+                           int64_t photon_offset, void* d_photon, void* d_seq, void* d_hit, int64_t hit_capacity,
+                           int64_t* n_hit_out, void* stream)
+{
+    return simulate_device_impl(sim, event_id, d_gensteps, n_gs, n_photon, photon_offset, d_photon, d_seq, d_hit,
+                                hit_capacity, n_hit_out, nullptr, stream);
+}
+
+int mi_sim_simulate_lite_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,
+                                int64_t photon_offset, void* d_photonlite, void* d_photon, void* d_seq, void* d_hit,
+                                int64_t hit_capacity, int64_t* n_hit_out, void* stream)
+{
+    if (!d_photonlite) return fail(MI_ERR_ARG, "null photonlite buffer");
+    return simulate_device_impl(sim, event_id, d_gensteps, n_gs, n_photon, photon_offset, d_photon, d_seq, d_hit,
+                                hit_capacity, n_hit_out, d_photonlite, stream);
 }
 
 int mi_sim_simulate_batch_device(mi_sim* sim, int32_t n_event, const int32_t* event_ids, const int64_t* gs_offsets,

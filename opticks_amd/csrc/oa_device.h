@@ -124,6 +124,7 @@ OA_D void set_flag(Photon& p, uint32_t flag)
 struct Prd {
     v3 normal; float t;
     float lposcost;
+    float lx, ly;                 /* local hit position x, y (azimuth for sphotonlite); miss: 0, NaN */
     uint32_t iindex_identity, boundary;
 };
 

@@ -62,6 +62,7 @@ struct Lane {
     uint64_t sh0, sh1, sb0, sb1;
     int bounce;
     int64_t g;
+    float lcost, lx, ly;          /* last trace's lposcost and local x, y (lite instantiations) */
 };
 
 OA_D void seq_add(Lane& L, unsigned slot, uint32_t flag, uint32_t boundary)
@@ -115,17 +116,19 @@ OA_D void lane_start(const SimParams& P, Lane& L, int64_t g)
     L.p.identity = ((uint32_t)((photon_idx >> 32) & 0xffu) << 24) | (L.p.identity & 0xffffffu);
     L.sh0 = L.sh1 = L.sb0 = L.sb1 = 0ull;
     L.bounce = 0;
+    L.lcost = 0.f; L.lx = 0.f; L.ly = __uint_as_float(0x7fc00000u);
     seq_add(L, 0u, ph_flag(L.p), ph_boundary(L.p));
 }
 
 /* one iteration of the CSGOptiX7.cu:477-509 loop; returns true when the photon is done */
-template <int STRIDE, class Tape>
+template <int STRIDE, class Tape, bool LITE = false>
 OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk, Tape& tape)
 {
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return true;
     const float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Prd prd;
     trace<STRIDE>(P, stk, L.p.pos, L.p.mom, tmin, P.tmax, prd);
+    if constexpr (LITE) { L.lcost = prd.lposcost; L.lx = prd.lx; L.ly = prd.ly; }
     if (prd.boundary == 0xffffu) return true;
     prd.normal = normalize(prd.normal);
     OA_COUNT(OA_ST_STEP);
@@ -137,9 +140,25 @@ OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk, Tape& tape)
     return !(L.bounce < P.max_bounce && L.p.time < P.max_time);
 }
 
+/* sphotonlite::init + set_lpos (sysrap/sphotonlite.h:192-198,234-244) from the
+   last trace (CSGOptiX7.cu:520-526); lposcost and lposfphi quantized to 16 bits
+   with the float->u16 conversion clamped to [0, 0xffff] */
+OA_D uint4 make_lite(const Lane& L)
+{
+    const float fphi = (L.ly != L.ly) ? 0.f : oa_fphi(L.ly, L.lx);
+    const float qc = fminf(fmaxf(L.lcost * 65535.f + 0.5f, 0.f), 65535.f);
+    const float qf = fminf(fmaxf(fphi * 65535.f + 0.5f, 0.f), 65535.f);
+    return make_uint4((1u << 16) | (L.p.identity & 0xffffu), __float_as_uint(L.p.time),
+                      ((uint32_t)qc << 16) | (uint32_t)qf, L.p.flagmask);
+}
+
+template <bool LITE = false>
 OA_D void lane_finish(const SimParams& P, const Lane& L)
 {
     const int64_t g = L.g;
+    if constexpr (LITE) {
+        if (P.photonlite) P.photonlite[g] = make_lite(L);
+    }
     if (P.photon) {
         float4* o = P.photon + 4 * g;
         o[0] = make_float4(L.p.pos.x, L.p.pos.y, L.p.pos.z, L.p.time);
@@ -157,7 +176,7 @@ OA_D void lane_finish(const SimParams& P, const Lane& L)
 
 /* ----------------------------------------------------- plain kernel */
 
-template <int BLOCK>
+template <int BLOCK, bool LITE>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
@@ -168,8 +187,8 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
         Lane L;
         NoTape tape;
         lane_start(P, L, g);
-        while (!lane_step<BLOCK>(P, L, stk, tape)) {}
-        lane_finish(P, L);
+        while (!lane_step<BLOCK, NoTape, LITE>(P, L, stk, tape)) {}
+        lane_finish<LITE>(P, L);
         hit = (L.p.flagmask & P.hit_mask) != 0u;
     }
     const unsigned long long m = __ballot(hit);
@@ -178,7 +197,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
 
 /* ------------------------------------------------ persistent kernel */
 
-template <int BLOCK>
+template <int BLOCK, bool LITE>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
@@ -214,8 +233,8 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
         if (lane == 0) { OA_COUNT(OA_ST_ITER); OA_COUNT_N(OA_ST_ACTIVE, __popcll(act)); }
 #endif
         if (active) {
-            if (lane_step<BLOCK>(P, L, stk, tape)) {
-                lane_finish(P, L);
+            if (lane_step<BLOCK, NoTape, LITE>(P, L, stk, tape)) {
+                lane_finish<LITE>(P, L);
                 if ((L.p.flagmask & P.hit_mask) != 0u)
                     atomicOr(P.hitwords + (L.g >> 6), 1ull << (L.g & 63));
                 active = false;
@@ -435,9 +454,11 @@ static hipError_t sim_t(const SimParams& P, int kernel, int grid, hipStream_t s)
 {
     if (kernel == 0) {
         const int64_t nb = (P.n_photon + B - 1) / B;
-        hipLaunchKernelGGL(k_sim_plain<B>, dim3((unsigned)nb), dim3(B), 0, s, P);
+        if (P.photonlite) hipLaunchKernelGGL((k_sim_plain<B, true>), dim3((unsigned)nb), dim3(B), 0, s, P);
+        else hipLaunchKernelGGL((k_sim_plain<B, false>), dim3((unsigned)nb), dim3(B), 0, s, P);
     } else {
-        hipLaunchKernelGGL(k_sim_wave<B>, dim3((unsigned)grid), dim3(B), 0, s, P);
+        if (P.photonlite) hipLaunchKernelGGL((k_sim_wave<B, true>), dim3((unsigned)grid), dim3(B), 0, s, P);
+        else hipLaunchKernelGGL((k_sim_wave<B, false>), dim3((unsigned)grid), dim3(B), 0, s, P);
     }
     return hipGetLastError();
 }
@@ -446,7 +467,7 @@ template <int B>
 static int occ_t()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sim_wave<B>, B, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sim_wave<B, false>, B, 0) != hipSuccess) return 0;
     return nb;
 }
 

@@ -93,6 +93,7 @@ struct SimParams {
     /* outputs */
     float4*        photon;        /* (n_photon,4) float4 or null */
     uint64_t*      seq;           /* (n_photon,2,2) or null */
+    uint4*         photonlite;    /* (n_photon) sphotonlite or null (lite kernel instantiation) */
     unsigned long long* hitwords; /* (n_photon+63)/64 bitmask, zeroed before launch */
     /* persistent kernel */
     unsigned long long* next;     /* global photon counter */

@@ -647,6 +647,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #endif
     if (best_inst == 0xffffffffu) {
         prd.normal = mk(0.f, 0.f, 0.f); prd.t = 0.f; prd.lposcost = 0.f;
+        prd.lx = 0.f; prd.ly = __uint_as_float(0x7fc00000u);
         prd.iindex_identity = 0xffffffffu; prd.boundary = 0xffffu;
         return;
     }
@@ -660,6 +661,8 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     inst_ray(m, o, d, oo, od);
     const v3 lpos = oo + best_t * od;
     prd.lposcost = lpos.z / sqrtf(dot(lpos, lpos));
+    prd.lx = lpos.x;
+    prd.ly = lpos.y;
     const uint32_t ident = __float_as_uint(r1.w);
     prd.iindex_identity = ((best_inst & 0xffffu) << 16) | (ident & 0xffffu);
     prd.boundary = P.prim[best_prim].boundary & 0xffffu;

@@ -15,6 +15,13 @@
  *   oa_sinf / oa_cosf / oa_sincosf
  *              Cody-Waite reduction by pi/2 (three-part split, 12+12+24 bits)
  *              and Taylor polynomials on [-pi/4, pi/4] (sin to r^9, cos to r^10).
+ *   oa_atan2f  octant reduction to a = min(|x|,|y|)/max(|x|,|y|) in [0,1], then
+ *              atan(a) = pi/6 + atan((a sqrt3 - 1)/(a + sqrt3)) for a > tan(pi/12)
+ *              so the argument t has |t| <= tan(pi/12), atan(t) = t P(t^2) with P
+ *              a degree-4 least-squares fit of atan(t)/t on [0, tan(pi/12)]
+ *              (fitted for this file: <= 1.02 ulp on that interval), and the
+ *              quadrant reconstruction pi/2 - r, pi - r, -r. Signed zeros are
+ *              not distinguished (atan2(0, -0) gives 0).
  *   Valid domains: oa_logf for positive finite normal x; sin/cos for |x| < 1e4.
  *   Accuracy is pinned against glibc in tests/test_portable_math.py.
  *
@@ -121,6 +128,36 @@ OA_FN void oa_sincosf(float x, float* sn, float* cs)
 
 OA_FN float oa_sinf(float x) { float s, c; oa_sincosf(x, &s, &c); return s; }
 OA_FN float oa_cosf(float x) { float s, c; oa_sincosf(x, &s, &c); return c; }
+
+#define OA_PORT_PI_F 3.14159265358979323846f
+#define OA_PI_6_F   0.52359877559829887308f
+#define OA_PI_2_F   1.57079632679489661923f
+#define OA_TAN_PI_12_F 0.26794919243112270647f
+#define OA_SQRT3_F  1.73205080756887729353f
+
+OA_FN float oa_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.f ? mn / mx : 0.f;
+    const int big = a > OA_TAN_PI_12_F;
+    const float t = big ? (a * OA_SQRT3_F - 1.0f) / (a + OA_SQRT3_F) : a;
+    const float z = t * t;
+    float p = 0.0966006294f;                 /* 0x3dc5d68d */
+    p = p * z + -0.142003193f;               /* 0xbe116949 */
+    p = p * z + 0.199979603f;                /* 0x3e4cc774 */
+    p = p * z + -0.333333164f;               /* 0xbeaaaaa5 */
+    p = p * z + 1.0f;
+    float r = t * p;
+    r = big ? r + OA_PI_6_F : r;
+    r = ay > ax ? OA_PI_2_F - r : r;
+    r = x < 0.f ? OA_PORT_PI_F - r : r;
+    return y < 0.f ? -r : r;
+}
+
+/* sphotonlite lposfphi: azimuth of a local position as a fraction of a turn
+   (normalize_fphi, sysrap/scuda.h:670-673) */
+OA_FN float oa_fphi(float y, float x) { return (oa_atan2f(y, x) + OA_PORT_PI_F) / (2.0f * OA_PORT_PI_F); }
 
 /* --------------------------------------------------------------- Philox */
 

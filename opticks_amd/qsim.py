@@ -216,6 +216,21 @@ class QSim:
         L.check(rc, "mi_sim_simulate_device")
         return n_hit.value
 
+    def simulate_lite_device(self, gs_t, lite_t, photon_t=None, seq_t=None, hit_t=None, eventID=0, n_photon=-1,
+                             photon_offset=0, stream=None):
+        """as simulate_device, also writing (N,4) sphotonlite records into lite_t (int32 or uint32 words)"""
+        n_hit = C.c_int64(0)
+        cap = 0 if hit_t is None else hit_t.shape[0]
+        rc = L.lib().mi_sim_simulate_lite_device(
+            self.handle, eventID, gs_t.data_ptr(), gs_t.shape[0], n_photon, photon_offset, lite_t.data_ptr(),
+            None if photon_t is None else photon_t.data_ptr(), None if seq_t is None else seq_t.data_ptr(),
+            None if hit_t is None else hit_t.data_ptr(), cap, C.byref(n_hit),
+            None if stream is None else C.c_void_p(stream.cuda_stream))
+        if rc == L.MI_NOTHING_TO_DO:
+            return 0
+        L.check(rc, "mi_sim_simulate_lite_device")
+        return n_hit.value
+
     def simulate_batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, stream=None):
         """All events' gensteps concatenated in gs_t; returns (photon_offsets, hit_offsets)."""
         ne = len(event_ids)

@@ -106,6 +106,7 @@ typedef struct {
     float lposcost, lposfphi;
     uint32_t iindex_identity, gp_boundary;
     uint32_t inst;          /* instance index of the hit (not in quad2) */
+    float lx, ly;           /* local hit position x, y (lposfphi source); miss: 0, NaN */
 } oprd;
 
 /* ------------------------------------------------------------ random */
@@ -726,6 +727,8 @@ static int test_instance(const oworld* w, int64_t i, f3 o, f3 d, float tmin, flo
             f3 lpos = add3(oo, smul3(is.w, od));
             prd->lposcost = lpos.z / sqrtf(dot3(lpos, lpos));
             prd->lposfphi = 0.f;
+            prd->lx = lpos.x;
+            prd->ly = lpos.y;
             uint32_t boundary = nd_boundary(root);
             uint32_t gp = (uint32_t)((const int32_t*)(w->g.prim + 16 * pidx))[15];
             prd->gp_boundary = ((gp & 0xffffu) << 16) | (boundary & 0xffffu);
@@ -743,6 +746,7 @@ static int world_trace(const oworld* w, f3 o, f3 d, float tmin, float tmax, oprd
     uint64_t best_key = ~0ull;
     prd->normal = mk3(0.f, 0.f, 0.f); prd->t = 0.f;
     prd->lposcost = 0.f; prd->lposfphi = 0.f;
+    prd->lx = 0.f; prd->ly = NAN;
     prd->iindex_identity = 0xffffffffu; prd->gp_boundary = 0xffffffffu; prd->inst = 0xffffffffu;
     if (traverse == 0) {
         for (int64_t i = 0; i < w->g.n_inst; ++i) {
@@ -1351,7 +1355,8 @@ static int64_t genstep_of(const int64_t* off, int64_t n_gs, int64_t idx)
 
 /* one photon through the CSGOptiX7.cu:432-530 simulate body; returns error bits */
 static int simulate_one(const oworld* w, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
-                        const float* gs, uint64_t photon_idx, oph* p, uint64_t* seqhis, uint64_t* seqbnd, oscore* sc)
+                        const float* gs, uint64_t photon_idx, oph* p, uint64_t* seqhis, uint64_t* seqbnd, oscore* sc,
+                        oprd* last)
 {
     int bad = 0;
     orng rng;
@@ -1373,6 +1378,8 @@ static int simulate_one(const oworld* w, const oracle_phys* ph, const oracle_cfg
     int bounce = 0;
     seq_add_nibble(seqhis, seqbnd, (unsigned)bounce, ph_flag(p), ph_boundary(p));
     oprd prd;
+    memset(&prd, 0, sizeof(prd));
+    prd.ly = NAN;
     while (bounce < cfg->max_bounce && p->time < cfg->max_time) {
         float tmin = (p->orient_boundary_flag & cfg->eps0_mask) ? cfg->tmin0 : cfg->tmin;
         if (world_trace(w, p->pos, p->mom, tmin, cfg->tmax, &prd, cfg->traverse) != 0) { bad |= 1; break; }
@@ -1383,7 +1390,24 @@ static int simulate_one(const oworld* w, const oracle_phys* ph, const oracle_cfg
         seq_add_nibble(seqhis, seqbnd, (unsigned)bounce, ph_flag(p), ph_boundary(p));
         if (command == C_BREAK) break;
     }
+    if (last) *last = prd;
     return bad;
+}
+
+/* sphotonlite::init + set_lpos (sysrap/sphotonlite.h:192-198,234-244) of the
+   final photon and its last trace (CSGOptiX7.cu:520-526), float->u16 clamped */
+static void make_lite(const oph* p, const oprd* prd, uint32_t* out4)
+{
+    float fphi = (prd->ly != prd->ly) ? 0.f : oa_fphi(prd->ly, prd->lx);
+    float qc = fminf(fmaxf(prd->lposcost * 65535.f + 0.5f, 0.f), 65535.f);
+    float qf = fminf(fmaxf(fphi * 65535.f + 0.5f, 0.f), 65535.f);
+    float t = p->time;
+    uint32_t tu;
+    memcpy(&tu, &t, 4);
+    out4[0] = (1u << 16) | (p->identity & 0xffffu);
+    out4[1] = tu;
+    out4[2] = ((uint32_t)qc << 16) | (uint32_t)qf;
+    out4[3] = p->flagmask;
 }
 
 static int64_t* genstep_offsets(const float* gensteps, int64_t n_gs)
@@ -1394,9 +1418,9 @@ static int64_t* genstep_offsets(const float* gensteps, int64_t n_gs)
     return off;
 }
 
-int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
-                        const float* gensteps, int64_t n_gs, int64_t photon_offset,
-                        int64_t p_begin, int64_t p_end, float* photon_out, uint64_t* seq_out)
+int64_t oracle_simulate_ex(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
+                           const float* gensteps, int64_t n_gs, int64_t photon_offset,
+                           int64_t p_begin, int64_t p_end, float* photon_out, uint64_t* seq_out, uint32_t* lite_out)
 {
     oworld* w = (oworld*)wp;
     if (!w || !ph || !cfg || (n_gs > 0 && !gensteps)) return -1;
@@ -1413,9 +1437,11 @@ int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
         const float* gs = gensteps + 24 * genstep_of(off, n_gs, idx);
         oph p;
         uint64_t seqhis[2], seqbnd[2];
-        bad |= simulate_one(w, ph, cfg, event_idx, gs, (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, NULL);
+        oprd last;
+        bad |= simulate_one(w, ph, cfg, event_idx, gs, (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, NULL, &last);
         int64_t o = idx - p_begin;
         if (photon_out) memcpy(photon_out + 16 * o, &p, 64);
+        if (lite_out) make_lite(&p, &last, lite_out + 4 * o);
         if (seq_out) {
             seq_out[4 * o + 0] = seqhis[0]; seq_out[4 * o + 1] = seqhis[1];
             seq_out[4 * o + 2] = seqbnd[0]; seq_out[4 * o + 3] = seqbnd[1];
@@ -1423,6 +1449,21 @@ int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
     }
     free(off);
     return bad ? -(int64_t)bad - 1 : total;
+}
+
+int64_t oracle_simulate(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
+                        const float* gensteps, int64_t n_gs, int64_t photon_offset,
+                        int64_t p_begin, int64_t p_end, float* photon_out, uint64_t* seq_out)
+{
+    return oracle_simulate_ex(wp, ph, cfg, event_idx, gensteps, n_gs, photon_offset, p_begin, p_end, photon_out,
+                              seq_out, NULL);
+}
+
+float oracle_atan2f(float y, float x) { return oa_atan2f(y, x); }
+
+void oracle_atan2f_n(const float* y, const float* x, float* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = oa_atan2f(y[i], x[i]);
 }
 
 int oracle_adjoint_rows(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
@@ -1449,7 +1490,7 @@ int oracle_adjoint_rows(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
         oph p;
         uint64_t seqhis[2], seqbnd[2];
         bad |= simulate_one(w, ph, cfg, event_idx, gensteps + 24 * genstep_of(off, n_gs, idx),
-                            (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, &sc);
+                            (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, &sc, NULL);
     }
     free(off);
     return bad ? -bad : 0;

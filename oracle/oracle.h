@@ -93,6 +93,15 @@ int oracle_adjoint_rows(void* world, const oracle_phys* ph, const oracle_cfg* cf
                         const int64_t* photons, int64_t n, const int32_t* mat_block, int32_t n_mat,
                         const int32_t* sample_bin, int32_t n_bin, int32_t n_param, float* rows);
 
+/* As oracle_simulate, also writing (p_end-p_begin,4) u32 sphotonlite records
+   (lite_out may be NULL). */
+int64_t oracle_simulate_ex(void* world, const oracle_phys* ph, const oracle_cfg* cfg,
+                           int32_t event_idx, const float* gensteps, int64_t n_gs,
+                           int64_t photon_offset, int64_t p_begin, int64_t p_end,
+                           float* photon_out, uint64_t* seq_out, uint32_t* lite_out);
+float oracle_atan2f(float y, float x);
+void  oracle_atan2f_n(const float* y, const float* x, float* out, int64_t n);
+
 /* Nearest hit for n rays; same record layout as mi_sim_trace_device. */
 int oracle_trace(void* world, const float* rays, int64_t n, float* isect, int32_t traverse);
 

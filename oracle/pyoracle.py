@@ -65,6 +65,11 @@ def lib():
                                       C.c_void_p, C.c_void_p]
         L.oracle_trace.restype = C.c_int
         L.oracle_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]
+        L.oracle_simulate_ex.argtypes = [C.c_void_p, C.POINTER(OraclePhys), C.POINTER(OracleCfg), C.c_int32,
+                                         C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_simulate_ex.restype = C.c_int64
+        L.oracle_atan2f_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_adjoint_rows.argtypes = [C.c_void_p, C.POINTER(OraclePhys), C.POINTER(OracleCfg), C.c_int32,
                                           C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
                                           C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
@@ -174,6 +179,24 @@ class World:
             raise RuntimeError(f"oracle_simulate failed rc={rc}")
         return photon, seq
 
+    def simulate_lite(self, gensteps, event_idx=0, photon_offset=0, p_begin=0, p_end=None, **cfgkw):
+        """-> photon (n,4,4) f32, seq (n,2,2) u64, lite (n,4) u32 sphotonlite"""
+        gs = np.ascontiguousarray(gensteps, np.float32).reshape(-1, 6, 4)
+        total = int(gs.view(np.uint32)[:, 0, 3].sum())
+        if p_end is None or p_end > total:
+            p_end = total
+        n = max(0, p_end - p_begin)
+        photon = np.zeros((n, 4, 4), np.float32)
+        seq = np.zeros((n, 2, 2), np.uint64)
+        lite = np.zeros((n, 4), np.uint32)
+        cfg = self.cfg(**cfgkw)
+        rc = lib().oracle_simulate_ex(self.handle, C.byref(self.phys), C.byref(cfg), event_idx, gs.ctypes.data,
+                                      len(gs), photon_offset, p_begin, p_end, photon.ctypes.data, seq.ctypes.data,
+                                      lite.ctypes.data)
+        if rc < 0:
+            raise RuntimeError(f"oracle_simulate_ex failed rc={rc}")
+        return photon, seq, lite
+
     def adjoint_rows(self, gensteps, photons, mat_block, sample_bin, n_bin, event_idx=0, photon_offset=0,
                      **cfgkw):
         """Score rows (n, n_param) f32 of the in-event photons listed in `photons`."""
@@ -237,6 +260,14 @@ def philox_block(ctr, k0, k1):
 
 def logf(x):
     return lib().oracle_logf(float(x))
+
+
+def atan2f_n(y, x):
+    y = np.ascontiguousarray(y, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(y)
+    lib().oracle_atan2f_n(y.ctypes.data, x.ctypes.data, out.ctypes.data, len(y))
+    return out
 
 
 def logf_n(x):

@@ -90,3 +90,41 @@ def test_merge_edge_cases(sim_and_photons):
         sim.merge_hits_device(t, SA, 1.0, out_t=small)
     with pytest.raises(RuntimeError):
         sim.merge_hits_device(t, SA, -1.0)
+
+
+def test_photonlite_output_matches_checker_and_merges():
+    import pyoracle as po
+    w = configs.config1(num_gs=20, per_gs=2000)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    N = w.num_photon
+    gs = torch.from_numpy(w.gensteps).to(DEV)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
+    lite = torch.empty((N, 4), dtype=torch.int32, device=DEV)
+    sim.simulate_lite_device(gs, lite, photon, seq, n_photon=N)
+    G = lite.cpu().numpy().view(np.uint32)
+    P = photon.cpu().numpy()
+    rp, rs, rl = po.World(w.geometry, w.physics).simulate_lite(w.gensteps, seed=w.seed)
+    same = np.all(seq.cpu().numpy().view(np.uint64) == rs, axis=(1, 2))
+    assert same.mean() >= 0.999
+    assert np.array_equal(G[same], rl[same])
+    # lite fields are the photon's identity, time and flagmask, hitcount 1
+    u = P.view(np.uint32)
+    assert np.array_equal(G[:, 0] & 0xFFFF, u[:, 3, 1] & 0xFFFF)
+    assert np.all(G[:, 0] >> 16 == 1)
+    assert np.array_equal(G[:, 1], u[:, 0, 3])
+    assert np.array_equal(G[:, 3], u[:, 3, 3])
+    # detected photons hit the detector front: lposcost > 0
+    sd = (u[:, 3, 3] & SD) != 0
+    assert sd.sum() > 100 and np.all((G[sd, 2] >> 16) > 0)
+    # same results with and without the lite instantiation
+    p2 = torch.empty_like(photon)
+    sim.simulate_device(gs, p2, None, None, n_photon=N)
+    assert np.array_equal(p2.cpu().numpy().view(np.uint32), u)
+    # lite hits selected and merged on the GPU equal the restatement on the checker's lites
+    out = sim.merge_hits_device(lite, SD, 1.0).cpu().numpy().view(np.uint32)
+    ref = spm.merge_lites(rl[same], SD, 1.0) if same.all() else None
+    if ref is not None:
+        assert np.array_equal(out, ref)
+    assert np.array_equal(out, spm.merge_lites(G, SD, 1.0))
+    sim.close()

@@ -87,3 +87,25 @@ def test_streams_are_distinct_per_photon_and_seed():
     b = po.uniforms(0, 1, 16)
     c = po.uniforms(1, 0, 16)
     assert not np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+def test_atan2_accuracy_and_quadrants():
+    rng = np.random.default_rng(11)
+    ang = rng.uniform(-np.pi, np.pi, 400000)
+    r = 10.0 ** rng.uniform(-3, 4, len(ang))
+    y = (r * np.sin(ang)).astype(np.float32)
+    x = (r * np.cos(ang)).astype(np.float32)
+    got = po.atan2f_n(y, x)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    # absolute error within 3e-7 rad everywhere (floats near pi are 2.4e-7 apart),
+    # and ulp level away from the zero crossings
+    assert np.abs(got - ref).max() < 3e-7
+    big = np.abs(ref) > 1e-2
+    assert ulp_err(got[big], ref[big]).max() <= 3.0
+    # octant edges and axes
+    e = np.array([[0, 1], [1, 1], [1, 0], [1, -1], [0, -1], [-1, -1], [-1, 0], [-1, 1], [0, 0]], np.float32)
+    g = po.atan2f_n(e[:, 0], e[:, 1])
+    assert np.allclose(g, np.arctan2(e[:, 0], e[:, 1]), atol=2.5e-7)
+    # lposfphi = (atan2 + pi) / (2 pi) stays in [0, 1]
+    f = (got + np.float32(np.pi)) / (np.float32(2.0) * np.float32(np.pi))
+    assert f.min() >= 0.0 and f.max() <= 1.0
