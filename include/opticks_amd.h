@@ -40,6 +40,8 @@
  *   prim     (n,4,4)  CSGPrim  (CSG/CSGPrim.h:75)
  *   solid    (n) 48 B CSGSolid (CSG/CSGSolid.h:37)
  *   itra     (n,4,4)  qat4 inverse node transforms (sysrap/sqat4.h)
+ *   plan     (n,4)    convex polyhedron planes (nx,ny,nz,d), addressed by
+ *                     CSGNode planeIdx/planeNum (CSG/CSGNode.h:204-207)
  *   inst     (n,4,4)  qat4 instance transforms, identity in the w column
  *                     (sysrap/sqat4.h:345-390)
  *   bnd      (nbnd,4,2,nwl,4) f32 boundary table (qudarap/QBnd.cc:130-175)

@@ -174,6 +174,62 @@ def _scint_world(num_det, det_r, ls_r, shell, world_half):
     return fd.arrays(), phys
 
 
+def leaf_zoo_nodes():
+    """One prim per entry: the cone, hyperboloid, convex polyhedra, and phicut /
+    halfspace leaves inside Boolean trees, with one transformed cone."""
+    from .foundry import Node, CSG_INTERSECTION, CSG_DIFFERENCE
+    cube = np.array([[1, 0, 0, 70], [-1, 0, 0, 70], [0, 1, 0, 70], [0, -1, 0, 70], [0, 0, 1, 70], [0, 0, -1, 70]],
+                    np.float32)
+    tn = np.array([[1, 1, 1], [-1, -1, 1], [-1, 1, -1], [1, -1, -1]], float) / np.sqrt(3.0)
+    tetra = np.concatenate([tn, np.full((4, 1), 60.0)], axis=1).astype(np.float32)
+    rot = rotation_z_to([0.3, -0.5, 0.81])
+    rot[3, :3] = (15.0, -20.0, 10.0)
+    tc = Node.cone(90.0, -60.0, 20.0, 70.0)
+    tc.transform = rot
+    return [
+        Node.cone(100.0, -50.0, 30.0, 80.0),
+        Node.hyperboloid(60.0, 80.0, -100.0, 120.0),
+        Node.convexpolyhedron(cube, (np.full(3, -70.0), np.full(3, 70.0))),
+        Node.convexpolyhedron(tetra, (np.full(3, -110.0), np.full(3, 110.0))),
+        Node.op(CSG_INTERSECTION, Node.cylinder(120.0, -80.0, 80.0), Node.phicut(0.3, 2.2)),
+        Node.op(CSG_INTERSECTION, Node.sphere(150.0), Node.phicut(0.5, 4.5)),
+        Node.op(CSG_DIFFERENCE, Node.sphere(140.0), Node.halfspace((0.0, 0.0, 1.0), 40.0)),
+        Node.op(CSG_INTERSECTION, Node.box3(200.0, 200.0, 200.0), Node.halfspace((1.0, 1.0, 0.0), 30.0)),
+        tc,
+    ]
+
+
+def leaf_zoo(num_photon=200_000, seed=42, spacing=400.0):
+    """The leaf_zoo_nodes solids as acrylic pieces on a 3x3 grid in a water box
+    with an absorbing, partly detecting wall; photons from a torch point source
+    at the centre of each cell's corner region."""
+    lib = BoundaryLib()
+    lib.add_material(Material("Water", 1.33, 20000.0, 30000.0))
+    lib.add_material(Material("Acrylic", 1.50, 1e6, 1e6))
+    lib.add_surface(Surface("wallSurf", detect=0.5, absorb=0.5))
+    b_world = lib.boundary("Water/wallSurf/wallSurf/Water")
+    b_obj = lib.boundary("Water///Acrylic")
+    fd = Foundry()
+    half = 1.6 * spacing
+    p0 = fd.addPrim(Node.box3(2 * half, 2 * half, 2 * half), b_world)
+    s0 = fd.addSolid("world", [p0])
+    fd.addInstance(np.eye(4), s0)
+    nodes = leaf_zoo_nodes()
+    for k, nd in enumerate(nodes):
+        pk = fd.addPrim(nd, b_obj)
+        sk = fd.addSolid(f"leaf{k}", [pk])
+        t = np.eye(4)
+        t[3, :3] = ((k % 3) - 1) * spacing, ((k // 3) - 1) * spacing, 0.0
+        fd.addInstance(t, sk)
+    phys = lib.arrays()
+    phys["icdf"], phys["icdf_hd_factor"] = gaussian_icdf()
+    phys["boundary_names"] = lib.boundaries
+    gs = np.stack([torch_genstep(num_photon // 4, T_POINT_ISO, pos=p, wavelength=300.0, weight=300.0)
+                   for p in ((200.0, 200.0, 0.0), (-200.0, 200.0, 50.0), (200.0, -200.0, -50.0), (-200.0, -200.0, 0.0))]
+                  ).astype(np.float32)
+    return Workload("leaf_zoo", fd.arrays(), phys, gs, seed=seed, meta=dict(n_leaf_prims=len(nodes)))
+
+
 def uniform_ball(rng, n, r):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)

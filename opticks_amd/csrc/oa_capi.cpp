@@ -123,7 +123,8 @@ struct mi_sim {
     std::vector<std::array<double, 6>> solid_box;   /* padded solid-frame prim box per solid */
     std::vector<float> h_o2w;                       /* (n_inst,16) current instance transforms */
     uint64_t generation = 0;                        /* bumped when a device buffer may move */
-    DBuf<float4> node, itra;
+    int32_t ext_leaves = 0;                         /* geometry uses the extended leaf set */
+    DBuf<float4> node, itra, plan;
     DBuf<InstRec> inst;
     DBuf<PrimRec> prim;
     DBuf<BvhNode> tlas, blas;
@@ -239,8 +240,15 @@ static int validate_geometry(const mi_geometry* g, const mi_physics* ph)
         for (int k = 0; k < num; ++k) {
             const uint32_t* nd = (const uint32_t*)(g->node + 16 * (off + k));
             uint32_t t = nd[14];
-            bool ok = t == 0 || (t >= 1 && t <= 3) || t == 101 || t == 103 || t == 105 || t == 110;
+            /* sphere, zsphere, cylinder, cone, box3, convexpolyhedron, hyperboloid, phicut, halfspace */
+            bool ok = t == 0 || (t >= 1 && t <= 3) || t == 101 || t == 103 || t == 105 || t == 108 || t == 110 ||
+                      t == 112 || t == 117 || t == 121 || t == 125;
             if (!ok) return fail(MI_ERR_UNSUPPORTED, "unsupported CSG node typecode " + std::to_string(t));
+            if (t == 112) {
+                const uint64_t pidx = nd[0], pnum = nd[1];
+                if (!g->plan || pidx + pnum > (uint64_t)g->n_plan)
+                    return fail(MI_ERR_GEOMETRY, "convexpolyhedron planes beyond the plan array");
+            }
             uint32_t gt = nd[15] & 0x7fffffffu;
             if (gt > (uint32_t)g->n_tran) return fail(MI_ERR_GEOMETRY, "gtransformIdx beyond itra");
             if (gt && !g->itra) return fail(MI_ERR_GEOMETRY, "node transform without itra array");
@@ -365,9 +373,16 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
         prim[p].global_prim = (uint32_t)pr[15];
         prim[p].pad = 0;
     }
+    sim->ext_leaves = 0;
+    for (int64_t k = 0; k < g->n_node; ++k) {
+        const uint32_t t = ((const uint32_t*)(g->node + 16 * k))[14];
+        if (t == 108 || t == 112 || t == 117 || t == 121 || t == 125) sim->ext_leaves = 1;
+    }
     HIPCHECK(upload(sim->node, (const float4*)g->node, (size_t)g->n_node * 4, s));
     if (g->n_tran > 0) HIPCHECK(upload(sim->itra, (const float4*)g->itra, (size_t)g->n_tran * 4, s));
     else HIPCHECK(sim->itra.reserve(4));
+    if (g->n_plan > 0 && g->plan) HIPCHECK(upload(sim->plan, (const float4*)g->plan, (size_t)g->n_plan, s));
+    else HIPCHECK(sim->plan.reserve(1));
     HIPCHECK(upload(sim->inst, inst.data(), inst.size(), s));
     HIPCHECK(upload(sim->prim, prim.data(), prim.size(), s));
     HIPCHECK(upload(sim->blas, blas.data(), blas.size(), s));
@@ -450,7 +465,7 @@ void mi_sim_destroy(mi_sim* sim)
 {
     if (!sim) return;
     if (sim->stream) (void)hipStreamSynchronize(sim->stream);
-    sim->node.release(); sim->itra.release(); sim->inst.release(); sim->prim.release();
+    sim->node.release(); sim->itra.release(); sim->plan.release(); sim->inst.release(); sim->prim.release();
     sim->tlas.release(); sim->blas.release(); sim->tlas_items.release(); sim->blas_items.release();
     sim->inst_blas_root.release(); sim->o2w.release(); sim->inst_box.release();
     sim->bnd.release(); sim->optical.release(); sim->icdf.release(); sim->precooked.release();
@@ -489,7 +504,7 @@ static SimParams base_params(const mi_sim* sim)
 {
     SimParams P;
     std::memset(&P, 0, sizeof(P));
-    P.node = sim->node.p; P.itra = sim->itra.p; P.inst = sim->inst.p; P.prim = sim->prim.p;
+    P.node = sim->node.p; P.itra = sim->itra.p; P.plan = sim->plan.p; P.ext_leaves = sim->ext_leaves; P.inst = sim->inst.p; P.prim = sim->prim.p;
     P.tlas = sim->tlas.p; P.tlas_items = sim->tlas_items.p; P.blas = sim->blas.p; P.blas_items = sim->blas_items.p;
     P.bnd = sim->bnd.p; P.optical = sim->optical.p; P.icdf = sim->icdf.p;
     P.n_wl = sim->n_wl; P.wl_low = sim->wl_low; P.wl_step = sim->wl_step;

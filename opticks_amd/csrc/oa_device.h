@@ -82,7 +82,9 @@ enum : uint32_t {
 enum : int { CMD_BREAK = 1, CMD_CONTINUE = 2, CMD_BOUNDARY = 3 };
 enum : int { EMS_NOSURFACE = 1, EMS_SURFACE = 2, EMS_ZPLUS_SENSOR_A = 3 };
 enum : uint32_t { TC_ZERO = 0, TC_UNION = 1, TC_INTERSECTION = 2, TC_DIFFERENCE = 3, TC_NODE = 11,
-                  TC_LEAF = 101, TC_SPHERE = 101, TC_ZSPHERE = 103, TC_CYLINDER = 105, TC_BOX3 = 110 };
+                  TC_LEAF = 101, TC_SPHERE = 101, TC_ZSPHERE = 103, TC_CYLINDER = 105, TC_CONE = 108,
+                  TC_BOX3 = 110, TC_CONVEXPOLYHEDRON = 112, TC_HYPERBOLOID = 117, TC_PHICUT = 121,
+                  TC_HALFSPACE = 125 };
 
 /* ------------------------------------------------------------ random */
 
@@ -312,8 +314,190 @@ OA_D void q_left_inplace(const float4* q, Isect& v, float w)
     v.x = x; v.y = y; v.z = z;
 }
 
-/* intersect_leaf (CSG/csg_intersect_leaf.h:174-324). Returns false for a miss. */
-OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, float t_min, v3 ro, v3 rd)
+#define OA_RT_DEFAULT_MAX 1.e27f
+OA_D float fmin4(float a, float b, float c, float d) { return fminf(fminf(a, b), fminf(c, d)); }
+
+/* truncated cone with caps (CSG/csg_intersect_leaf_newcone.h:45-123) */
+OA_D bool leaf_cone(Isect& is, float4 q0, float t_min, v3 o, v3 d)
+{
+    const float r1 = q0.x, z1 = q0.y, r2 = q0.z, z2 = q0.w;
+    const float r1r1 = r1 * r1;
+    const float r2r2 = r2 * r2;
+    const float tth = (r2 - r1) / (z2 - z1);
+    const float tth2 = tth * tth;
+    const float z0 = (z2 * r1 - z1 * r2) / (r1 - r2);
+    const float idz = 1.f / d.z;
+    float t_cap1 = d.z == 0.f ? OA_RT_DEFAULT_MAX : (z1 - o.z) * idz;
+    float t_cap2 = d.z == 0.f ? OA_RT_DEFAULT_MAX : (z2 - o.z) * idz;
+    const float rr_cap1 = (o.x + t_cap1 * d.x) * (o.x + t_cap1 * d.x) + (o.y + t_cap1 * d.y) * (o.y + t_cap1 * d.y);
+    const float rr_cap2 = (o.x + t_cap2 * d.x) * (o.x + t_cap2 * d.x) + (o.y + t_cap2 * d.y) * (o.y + t_cap2 * d.y);
+    t_cap1 = rr_cap1 < r1r1 && t_cap1 > t_min ? t_cap1 : OA_RT_DEFAULT_MAX;
+    t_cap2 = rr_cap2 < r2r2 && t_cap2 > t_min ? t_cap2 : OA_RT_DEFAULT_MAX;
+    const float c2 = d.x * d.x + d.y * d.y - d.z * d.z * tth2;
+    const float c1 = o.x * d.x + o.y * d.y - (o.z - z0) * d.z * tth2;
+    const float c0 = o.x * o.x + o.y * o.y - (o.z - z0) * (o.z - z0) * tth2;
+    float t_near, t_far, sdisc;
+    rq_roots_disq(OA_RT_DEFAULT_MAX, t_near, t_far, sdisc, c2, c1, c0);
+    const float z_near = o.z + t_near * d.z;
+    const float z_far = o.z + t_far * d.z;
+    t_near = z_near > z1 && z_near < z2 && t_near > t_min ? t_near : OA_RT_DEFAULT_MAX;
+    t_far = z_far > z1 && z_far < z2 && t_far > t_min ? t_far : OA_RT_DEFAULT_MAX;
+    const float t_cand = fmin4(t_near, t_far, t_cap1, t_cap2);
+    const bool valid = t_cand > t_min && t_cand < OA_RT_DEFAULT_MAX;
+    if (valid) {
+        if (t_cand == t_cap1 || t_cand == t_cap2) {
+            is.x = 0.f; is.y = 0.f; is.z = t_cand == t_cap2 ? 1.f : -1.f;
+        } else {
+            const v3 n = normalize(mk(o.x + t_cand * d.x, o.y + t_cand * d.y, (z0 - (o.z + t_cand * d.z)) * tth2));
+            is.x = n.x; is.y = n.y; is.z = n.z;
+        }
+        is.w = t_cand;
+    }
+    return valid;
+}
+
+/* intersection of plane half-spaces plan[planeIdx + i], i < planeNum
+   (CSG/csg_intersect_leaf_convexpolyhedron.h:48-106) */
+OA_D bool leaf_convexpolyhedron(Isect& is, float4 q0, const float4* plan, float t_min, v3 o, v3 d)
+{
+    const uint32_t planeIdx = __float_as_uint(q0.x), planeNum = __float_as_uint(q0.y);
+    float t0 = -__uint_as_float(0x7f800000u), t1 = __uint_as_float(0x7f800000u);
+    v3 t0_normal = mk(0.f, 0.f, 0.f), t1_normal = mk(0.f, 0.f, 0.f);
+    for (uint32_t i = 0; i < planeNum; ++i) {
+        const float4 pl = plan[planeIdx + i];
+        const v3 n = mk(pl.x, pl.y, pl.z);
+        const float dplane = pl.w;
+        const float nd = dot(n, d);
+        const float no = dot(n, o);
+        const float dist = no - dplane;
+        const float t_cand = -dist / nd;
+        const bool parallel_inside = nd == 0.f && dist < 0.f;
+        const bool parallel_outside = nd == 0.f && dist > 0.f;
+        if (parallel_inside) continue;
+        if (parallel_outside) return false;
+        if (nd < 0.f) {
+            if (t_cand > t0) { t0 = t_cand; t0_normal = n; }
+        } else {
+            if (t_cand < t1) { t1 = t_cand; t1_normal = n; }
+        }
+    }
+    const bool valid = t0 < t1;
+    if (valid) {
+        if (t0 > t_min) {
+            is.x = t0_normal.x; is.y = t0_normal.y; is.z = t0_normal.z; is.w = t0;
+        } else if (t1 > t_min) {
+            is.x = t1_normal.x; is.y = t1_normal.y; is.z = t1_normal.z; is.w = t1;
+        }
+    }
+    return valid;
+}
+
+/* one-sheet hyperboloid with caps (CSG/csg_intersect_leaf_hyperboloid.h:34-106) */
+OA_D bool leaf_hyperboloid(Isect& is, float4 q0, float t_min, v3 o, v3 s)
+{
+    const float zero = 0.f, one = 1.f;
+    const float r0 = q0.x, zf = q0.y, z1 = q0.z, z2 = q0.w;
+    const float rr0 = r0 * r0;
+    const float z1s = z1 / zf;
+    const float z2s = z2 / zf;
+    const float rr1 = rr0 * (z1s * z1s + one);
+    const float rr2 = rr0 * (z2s * z2s + one);
+    const float A = -rr0 / (zf * zf);
+    const float B = -rr0;
+    const float dd = s.x * s.x + s.y * s.y + A * s.z * s.z;
+    const float b = o.x * s.x + o.y * s.y + A * o.z * s.z;
+    const float c = o.x * o.x + o.y * o.y + A * o.z * o.z + B;
+    float t1hyp, t2hyp, sdisc;
+    const float disc = b * b - dd * c;
+    rq_roots(t1hyp, t2hyp, sdisc, dd, b, c);
+    const float h1z = o.z + t1hyp * s.z;
+    const float h2z = o.z + t2hyp * s.z;
+    const float osz = one / s.z;
+    const float t2cap = (z2 - o.z) * osz;
+    const float t1cap = (z1 - o.z) * osz;
+    const v3 c1 = o + t1cap * s;
+    const v3 c2 = o + t2cap * s;
+    const float crr1 = c1.x * c1.x + c1.y * c1.y;
+    const float crr2 = c2.x * c2.x + c2.y * c2.y;
+    const float ca = t1hyp > t_min && disc > zero && h1z > z1 && h1z < z2 ? t1hyp : OA_RT_DEFAULT_MAX;
+    const float cb = t2hyp > t_min && disc > zero && h2z > z1 && h2z < z2 ? t2hyp : OA_RT_DEFAULT_MAX;
+    const float cc = t2cap > t_min && crr2 < rr2 ? t2cap : OA_RT_DEFAULT_MAX;
+    const float cd = t1cap > t_min && crr1 < rr1 ? t1cap : OA_RT_DEFAULT_MAX;
+    const float t_cand = fmin4(ca, cb, cc, cd);
+    const bool valid = t_cand > t_min && t_cand < OA_RT_DEFAULT_MAX;
+    if (valid) {
+        is.w = t_cand;
+        if (t_cand == t1hyp || t_cand == t2hyp) {
+            const v3 p = o + t_cand * s;
+            const v3 n = normalize(mk(p.x, p.y, A * p.z));
+            is.x = n.x; is.y = n.y; is.z = n.z;
+        } else {
+            is.x = zero; is.y = zero; is.z = t_cand == t1cap ? -one : one;
+        }
+    }
+    return valid;
+}
+
+/* wedge between the half-planes at phi0, phi1 (intersect_leaf_phicut_simple,
+   CSG/csg_intersect_leaf_phicut.h); a miss that exits at infinity sets y = -0 */
+OA_D bool leaf_phicut(Isect& is, float4 q0, float t_min, v3 o, v3 d)
+{
+    const float cosPhi0 = q0.x, sinPhi0 = q0.y, cosPhi1 = q0.z, sinPhi1 = q0.w;
+    const float d_n0 = d.x * sinPhi0 + d.y * (-cosPhi0);
+    const float d_n1 = d.x * (-sinPhi1) + d.y * (cosPhi1);
+    const float o_n0 = o.x * sinPhi0 + o.y * (-cosPhi0);
+    const float o_n1 = o.x * (-sinPhi1) + o.y * (cosPhi1);
+    float t0 = d_n0 == 0.f ? t_min : -o_n0 / d_n0;
+    float t1 = d_n1 == 0.f ? t_min : -o_n1 / d_n1;
+    const float PR = d_n0 == 0.f ? -o_n0 : -d_n0;
+    const float QR = d_n1 == 0.f ? -o_n1 : -d_n1;
+    const float PQ = cosPhi0 * sinPhi1 - cosPhi1 * sinPhi0;
+    const bool unbounded_exit = PQ >= 0.f ? (PR >= 0.f && QR <= 0.f) : (PR >= 0.f || QR <= 0.f);
+    const float side0 = o.x * cosPhi0 + o.y * sinPhi0 + (d.x * cosPhi0 + d.y * sinPhi0) * t0;
+    const float side1 = o.x * cosPhi1 + o.y * sinPhi1 + (d.x * cosPhi1 + d.y * sinPhi1) * t1;
+    if (side0 < 0.f) t0 = t_min;
+    if (side1 < 0.f) t1 = t_min;
+    const float t_near = fminf(t0, t1);
+    const float t_far = fmaxf(t0, t1);
+    const float t_cand = t_near > t_min ? t_near : (t_far > t_min ? t_far : t_min);
+    const bool valid = t_cand > t_min;
+    if (valid) {
+        is.x = t_cand == t1 ? -sinPhi1 : sinPhi0;
+        is.y = t_cand == t1 ? cosPhi1 : -cosPhi0;
+        is.z = 0.f;
+        is.w = t_cand;
+    } else if (unbounded_exit) {
+        is.y = -is.y;
+    }
+    return valid;
+}
+
+/* the region n.x <= w (CSG/csg_intersect_leaf_halfspace.h) */
+OA_D bool leaf_halfspace(Isect& is, float4 q0, float t_min, v3 o, v3 d)
+{
+    const v3 n = mk(q0.x, q0.y, q0.z);
+    const float w = q0.w;
+    const float on = dot(o, n);
+    const float dn = dot(d, n);
+    const float on_w = on - w;
+    const float adn = fabsf(dn);
+    const float eps = 1e-9f;
+    const bool inside = on_w < -eps;
+    const float t = adn > 0.f ? -on_w / dn : t_min;
+    const bool valid = t > t_min;
+    if (valid) {
+        is.x = n.x; is.y = n.y; is.z = n.z; is.w = t;
+    } else if (inside) {
+        is.y = -0.f;
+    }
+    return valid;
+}
+
+/* intersect_leaf (CSG/csg_intersect_leaf.h:174-324). Returns false for a miss.
+   EXT: also the cone, convexpolyhedron, hyperboloid, phicut and halfspace leaves
+   (a separate kernel instantiation, chosen when the geometry holds them). */
+template <bool EXT>
+OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, const float4* plan, float t_min, v3 ro, v3 rd)
 {
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     const float4 q0 = nd[0];
@@ -331,7 +515,18 @@ OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, float 
         case TC_ZSPHERE:  valid = leaf_zsphere(is, q0, q1, t_min, origin, direction); break;
         case TC_CYLINDER: valid = leaf_cylinder(is, q0, q1, t_min, origin, direction); break;
         case TC_BOX3:     valid = leaf_box3(is, q0, t_min, origin, direction); break;
-        default: break;   /* rejected by the host at create time */
+        default:
+            if constexpr (EXT) {
+                switch (typecode) {
+                    case TC_CONE:     valid = leaf_cone(is, q0, t_min, origin, direction); break;
+                    case TC_CONVEXPOLYHEDRON: valid = leaf_convexpolyhedron(is, q0, plan, t_min, origin, direction); break;
+                    case TC_HYPERBOLOID: valid = leaf_hyperboloid(is, q0, t_min, origin, direction); break;
+                    case TC_PHICUT:   valid = leaf_phicut(is, q0, t_min, origin, direction); break;
+                    case TC_HALFSPACE: valid = leaf_halfspace(is, q0, t_min, origin, direction); break;
+                    default: break;
+                }
+            }
+            break;
     }
     if (valid && q) q_left_inplace(q, is, 0.f);
     if (complement) {
@@ -373,8 +568,9 @@ OA_D uint32_t postorder_next(uint32_t i, uint32_t elev) { return (i & 1u) ? i >>
 #define OA_TRANCHE_STACK 4
 
 /* intersect_tree (CSG/csg_intersect_tree.h:277-672) */
-template <int D>
-OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
+template <int D, bool EXT>
+OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, const float4* plan, float t_min,
+                           v3 ro, v3 rd)
 {
     const uint32_t numNode = __float_as_uint(root[0].x);
     const uint32_t height = (uint32_t)(__builtin_ffs((int)numNode + 1) - 2);
@@ -410,7 +606,7 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, f
             if (typecode >= TC_NODE) {
                 Isect e;
                 OA_COUNT(OA_ST_LEAF);
-                intersect_leaf(e, nd, itra, tmin, ro, rd);
+                intersect_leaf<EXT>(e, nd, itra, plan, tmin, ro, rd);
                 e.w = copysignf(e.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
                 if (sn >= D) { ierr = true; break; }
 #pragma unroll
@@ -476,24 +672,28 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, f
 }
 
 /* deep trees (height 4..OA_CSG_MAX_HEIGHT): out of line, result returned by value */
+template <bool EXT>
 __device__ static __attribute__((noinline)) float4 intersect_tree_deep(const float4* root, const float4* itra,
+                                                                      const float4* plan,
                                                                       float t_min, float ox, float oy, float oz,
                                                                       float dx, float dy, float dz)
 {
     Isect r;
-    intersect_tree_t<8>(r, root, itra, t_min, mk(ox, oy, oz), mk(dx, dy, dz));
+    intersect_tree_t<8, EXT>(r, root, itra, plan, t_min, mk(ox, oy, oz), mk(dx, dy, dz));
     return make_float4(r.x, r.y, r.z, r.w);
 }
 
-OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, float t_min, v3 ro, v3 rd)
+template <bool EXT>
+OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, const float4* plan, float t_min, v3 ro,
+                         v3 rd)
 {
     const uint32_t typecode = __float_as_uint(root[3].z);
-    if (typecode >= TC_LEAF) return intersect_leaf(is, root, itra, t_min, ro, rd);
+    if (typecode >= TC_LEAF) return intersect_leaf<EXT>(is, root, itra, plan, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     if (typecode < TC_NODE) {
         OA_COUNT(OA_ST_TREE);
-        if (__float_as_uint(root[0].x) <= 15u) return intersect_tree_t<4>(is, root, itra, t_min, ro, rd);
-        const float4 r = intersect_tree_deep(root, itra, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
+        if (__float_as_uint(root[0].x) <= 15u) return intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd);
+        const float4 r = intersect_tree_deep<EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
         is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
         return is.w > 0.f;
     }

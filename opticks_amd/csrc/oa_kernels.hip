@@ -121,13 +121,13 @@ OA_D void lane_start(const SimParams& P, Lane& L, int64_t g)
 }
 
 /* one iteration of the CSGOptiX7.cu:477-509 loop; returns true when the photon is done */
-template <int STRIDE, class Tape, bool LITE = false>
+template <int STRIDE, class Tape, bool LITE = false, bool EXT = false>
 OA_D bool lane_step(const SimParams& P, Lane& L, uint32_t* stk, Tape& tape)
 {
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return true;
     const float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Prd prd;
-    trace<STRIDE>(P, stk, L.p.pos, L.p.mom, tmin, P.tmax, prd);
+    trace<STRIDE, EXT>(P, stk, L.p.pos, L.p.mom, tmin, P.tmax, prd);
     if constexpr (LITE) { L.lcost = prd.lposcost; L.lx = prd.lx; L.ly = prd.ly; }
     if (prd.boundary == 0xffffu) return true;
     prd.normal = normalize(prd.normal);
@@ -176,7 +176,8 @@ OA_D void lane_finish(const SimParams& P, const Lane& L)
 
 /* ----------------------------------------------------- plain kernel */
 
-template <int BLOCK, bool LITE>
+/* F: 1 = sphotonlite output, 2 = extended leaves */
+template <int BLOCK, int F>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
@@ -187,8 +188,8 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
         Lane L;
         NoTape tape;
         lane_start(P, L, g);
-        while (!lane_step<BLOCK, NoTape, LITE>(P, L, stk, tape)) {}
-        lane_finish<LITE>(P, L);
+        while (!lane_step<BLOCK, NoTape, (F & 1) != 0, (F & 2) != 0>(P, L, stk, tape)) {}
+        lane_finish<(F & 1) != 0>(P, L);
         hit = (L.p.flagmask & P.hit_mask) != 0u;
     }
     const unsigned long long m = __ballot(hit);
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
 
 /* ------------------------------------------------ persistent kernel */
 
-template <int BLOCK, bool LITE>
+template <int BLOCK, int F>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
@@ -233,8 +234,8 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
         if (lane == 0) { OA_COUNT(OA_ST_ITER); OA_COUNT_N(OA_ST_ACTIVE, __popcll(act)); }
 #endif
         if (active) {
-            if (lane_step<BLOCK, NoTape, LITE>(P, L, stk, tape)) {
-                lane_finish<LITE>(P, L);
+            if (lane_step<BLOCK, NoTape, (F & 1) != 0, (F & 2) != 0>(P, L, stk, tape)) {
+                lane_finish<(F & 1) != 0>(P, L);
                 if ((L.p.flagmask & P.hit_mask) != 0u)
                     atomicOr(P.hitwords + (L.g >> 6), 1ull << (L.g & 63));
                 active = false;
@@ -322,7 +323,7 @@ __global__ void k_hit_event_offsets(const unsigned long long* words, const int32
 
 /* --------------------------------------------------------- trace query */
 
-template <int BLOCK>
+template <int BLOCK, bool EXT>
 __global__ void __launch_bounds__(BLOCK) k_trace(SimParams P, const float4* rays, int64_t n, float4* out)
 {
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
@@ -330,7 +331,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace(SimParams P, const float4* rays
     if (i >= n) return;
     const float4 a = rays[2 * i], b = rays[2 * i + 1];
     Prd prd;
-    trace<BLOCK>(P, s_stack + threadIdx.x, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, prd);
+    trace<BLOCK, EXT>(P, s_stack + threadIdx.x, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, prd);
     const bool miss = prd.boundary == 0xffffu;
     const v3 nn = miss ? prd.normal : normalize(prd.normal);
     out[2 * i] = make_float4(nn.x, nn.y, nn.z, prd.t);
@@ -454,11 +455,19 @@ static hipError_t sim_t(const SimParams& P, int kernel, int grid, hipStream_t s)
 {
     if (kernel == 0) {
         const int64_t nb = (P.n_photon + B - 1) / B;
-        if (P.photonlite) hipLaunchKernelGGL((k_sim_plain<B, true>), dim3((unsigned)nb), dim3(B), 0, s, P);
-        else hipLaunchKernelGGL((k_sim_plain<B, false>), dim3((unsigned)nb), dim3(B), 0, s, P);
+        switch ((P.photonlite ? 1 : 0) | (P.ext_leaves ? 2 : 0)) {
+            case 0: hipLaunchKernelGGL((k_sim_plain<B, 0>), dim3((unsigned)nb), dim3(B), 0, s, P); break;
+            case 1: hipLaunchKernelGGL((k_sim_plain<B, 1>), dim3((unsigned)nb), dim3(B), 0, s, P); break;
+            case 2: hipLaunchKernelGGL((k_sim_plain<B, 2>), dim3((unsigned)nb), dim3(B), 0, s, P); break;
+            default: hipLaunchKernelGGL((k_sim_plain<B, 3>), dim3((unsigned)nb), dim3(B), 0, s, P); break;
+        }
     } else {
-        if (P.photonlite) hipLaunchKernelGGL((k_sim_wave<B, true>), dim3((unsigned)grid), dim3(B), 0, s, P);
-        else hipLaunchKernelGGL((k_sim_wave<B, false>), dim3((unsigned)grid), dim3(B), 0, s, P);
+        switch ((P.photonlite ? 1 : 0) | (P.ext_leaves ? 2 : 0)) {
+            case 0: hipLaunchKernelGGL((k_sim_wave<B, 0>), dim3((unsigned)grid), dim3(B), 0, s, P); break;
+            case 1: hipLaunchKernelGGL((k_sim_wave<B, 1>), dim3((unsigned)grid), dim3(B), 0, s, P); break;
+            case 2: hipLaunchKernelGGL((k_sim_wave<B, 2>), dim3((unsigned)grid), dim3(B), 0, s, P); break;
+            default: hipLaunchKernelGGL((k_sim_wave<B, 3>), dim3((unsigned)grid), dim3(B), 0, s, P); break;
+        }
     }
     return hipGetLastError();
 }
@@ -467,7 +476,7 @@ template <int B>
 static int occ_t()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sim_wave<B, false>, B, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sim_wave<B, 0>, B, 0) != hipSuccess) return 0;
     return nb;
 }
 
@@ -541,7 +550,8 @@ hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s)
 
 hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trace<128>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, P, rays, n, out);
+    if (P.ext_leaves) hipLaunchKernelGGL((k_trace<128, true>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, P, rays, n, out);
+    else hipLaunchKernelGGL((k_trace<128, false>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, P, rays, n, out);
     return hipGetLastError();
 }
 
@@ -646,7 +656,7 @@ OA_D int64_t hit_photon(const SimParams& P, const float4* hits, int64_t h, const
 /* persistent replay: lanes take the next hit of the chunk from a global
    counter (zeroed before the launch) as soon as their photon is done, as
    k_sim_wave does; the score row of hit h is S + (h - begin) * n_param */
-template <int BLOCK>
+template <int BLOCK, bool EXT>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjParams A, const float4* hits,
                                                                int64_t begin, int64_t n_max, const int64_t* hit_off,
                                                                int32_t n_ev, int64_t hit_cap, float* S)
@@ -689,7 +699,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjP
             if (exhausted) break;
             continue;
         }
-        if (active && lane_step<BLOCK>(P, L, stk, tape)) active = false;
+        if (active && lane_step<BLOCK, ScoreTape, false, EXT>(P, L, stk, tape)) active = false;
     }
 }
 
@@ -758,7 +768,7 @@ namespace oa_launch {
 int adjoint_blocks_per_cu()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint_wave<128>, 128, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint_wave<128, false>, 128, 0) != hipSuccess) return 0;
     return nb;
 }
 
@@ -772,8 +782,12 @@ hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_
     hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)std::max(1, grid * 4)), dim3(256), 0, s, d_S, n_param, begin, n_max,
                        d_hit_off, n_ev, hit_cap);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, P.next, (int64_t)1);
-    hipLaunchKernelGGL(k_adjoint_wave<128>, dim3((unsigned)std::max(1, grid)), dim3(128), 0, s, P, A, d_hit, begin, n_max,
-                       d_hit_off, n_ev, hit_cap, d_S);
+    if (P.ext_leaves)
+        hipLaunchKernelGGL((k_adjoint_wave<128, true>), dim3((unsigned)std::max(1, grid)), dim3(128), 0, s, P, A, d_hit,
+                           begin, n_max, d_hit_off, n_ev, hit_cap, d_S);
+    else
+        hipLaunchKernelGGL((k_adjoint_wave<128, false>), dim3((unsigned)std::max(1, grid)), dim3(128), 0, s, P, A, d_hit,
+                           begin, n_max, d_hit_off, n_ev, hit_cap, d_S);
     return hipGetLastError();
 }
 

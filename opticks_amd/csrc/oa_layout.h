@@ -58,6 +58,8 @@ struct SimParams {
     /* geometry */
     const float4*  node;
     const float4*  itra;
+    const float4*  plan;          /* convex polyhedron planes (n.xyz, d) */
+    int32_t        ext_leaves;    /* geometry holds cone/polyhedron/hyperboloid/phicut/halfspace leaves */
     const InstRec* inst;
     const PrimRec* prim;
     const BvhNode* tlas;

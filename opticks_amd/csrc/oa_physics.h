@@ -520,7 +520,7 @@ OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
  * transform, BLAS walk, CSG prim tests) run together, so the expensive CSG
  * code executes with most lanes of the wave active instead of once per lane.
  */
-template <int STRIDE>
+template <int STRIDE, bool EXT = false>
 OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float tmax, Prd& prd)
 {
     OA_COUNT(OA_ST_TRACE);
@@ -623,7 +623,10 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                     Isect is;
                     OA_COUNT(OA_ST_PRIM);
                     OA_WAVE_COUNT(OA_ST_W_PRIM, OA_ST_L_PRIM);
-                    if (!intersect_prim(is, P.node + 4u * pr.node_offset, P.itra, tmin, oo, od)) continue;
+                    if (!intersect_prim<EXT>(is, P.node + 4u * pr.node_offset, P.itra, P.plan, tmin, oo, od)) continue;
+                    /* a reported hit outside [tmin, tmax] is ignored, as optixReportIntersection does
+                       (a convex polyhedron behind the ray reports valid with t = 0) */
+                    if (!(is.w >= tmin)) continue;
                     const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
                     if (is.w < best_t || (is.w == best_t && key < best_key)) {
                         best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;

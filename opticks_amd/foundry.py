@@ -33,7 +33,13 @@ CSG_DIFFERENCE = 3
 CSG_SPHERE = 101
 CSG_ZSPHERE = 103
 CSG_CYLINDER = 105
+CSG_CONE = 108
 CSG_BOX3 = 110
+CSG_CONVEXPOLYHEDRON = 112
+CSG_HYPERBOLOID = 117
+CSG_PHICUT = 121
+CSG_HALFSPACE = 125
+UNBOUNDED = 1.0e9     # AABB half-extent given to unbounded leaves (bounded by their Boolean partners)
 
 SOLID_DTYPE = np.dtype([
     ("label", "S16"), ("numPrim", "<i4"), ("primOffset", "<i4"), ("type", "<i4"),
@@ -46,8 +52,9 @@ class Node:
     """One CSG node: a leaf with parameters or a Boolean operator."""
 
     def __init__(self, typecode, q0=(0, 0, 0, 0), q1=(0, 0), aabb=None, transform=None,
-                 complement=False, left=None, right=None):
+                 complement=False, left=None, right=None, planes=None):
         self.typecode = typecode
+        self.planes = None if planes is None else np.asarray(planes, np.float32).reshape(-1, 4)
         self.q0 = tuple(float(v) for v in q0)
         self.q1 = tuple(float(v) for v in q1)
         self.aabb = aabb
@@ -79,6 +86,38 @@ class Node:
     def box3(fx, fy, fz):
         h = np.array([fx, fy, fz], float) / 2.0
         return Node(CSG_BOX3, (fx, fy, fz, 0.0), aabb=(-h, h))
+
+    @staticmethod
+    def cone(r1, z1, r2, z2):
+        """truncated cone, radius r1 at z1, r2 at z2 (z2 > z1), with caps (CSG_CONE, q0 = r1,z1,r2,z2)"""
+        r = max(r1, r2)
+        return Node(CSG_CONE, (r1, z1, r2, z2), aabb=(np.array([-r, -r, z1]), np.array([r, r, z2])))
+
+    @staticmethod
+    def hyperboloid(r0, zf, z1, z2):
+        """one-sheet hyperboloid x^2+y^2 = r0^2 (1 + (z/zf)^2) between z1 < z2, with caps"""
+        r = r0 * np.sqrt(1.0 + max((z1 / zf) ** 2, (z2 / zf) ** 2))
+        return Node(CSG_HYPERBOLOID, (r0, zf, z1, z2), aabb=(np.array([-r, -r, z1]), np.array([r, r, z2])))
+
+    @staticmethod
+    def phicut(phi0, phi1):
+        """the unbounded wedge phi0 <= phi <= phi1 about the z axis (radians)"""
+        return Node(CSG_PHICUT, (np.cos(phi0), np.sin(phi0), np.cos(phi1), np.sin(phi1)),
+                    aabb=(np.full(3, -UNBOUNDED), np.full(3, UNBOUNDED)))
+
+    @staticmethod
+    def halfspace(normal, w):
+        """the unbounded region n.x <= w (n unit)"""
+        n = np.asarray(normal, float)
+        n = n / np.linalg.norm(n)
+        return Node(CSG_HALFSPACE, (n[0], n[1], n[2], w), aabb=(np.full(3, -UNBOUNDED), np.full(3, UNBOUNDED)))
+
+    @staticmethod
+    def convexpolyhedron(planes, aabb):
+        """intersection of the half-spaces n.x <= d, planes (k,4) rows (nx,ny,nz,d), n unit;
+        aabb (lo, hi) bounds it"""
+        return Node(CSG_CONVEXPOLYHEDRON, aabb=(np.asarray(aabb[0], float), np.asarray(aabb[1], float)),
+                    planes=planes)
 
     @staticmethod
     def op(typecode, left, right):
@@ -126,6 +165,7 @@ class Foundry:
         self.nodes = []
         self.trans = []
         self.insts = []
+        self.plans = []
 
     def addTran(self, t):
         t = np.asarray(t, float)
@@ -141,6 +181,10 @@ class Foundry:
             return rec
         rec[0:4] = nd.q0
         rec[4:6] = nd.q1
+        if nd.planes is not None:
+            u[0] = len(self.plans)            # planeIdx: offset into plan (CSGNode.h:204-207)
+            u[1] = len(nd.planes)             # planeNum
+            self.plans.extend(list(nd.planes))
         u[6] = boundary
         u[7] = index
         if nd.is_leaf():
@@ -225,7 +269,8 @@ class Foundry:
         itra = (np.array([np.linalg.inv(t) for t in self.trans], np.float32).reshape(-1, 4, 4)
                 if self.trans else np.zeros((0, 4, 4), np.float32))
         inst = np.array(self.insts, np.float32).reshape(-1, 4, 4)
-        return dict(node=node, prim=prim, solid=solid, tran=tran, itra=itra, inst=inst)
+        plan = np.array(self.plans, np.float32).reshape(-1, 4)
+        return dict(node=node, prim=prim, solid=solid, tran=tran, itra=itra, inst=inst, plan=plan)
 
 
 def rotation_z_to(direction):

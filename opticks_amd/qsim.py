@@ -134,7 +134,8 @@ class QSim:
         itra = g.get("itra")
         self._geo = L.MiGeometry(_ptr(g["node"]), len(g["node"]), _ptr(g["prim"]), len(g["prim"]),
                                  _ptr(g["solid"]), len(g["solid"]), _ptr(itra), 0 if itra is None else len(itra),
-                                 None, 0, _ptr(g["inst"]), len(g["inst"]))
+                                 _ptr(g.get("plan")), 0 if g.get("plan") is None else len(g["plan"]),
+                                 _ptr(g["inst"]), len(g["inst"]))
         self._bnd = np.ascontiguousarray(physics["bnd"], np.float32)
         self._opt = np.ascontiguousarray(physics["optical"], np.int32)
         icdf = physics.get("icdf")

@@ -57,7 +57,8 @@ enum { EMS_MATERIAL = 0, EMS_NOSURFACE = 1, EMS_SURFACE = 2, EMS_ZPLUS_SENSOR_A 
        EMS_ZPLUS_SENSOR_CUSTOMART = 4, EMS_ZMINUS = 5 };
 /* sysrap/OpticksCSG.h */
 enum { CSG_ZERO = 0, CSG_UNION = 1, CSG_INTERSECTION = 2, CSG_DIFFERENCE = 3, CSG_NODE = 11,
-       CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103, CSG_CYLINDER = 105, CSG_BOX3 = 110 };
+       CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103, CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110,
+       CSG_CONVEXPOLYHEDRON = 112, CSG_HYPERBOLOID = 117, CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
 /* sysrap/OpticksGenstep.h:20-45 */
 enum { GS_TORCH = 6, GS_SCINT_R4695 = 5, GS_SCINTILLATION = 16 };
 /* sysrap/storchtype.h:10-18; T_POINT_ISO is this project's isotropic point source */
@@ -346,7 +347,180 @@ static void leaf_box3(int* valid, f4* isect, const float* q0, float t_min, f3 ro
 }
 
 /* intersect_leaf (CSG/csg_intersect_leaf.h:174-324), non-HEISENBUG form */
-static int intersect_leaf(int* valid, f4* isect, const float* nd, const float* itra, float t_min, f3 ro, f3 rd)
+#define RT_DEFAULT_MAX_F 1.e27f
+static inline float fmin4(float a, float b, float c, float d) { return fminf(fminf(a, b), fminf(c, d)); }
+
+/* csg_intersect_leaf_newcone.h:45-123 intersect_leaf_newcone */
+static void leaf_cone(int* valid, f4* isect, const float* q0, float t_min, f3 o, f3 d)
+{
+    const float r1 = q0[0], z1 = q0[1], r2 = q0[2], z2 = q0[3];
+    const float r1r1 = r1 * r1;
+    const float r2r2 = r2 * r2;
+    const float tth = (r2 - r1) / (z2 - z1);
+    const float tth2 = tth * tth;
+    const float z0 = (z2 * r1 - z1 * r2) / (r1 - r2);
+    const float idz = 1.f / d.z;
+    float t_cap1 = d.z == 0.f ? RT_DEFAULT_MAX_F : (z1 - o.z) * idz;
+    float t_cap2 = d.z == 0.f ? RT_DEFAULT_MAX_F : (z2 - o.z) * idz;
+    const float rr_cap1 = (o.x + t_cap1 * d.x) * (o.x + t_cap1 * d.x) + (o.y + t_cap1 * d.y) * (o.y + t_cap1 * d.y);
+    const float rr_cap2 = (o.x + t_cap2 * d.x) * (o.x + t_cap2 * d.x) + (o.y + t_cap2 * d.y) * (o.y + t_cap2 * d.y);
+    t_cap1 = rr_cap1 < r1r1 && t_cap1 > t_min ? t_cap1 : RT_DEFAULT_MAX_F;
+    t_cap2 = rr_cap2 < r2r2 && t_cap2 > t_min ? t_cap2 : RT_DEFAULT_MAX_F;
+    const float c2 = d.x * d.x + d.y * d.y - d.z * d.z * tth2;
+    const float c1 = o.x * d.x + o.y * d.y - (o.z - z0) * d.z * tth2;
+    const float c0 = o.x * o.x + o.y * o.y - (o.z - z0) * (o.z - z0) * tth2;
+    float t_near, t_far, disc, sdisc;
+    robust_quadratic_roots_disqualifying(RT_DEFAULT_MAX_F, &t_near, &t_far, &disc, &sdisc, c2, c1, c0);
+    const float z_near = o.z + t_near * d.z;
+    const float z_far = o.z + t_far * d.z;
+    t_near = z_near > z1 && z_near < z2 && t_near > t_min ? t_near : RT_DEFAULT_MAX_F;
+    t_far = z_far > z1 && z_far < z2 && t_far > t_min ? t_far : RT_DEFAULT_MAX_F;
+    const float t_cand = fmin4(t_near, t_far, t_cap1, t_cap2);
+    *valid = t_cand > t_min && t_cand < RT_DEFAULT_MAX_F;
+    if (*valid) {
+        if (t_cand == t_cap1 || t_cand == t_cap2) {
+            isect->x = 0.f; isect->y = 0.f; isect->z = t_cand == t_cap2 ? 1.f : -1.f;
+        } else {
+            f3 n = norm3(mk3(o.x + t_cand * d.x, o.y + t_cand * d.y, (z0 - (o.z + t_cand * d.z)) * tth2));
+            isect->x = n.x; isect->y = n.y; isect->z = n.z;
+        }
+        isect->w = t_cand;
+    }
+}
+
+/* csg_intersect_leaf_convexpolyhedron.h:48-106; planes plan[planeIdx + i] (q0.u.x, q0.u.y) */
+static void leaf_convexpolyhedron(int* valid, f4* isect, const float* nd, const float* plan, float t_min, f3 o, f3 d)
+{
+    const uint32_t planeIdx = nodeu(nd)[0], planeNum = nodeu(nd)[1];
+    float t0 = -INFINITY, t1 = INFINITY;
+    f3 t0_normal = mk3(0.f, 0.f, 0.f), t1_normal = mk3(0.f, 0.f, 0.f);
+    for (uint32_t i = 0; i < planeNum; ++i) {
+        const float* pl = plan + 4 * (planeIdx + i);
+        f3 n = mk3(pl[0], pl[1], pl[2]);
+        float dplane = pl[3];
+        float ndd = dot3(n, d);
+        float no = dot3(n, o);
+        float dist = no - dplane;
+        float t_cand = -dist / ndd;
+        int parallel_inside = ndd == 0.f && dist < 0.f;
+        int parallel_outside = ndd == 0.f && dist > 0.f;
+        if (parallel_inside) continue;
+        if (parallel_outside) { *valid = 0; return; }
+        if (ndd < 0.f) {
+            if (t_cand > t0) { t0 = t_cand; t0_normal = n; }
+        } else {
+            if (t_cand < t1) { t1 = t_cand; t1_normal = n; }
+        }
+    }
+    *valid = t0 < t1;
+    if (*valid) {
+        if (t0 > t_min) {
+            isect->x = t0_normal.x; isect->y = t0_normal.y; isect->z = t0_normal.z; isect->w = t0;
+        } else if (t1 > t_min) {
+            isect->x = t1_normal.x; isect->y = t1_normal.y; isect->z = t1_normal.z; isect->w = t1;
+        }
+    }
+}
+
+/* csg_intersect_leaf_hyperboloid.h:34-106 */
+static void leaf_hyperboloid(int* valid, f4* isect, const float* q0, float t_min, f3 o, f3 s)
+{
+    const float zero = 0.f, one = 1.f;
+    const float r0 = q0[0], zf = q0[1], z1 = q0[2], z2 = q0[3];
+    const float rr0 = r0 * r0;
+    const float z1s = z1 / zf;
+    const float z2s = z2 / zf;
+    const float rr1 = rr0 * (z1s * z1s + one);
+    const float rr2 = rr0 * (z2s * z2s + one);
+    const float A = -rr0 / (zf * zf);
+    const float B = -rr0;
+    const float dd = s.x * s.x + s.y * s.y + A * s.z * s.z;
+    const float b = o.x * s.x + o.y * s.y + A * o.z * s.z;
+    const float c = o.x * o.x + o.y * o.y + A * o.z * o.z + B;
+    float t1hyp, t2hyp, disc, sdisc;
+    robust_quadratic_roots(&t1hyp, &t2hyp, &disc, &sdisc, dd, b, c);
+    const float h1z = o.z + t1hyp * s.z;
+    const float h2z = o.z + t2hyp * s.z;
+    float osz = one / s.z;
+    float t2cap = (z2 - o.z) * osz;
+    float t1cap = (z1 - o.z) * osz;
+    const f3 c1 = add3(o, smul3(t1cap, s));
+    const f3 c2 = add3(o, smul3(t2cap, s));
+    float crr1 = c1.x * c1.x + c1.y * c1.y;
+    float crr2 = c2.x * c2.x + c2.y * c2.y;
+    const float ca = t1hyp > t_min && disc > zero && h1z > z1 && h1z < z2 ? t1hyp : RT_DEFAULT_MAX_F;
+    const float cb = t2hyp > t_min && disc > zero && h2z > z1 && h2z < z2 ? t2hyp : RT_DEFAULT_MAX_F;
+    const float cc = t2cap > t_min && crr2 < rr2 ? t2cap : RT_DEFAULT_MAX_F;
+    const float cd = t1cap > t_min && crr1 < rr1 ? t1cap : RT_DEFAULT_MAX_F;
+    float t_cand = fmin4(ca, cb, cc, cd);
+    *valid = t_cand > t_min && t_cand < RT_DEFAULT_MAX_F;
+    if (*valid) {
+        isect->w = t_cand;
+        if (t_cand == t1hyp || t_cand == t2hyp) {
+            const f3 p = add3(o, smul3(t_cand, s));
+            f3 n = norm3(mk3(p.x, p.y, A * p.z));
+            isect->x = n.x; isect->y = n.y; isect->z = n.z;
+        } else {
+            isect->x = zero; isect->y = zero; isect->z = t_cand == t1cap ? -one : one;
+        }
+    }
+}
+
+/* csg_intersect_leaf_phicut.h intersect_leaf_phicut_simple: the wedge between
+   the half-planes at phi0 and phi1; a miss that exits at infinity flags -0 in y */
+static void leaf_phicut(int* valid, f4* isect, const float* q0, float t_min, f3 o, f3 d)
+{
+    const float cosPhi0 = q0[0], sinPhi0 = q0[1], cosPhi1 = q0[2], sinPhi1 = q0[3];
+    float d_n0 = d.x * sinPhi0 + d.y * (-cosPhi0);
+    float d_n1 = d.x * (-sinPhi1) + d.y * (cosPhi1);
+    float o_n0 = o.x * sinPhi0 + o.y * (-cosPhi0);
+    float o_n1 = o.x * (-sinPhi1) + o.y * (cosPhi1);
+    float t0 = d_n0 == 0.f ? t_min : -o_n0 / d_n0;
+    float t1 = d_n1 == 0.f ? t_min : -o_n1 / d_n1;
+    const float PR = d_n0 == 0.f ? -o_n0 : -d_n0;
+    const float QR = d_n1 == 0.f ? -o_n1 : -d_n1;
+    const float PQ = cosPhi0 * sinPhi1 - cosPhi1 * sinPhi0;
+    int unbounded_exit = PQ >= 0.f ? (PR >= 0.f && QR <= 0.f) : (PR >= 0.f || QR <= 0.f);
+    float side0 = o.x * cosPhi0 + o.y * sinPhi0 + (d.x * cosPhi0 + d.y * sinPhi0) * t0;
+    float side1 = o.x * cosPhi1 + o.y * sinPhi1 + (d.x * cosPhi1 + d.y * sinPhi1) * t1;
+    if (side0 < 0.f) t0 = t_min;
+    if (side1 < 0.f) t1 = t_min;
+    float t_near = fminf(t0, t1);
+    float t_far = fmaxf(t0, t1);
+    float t_cand = t_near > t_min ? t_near : (t_far > t_min ? t_far : t_min);
+    *valid = t_cand > t_min;
+    if (*valid) {
+        isect->x = t_cand == t1 ? -sinPhi1 : sinPhi0;
+        isect->y = t_cand == t1 ? cosPhi1 : -cosPhi0;
+        isect->z = 0.f;
+        isect->w = t_cand;
+    } else if (unbounded_exit) {
+        isect->y = -isect->y;
+    }
+}
+
+/* csg_intersect_leaf_halfspace.h intersect_leaf_halfspace: n.x <= w side */
+static void leaf_halfspace(int* valid, f4* isect, const float* q0, float t_min, f3 o, f3 d)
+{
+    f3 n = mk3(q0[0], q0[1], q0[2]);
+    float w = q0[3];
+    float on = dot3(o, n);
+    float dn = dot3(d, n);
+    float on_w = on - w;
+    float adn = fabsf(dn);
+    float eps = 1e-9f;
+    int inside = on_w < -eps;
+    float t = adn > 0.f ? -on_w / dn : t_min;
+    *valid = t > t_min;
+    if (*valid) {
+        isect->x = n.x; isect->y = n.y; isect->z = n.z; isect->w = t;
+    } else if (inside) {
+        isect->y = -0.f;
+    }
+}
+
+static int intersect_leaf(int* valid, f4* isect, const float* nd, const float* itra, const float* plan, float t_min,
+                          f3 ro, f3 rd)
 {
     *valid = 0;
     isect->x = 0.f; isect->y = 0.f; isect->z = 0.f; isect->w = 0.f;
@@ -361,6 +535,11 @@ static int intersect_leaf(int* valid, f4* isect, const float* nd, const float* i
         case CSG_ZSPHERE:  leaf_zsphere(valid, isect, nd, nd + 4, t_min, origin, direction); break;
         case CSG_CYLINDER: leaf_cylinder(valid, isect, nd, nd + 4, t_min, origin, direction); break;
         case CSG_BOX3:     leaf_box3(valid, isect, nd, t_min, origin, direction); break;
+        case CSG_CONE:     leaf_cone(valid, isect, nd, t_min, origin, direction); break;
+        case CSG_CONVEXPOLYHEDRON: leaf_convexpolyhedron(valid, isect, nd, plan, t_min, origin, direction); break;
+        case CSG_HYPERBOLOID: leaf_hyperboloid(valid, isect, nd, t_min, origin, direction); break;
+        case CSG_PHICUT:   leaf_phicut(valid, isect, nd, t_min, origin, direction); break;
+        case CSG_HALFSPACE: leaf_halfspace(valid, isect, nd, t_min, origin, direction); break;
         default: return -1;   /* leaf type not restated here */
     }
     if (*valid && q) q_left_multiply_inplace(q, isect, 0.f);
@@ -426,7 +605,8 @@ static inline unsigned postorder_next(unsigned i, unsigned elev) { return (i & 1
 #define PACK_SLICE(b, e) ((((b) & 0xffu) << 16) | (((e) & 0xffu) << 24))
 
 /* intersect_tree (CSG/csg_intersect_tree.h:277-672) */
-static int intersect_tree(int* valid, f4* isect, const float* node, const float* itra, float t_min, f3 ro, f3 rd)
+static int intersect_tree(int* valid, f4* isect, const float* node, const float* itra, const float* plan, float t_min,
+                          f3 ro, f3 rd)
 {
     int numNode = (int)nd_subNum(node);
     unsigned height = tree_height((unsigned)numNode);
@@ -453,7 +633,7 @@ static int intersect_tree(int* valid, f4* isect, const float* node, const float*
             if (node_or_leaf) {
                 f4 nd_isect = mk4(0.f, 0.f, 0.f, 0.f);
                 int nd_valid = 0;
-                if (intersect_leaf(&nd_valid, &nd_isect, nd, itra, tmin, ro, rd) != 0) return -1;
+                if (intersect_leaf(&nd_valid, &nd_isect, nd, itra, plan, tmin, ro, rd) != 0) return -1;
                 nd_isect.w = copysignf(nd_isect.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
                 if (csg_curr >= CSG_STACK_SIZE - 1) { ierr = ERR_OVERFLOW; break; }
                 csg_curr++; csg[csg_curr] = nd_isect;
@@ -521,12 +701,13 @@ static int intersect_tree(int* valid, f4* isect, const float* node, const float*
 }
 
 /* intersect_prim (CSG/csg_intersect_tree.h:684-719); list nodes not restated */
-static int intersect_prim(int* valid, f4* isect, const float* node, const float* itra, float t_min, f3 ro, f3 rd)
+static int intersect_prim(int* valid, f4* isect, const float* node, const float* itra, const float* plan, float t_min,
+                          f3 ro, f3 rd)
 {
     uint32_t typecode = nd_typecode(node);
     *valid = 0;
-    if (typecode >= CSG_LEAF) return intersect_leaf(valid, isect, node, itra, t_min, ro, rd);
-    if (typecode < CSG_NODE) return intersect_tree(valid, isect, node, itra, t_min, ro, rd);
+    if (typecode >= CSG_LEAF) return intersect_leaf(valid, isect, node, itra, plan, t_min, ro, rd);
+    if (typecode < CSG_NODE) return intersect_tree(valid, isect, node, itra, plan, t_min, ro, rd);
     return -1;
 }
 
@@ -538,7 +719,7 @@ int oracle_intersect_prim(const oracle_geom* g, int32_t prim_idx, float t_min,
     const float* root = g->node + 16 * pr[1];
     f4 is = mk4(0.f, 0.f, 0.f, 0.f);
     int valid = 0;
-    int rc = intersect_prim(&valid, &is, root, g->itra, t_min, mk3(o[0], o[1], o[2]), mk3(d[0], d[1], d[2]));
+    int rc = intersect_prim(&valid, &is, root, g->itra, g->plan, t_min, mk3(o[0], o[1], o[2]), mk3(d[0], d[1], d[2]));
     isect4[0] = is.x; isect4[1] = is.y; isect4[2] = is.z; isect4[3] = is.w;
     return rc != 0 ? rc : valid;
 }
@@ -711,8 +892,9 @@ static int test_instance(const oworld* w, int64_t i, f3 o, f3 d, float tmin, flo
         const float* root = w->g.node + 16 * pr[1];
         f4 is = mk4(0.f, 0.f, 0.f, 0.f);
         int valid = 0;
-        if (intersect_prim(&valid, &is, root, w->g.itra, tmin, oo, od) != 0) return -1;
+        if (intersect_prim(&valid, &is, root, w->g.itra, w->g.plan, tmin, oo, od) != 0) return -1;
         if (!valid) continue;
+        if (!(is.w >= tmin)) continue;   /* outside [tmin, tmax]: ignored like optixReportIntersection */
         uint64_t key = ((uint64_t)i << 32) | (uint64_t)p;
         if (is.w < *best_t || (is.w == *best_t && key < *best_key)) {
             *best_t = is.w; *best_key = key;

@@ -38,6 +38,7 @@ typedef struct oracle_geom {
     const void*  solid; int64_t n_solid;
     const float* itra;  int64_t n_tran;
     const float* inst;  int64_t n_inst;
+    const float* plan;  int64_t n_plan;   /* (n_plan,4) planes of convex polyhedra, may be NULL */
 } oracle_geom;
 
 typedef struct oracle_phys {

@@ -23,7 +23,16 @@ class OracleGeom(C.Structure):
                 ("prim", C.c_void_p), ("n_prim", C.c_int64),
                 ("solid", C.c_void_p), ("n_solid", C.c_int64),
                 ("itra", C.c_void_p), ("n_tran", C.c_int64),
-                ("inst", C.c_void_p), ("n_inst", C.c_int64)]
+                ("inst", C.c_void_p), ("n_inst", C.c_int64),
+                ("plan", C.c_void_p), ("n_plan", C.c_int64)]
+
+
+def _geom(g):
+    """OracleGeom over the (kept alive by the caller) contiguous arrays of g"""
+    plan = g.get("plan")
+    return OracleGeom(_p(g["node"]), len(g["node"]), _p(g["prim"]), len(g["prim"]),
+                      _p(g["solid"]), len(g["solid"]), _p(g["itra"]), len(g["itra"]),
+                      _p(g["inst"]), len(g["inst"]), _p(plan), 0 if plan is None else len(plan))
 
 
 class OraclePhys(C.Structure):
@@ -120,9 +129,7 @@ class World:
     def __init__(self, geometry, physics):
         self._keep = {k: np.ascontiguousarray(v) for k, v in geometry.items()}
         g = self._keep
-        self.geom = OracleGeom(_p(g["node"]), len(g["node"]), _p(g["prim"]), len(g["prim"]),
-                               _p(g["solid"]), len(g["solid"]), _p(g["itra"]), len(g["itra"]),
-                               _p(g["inst"]), len(g["inst"]))
+        self.geom = _geom(g)
         self._bnd = np.ascontiguousarray(physics["bnd"], np.float32)
         self._opt = np.ascontiguousarray(physics["optical"], np.int32)
         icdf = physics.get("icdf")
@@ -225,9 +232,7 @@ class World:
 
 def intersect_prim(geometry, prim_idx, t_min, origin, direction):
     g = {k: np.ascontiguousarray(v) for k, v in geometry.items()}
-    geom = OracleGeom(_p(g["node"]), len(g["node"]), _p(g["prim"]), len(g["prim"]),
-                      _p(g["solid"]), len(g["solid"]), _p(g["itra"]), len(g["itra"]),
-                      _p(g["inst"]), len(g["inst"]))
+    geom = _geom(g)
     o = np.ascontiguousarray(origin, np.float32)
     d = np.ascontiguousarray(direction, np.float32)
     out = np.zeros(4, np.float32)
@@ -322,7 +327,8 @@ def ref_intersect_prim(geometry, prim_idx, t_min, origin, direction):
     itra = np.ascontiguousarray(geometry["itra"], np.float32)
     prim = np.ascontiguousarray(geometry["prim"], np.float32)
     off = int(prim.view(np.int32)[prim_idx, 0, 1])
-    plan = np.zeros((1, 4), np.float32)
+    plan = geometry.get("plan")
+    plan = np.ascontiguousarray(plan, np.float32) if plan is not None and len(plan) else np.zeros((1, 4), np.float32)
     o = np.ascontiguousarray(origin, np.float32)
     d = np.ascontiguousarray(direction, np.float32)
     out = np.zeros(4, np.float32)

@@ -7,6 +7,9 @@ Writes the golden fixtures of tests/golden from the reference checkout:
                   intersected by the reference's own CSG code
                   (CSG/csg_intersect_tree.h intersect_prim) compiled into
                   oracle/_ref/libref_csg.so by oracle/Makefile.
+  csg_golden_leaves.npz  rays against the leaf-zoo prims (cone, hyperboloid, convex
+                  polyhedra, phicut and halfspace inside Boolean trees, a
+                  transformed cone), intersected by the same reference code.
   csg_lut.json    the packed Boolean LUT words parsed from CSG/csg_classify.h.
 
 Run in the container that holds the reference (python tests/golden/make_golden.py);
@@ -68,8 +71,39 @@ def csg():
                 valid=a[:, 8].astype(np.int32), isect=a[:, 9:13].astype(np.float32))
 
 
+def leaves():
+    import pyoracle as po
+    from opticks_amd import configs
+    po.build()
+    assert po.reflib() is not None, "oracle/_ref/libref_csg.so missing"
+    w = configs.leaf_zoo(num_photon=4)
+    g = w.geometry
+    # prim k+1 is leaf_zoo_nodes()[k], in its own solid frame (instance transforms not applied)
+    rng = np.random.default_rng(20261203)
+    rows = []
+    for prim in range(1, len(g["prim"])):
+        for k in range(700):
+            o = rng.uniform(-260, 260, 3)
+            if k % 7 == 0:
+                o = rng.uniform(-30, 30, 3)              # from inside / near the axis
+            d = rng.normal(size=3)
+            if k % 11 == 0:
+                d = np.zeros(3); d[k % 3] = 1.0 if k % 2 else -1.0
+            if k % 13 == 0 and k % 11 != 0:
+                d[2] = 0.0                               # parallel to cap planes
+            d /= np.linalg.norm(d)
+            tmin = [0.0, 0.05, 1.0][k % 3]
+            v, isect = po.ref_intersect_prim(g, prim, np.float32(tmin), o.astype(np.float32), d.astype(np.float32))
+            rows.append((prim, tmin, *o.astype(np.float32), *d.astype(np.float32), int(v), *isect))
+    a = np.array(rows, np.float64)
+    return dict(prim=a[:, 0].astype(np.int32), tmin=a[:, 1].astype(np.float32),
+                origin=a[:, 2:5].astype(np.float32), direction=a[:, 5:8].astype(np.float32),
+                valid=a[:, 8].astype(np.int32), isect=a[:, 9:13].astype(np.float32))
+
+
 if __name__ == "__main__":
     with open(os.path.join(HERE, "csg_lut.json"), "w") as f:
         json.dump(lut(), f, indent=1)
     np.savez_compressed(os.path.join(HERE, "csg_golden.npz"), **csg())
+    np.savez_compressed(os.path.join(HERE, "csg_golden_leaves.npz"), **leaves())
     print("wrote", os.listdir(HERE))

@@ -149,3 +149,57 @@ def test_boolean_trees_against_point_membership():
             assert (np.dot(i[:3], d) < 0) == (not before)
             checked += 1
         assert checked > 100
+
+
+@pytest.fixture(scope="module")
+def zoo():
+    return configs.leaf_zoo(num_photon=4).geometry
+
+
+def test_leaf_golden_vectors_bitwise(zoo):
+    """cone, hyperboloid, convex polyhedra, phicut and halfspace (in trees), transformed
+    cone: oracle == reference CSG code on committed vectors"""
+    z = np.load(os.path.join(GOLD, "csg_golden_leaves.npz"))
+    bad = []
+    for i in range(len(z["prim"])):
+        v, isect = po.intersect_prim(zoo, int(z["prim"][i]), z["tmin"][i], z["origin"][i], z["direction"][i])
+        if int(v) != int(z["valid"][i]) or not np.array_equal(isect.view(np.uint32), z["isect"][i].view(np.uint32)):
+            bad.append(i)
+    assert not bad, (len(bad), bad[:5])
+    for p in range(1, 10):
+        assert z["valid"][z["prim"] == p].sum() > 50          # every prim is hit
+
+
+@pytest.mark.skipif(po.reflib() is None, reason="oracle/_ref not built (reference checkout absent)")
+def test_leaf_random_rays_vs_reference_build(zoo):
+    rng = np.random.default_rng(8)
+    bad = 0
+    for k in range(6000):
+        prim = 1 + k % 9
+        o = rng.uniform(-300, 300, 3).astype(np.float32)
+        d = rng.normal(size=3); d = (d / np.linalg.norm(d)).astype(np.float32)
+        t = np.float32([0.0, 0.05, 2.0][k % 3])
+        v1, i1 = po.intersect_prim(zoo, prim, t, o, d)
+        v2, i2 = po.ref_intersect_prim(zoo, prim, t, o, d)
+        bad += (v1 != v2) or not np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
+    assert bad == 0
+
+
+def test_leaf_closed_forms(zoo):
+    """cone side / cap distances, the hyperboloid waist, a cube face, and the halfspace cut"""
+    # cone prim 1: r 100 at z -50 to r 30 at z 80; a ray along +x at z = 15 meets the side at
+    # r(z) = 100 - 70 (z + 50) / 130
+    r15 = 100.0 - 70.0 * 65.0 / 130.0
+    v, i = po.intersect_prim(zoo, 1, 0.0, (-500.0, 0.0, 15.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - (500.0 - r15)) < 1e-3
+    v, i = po.intersect_prim(zoo, 1, 0.0, (0.0, 0.0, -500.0), (0.0, 0.0, 1.0))
+    assert v and abs(i[3] - 450.0) < 1e-3 and i[2] == -1.0           # bottom cap
+    # hyperboloid prim 2: waist radius 60 at z = 0
+    v, i = po.intersect_prim(zoo, 2, 0.0, (-500.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 440.0) < 1e-3 and abs(i[0] + 1.0) < 1e-6
+    # cube prim 3: faces at +-70
+    v, i = po.intersect_prim(zoo, 3, 0.0, (0.0, -500.0, 3.0), (0.0, 1.0, 0.0))
+    assert v and abs(i[3] - 430.0) < 1e-4 and tuple(i[:3]) == (0.0, -1.0, 0.0)
+    # sphere r 140 minus the halfspace z <= 40 (prim 7): from below along +z the first surface is z = 40
+    v, i = po.intersect_prim(zoo, 7, 0.0, (0.0, 0.0, -500.0), (0.0, 0.0, 1.0))
+    assert v and abs(i[3] - 540.0) < 1e-3

@@ -114,9 +114,11 @@ def test_photonlite_output_matches_checker_and_merges():
     assert np.all(G[:, 0] >> 16 == 1)
     assert np.array_equal(G[:, 1], u[:, 0, 3])
     assert np.array_equal(G[:, 3], u[:, 3, 3])
-    # detected photons hit the detector front: lposcost > 0
+    # most detected photons arrive from the centre onto the detector front (local z > 0),
+    # some on the side or neck (lposcost <= 0, clamped to 0 by the 16-bit packing)
     sd = (u[:, 3, 3] & SD) != 0
-    assert sd.sum() > 100 and np.all((G[sd, 2] >> 16) > 0)
+    assert sd.sum() > 100 and ((G[sd, 2] >> 16) > 0).mean() > 0.5
+    assert np.ptp((G[sd, 2] & 0xFFFF).astype(np.int64)) > 30000      # azimuths spread round the axis
     # same results with and without the lite instantiation
     p2 = torch.empty_like(photon)
     sim.simulate_device(gs, p2, None, None, n_photon=N)
