@@ -154,6 +154,13 @@ class QSim:
         self.photon = self.seq = self.hit = None
         self.device_ms = None
 
+    @classmethod
+    def load(cls, base, config: SEventConfig | None = None):
+        """geometry and physics from a reference-layout folder (CSGFoundry::Load); see loader.py"""
+        from .loader import load_csgfoundry
+        geometry, physics = load_csgfoundry(base)
+        return cls(geometry, physics, config)
+
     def close(self):
         if getattr(self, "handle", None):
             L.lib().mi_sim_destroy(self.handle)

@@ -123,3 +123,53 @@ int main() {
     subprocess.run(cmd, check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     assert out.startswith("opticks_amd") and "caught: mi_sim_create:" in out
+
+
+def test_cpp_npy_reader_and_folder_load(tmp_path):
+    """mi_npy.hh reads the reference-layout arrays as numpy wrote them, and
+    MI355XSimulator::Load passes them to mi_sim_create: on a host without a GPU
+    the geometry validates and creation stops at the device query."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    from opticks_amd.loader import save_csgfoundry
+    w = configs.leaf_zoo(num_photon=10)
+    cf = save_csgfoundry(str(tmp_path / "geom"), w.geometry, w.physics)
+    src = tmp_path / "npy.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "MI355XSimulator.hh"
+int main(int argc, char** argv) {
+    std::string cf = argv[1];
+    const char* names[] = {"solid", "prim", "node", "plan", "inst", "SSim/stree/standard/bnd",
+                           "SSim/stree/standard/optical", "SSim/stree/standard/icdf"};
+    for (const char* n : names) {
+        mi_npy::Array a = mi_npy::load(cf + "/" + n + ".npy");
+        unsigned long long sum = 0;        /* checksum of the 32-bit words as read */
+        if (a.kind == 'f') for (float v : a.as_f32()) { unsigned u; std::memcpy(&u, &v, 4); sum += u; }
+        else for (int32_t v : a.as_i32()) sum += (unsigned)v;
+        std::printf("%s %c%d %lld %llu\n", n, a.kind, a.itemsize, (long long)a.size(), sum);
+    }
+    std::printf("domain_low %g\n", mi_npy::meta_value(cf + "/SSim/stree/standard/bnd_meta.txt", "domain_low", -1));
+    try { delete MI355XSimulator::Load(argv[2]); std::printf("created\n"); }
+    catch (const std::exception& e) { std::printf("caught: %s\n", e.what()); }
+    return 0;
+}
+''')
+    exe = tmp_path / "npy"
+    lib = os.path.join(ROOT, "opticks_amd")
+    cmd = ["g++", "-std=c++17", "-I", os.path.join(ROOT, "opticks_amd", "csrc"), str(src), "-o", str(exe),
+           "-L", lib, "-lopticks_amd", f"-Wl,-rpath,{lib}", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True)
+    out = subprocess.run([str(exe), cf, str(tmp_path / "geom")], capture_output=True, text=True, check=True).stdout
+    lines = dict(l.split(" ", 1) for l in out.strip().splitlines())
+    ref = {"solid": w.geometry["solid"].view(np.int32), "prim": w.geometry["prim"], "node": w.geometry["node"],
+           "plan": w.geometry["plan"], "inst": w.geometry["inst"], "SSim/stree/standard/bnd": w.physics["bnd"],
+           "SSim/stree/standard/optical": w.physics["optical"], "SSim/stree/standard/icdf": w.physics["icdf"]}
+    for k, a in ref.items():
+        kind, n, s = lines[k].split()
+        assert int(n) == a.size, k
+        assert int(s) == int(np.ascontiguousarray(a).view(np.uint32).astype(np.uint64).sum()), k
+    assert lines["domain_low"] == "60"
+    assert "created" in out or "no HIP device" in out
