@@ -18,7 +18,8 @@
  *     (qudarap/QSim.cc:436-536, CSGOptiX.cc:1091-1189)    (device-resident arrays)
  *   evt->photonlite output (CSGOptiX7.cu:520-526)       mi_sim_simulate_lite_device
  *   SSimulator::reset(eventID) (G4CXOpticks.cc:491-511)  mi_sim_reset
- *   CSGOptiX simtrace (CSGOptiX7.cu:563-604)            mi_sim_trace_device
+ *   CSGOptiX simtrace (CSGOptiX7.cu:563-604)            mi_sim_simtrace_device
+ *   nearest-hit query (CSGOptiX7.cu:113-218)             mi_sim_trace_device
  *   (no counterpart: batched events in one launch)      mi_sim_simulate_batch_device
  *   QBnd upload of bnd (qudarap/QBnd.cc:130-175)         mi_sim_set_bnd (new values,
  *                                                         same shape, no rebuild)
@@ -227,6 +228,24 @@ int  mi_sim_simulate_batch_device(mi_sim* sim, int32_t n_event,
  */
 int  mi_sim_trace_device(mi_sim* sim, const void* d_rays, int64_t n,
                          void* d_isect, void* stream);
+
+/*
+ * Simtrace (CSGOptiX7.cu:563-604, the reference's SRG_SIMTRACE raygen): one
+ * nearest-hit ray per photon of OpticksGenstep_FRAME gensteps (gencode 17;
+ * q0 = (gencode, gridaxes, ., numphoton), q1.xyz = local origin, q2..q5 the
+ * local->world qat4), directions from qsim::generate_photon_simtrace_frame
+ * (qudarap/qsim.h:2460-2522) with RNG stream (event 0, photon index).
+ * d_simtrace receives (n_photon,4,4) f32 sevent::add_simtrace records
+ * (sysrap/sevent.h:670-696):
+ *   q0 world normal.xyz, t       q1 pos + t dir, tmin
+ *   q2 pos, u32 globalPrimIdx<<16 | boundary
+ *   q3 dir, u32 iindex<<16 | identity
+ * A miss writes q0 = (0.6, 0.6, 0.6, 1), the miss program's midgrey
+ * background (CSGOptiX/SBT.cc:185-191), with both u32 words 0xffffffff.
+ * n_photon < 0 takes the count from the gensteps.
+ */
+int  mi_sim_simtrace_device(mi_sim* sim, const void* d_gensteps, int64_t n_gs,
+                            int64_t n_photon, void* d_simtrace, void* stream);
 
 /* Replace instance transforms (n_inst,4,4) and refit the TLAS on the GPU. */
 int  mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst);

@@ -86,6 +86,96 @@ def scint_genstep(numphoton, pos, time=0.0, delta=(0, 0, 0), step_length=0.0, ch
     return g.reshape(6, 4)
 
 
+OpticksGenstep_CERENKOV = 15
+
+
+def cerenkov_genstep(numphoton, matline, pos, time, delta, beta, wmin=200.0, wmax=600.0, n_max=1.333,
+                     n1=None, n2=None, charge=-1.0, weight=1.0, post_beta=None):
+    """quad6 genstep with the scerenkov layout (sysrap/scerenkov.h:29-58) for a charged
+    track step of displacement `delta` (mm) at speed beta (post-step speed post_beta);
+    matline is the bnd line of the medium (boundary*4 + 3 for a boundary's inner material)."""
+    g = np.zeros(24, np.float32)
+    u = g.view(np.uint32)
+    u[0] = OpticksGenstep_CERENKOV
+    u[1] = 0
+    u[2] = matline
+    u[3] = numphoton
+    g[4:7] = pos
+    g[7] = time
+    g[8:11] = delta
+    step_length = float(np.linalg.norm(delta))
+    g[11] = step_length
+    g[12] = 0.0
+    g[13] = charge
+    g[14] = weight
+    c = 299.792458
+    g[15] = beta * c
+    bi = 1.0 / beta
+    max_cos = bi / n_max
+    g[16] = bi
+    g[17] = wmin
+    g[18] = wmax
+    g[19] = max_cos
+    g[20] = (1.0 - max_cos) * (1.0 + max_cos)
+    g[21] = numphoton if n1 is None else n1
+    g[22] = numphoton if n2 is None else n2
+    g[23] = (beta if post_beta is None else post_beta) * c
+    return g.reshape(6, 4)
+
+
+OpticksGenstep_FRAME = 17
+GRID_XYZ, GRID_YZ, GRID_XZ, GRID_XY = 0, 1, 2, 3
+
+
+def grid_axes(nx, ny, nz):
+    """SGenstep::GridAxes (sysrap/SGenstep.h:138-154): planar grids have one zero extent."""
+    if nx == 0 and ny > 0 and nz > 0:
+        return GRID_YZ
+    if nx > 0 and ny == 0 and nz > 0:
+        return GRID_XZ
+    if nx > 0 and ny > 0 and nz == 0:
+        return GRID_XY
+    return GRID_XYZ
+
+
+def frame_gensteps(ce, cegs, gridscale=0.1, geotran=None, ce_offset=((0.0, 0.0, 0.0),), ce_scale=True):
+    """OpticksGenstep_FRAME gensteps of a simtrace grid, as SFrameGenstep::MakeCenterExtentGenstep
+    (sysrap/SFrameGenstep.cc:566-662) lays them out:
+      ce      (cx, cy, cz, extent) of the frame
+      cegs    (ix0, ix1, iy0, iy1, iz0, iz1, photons_per_genstep, high): grid index ranges
+              (inclusive), photons per grid point and a resolution multiplier 1..8
+      geotran 4x4 local->world transform in row-vector form (translation in the last row),
+              identity when None
+    One genstep per (offset, ix, iy, iz); q0 = (17, gridaxes, gsid, photons), q1 = (offset, 1),
+    q2..q5 = translate(ix, iy, iz scaled) followed by geotran, as a qat4."""
+    ix0, ix1, iy0, iy1, iz0, iz1, ppg, high = (int(v) for v in cegs)
+    assert 1 <= high <= 8
+    scale = float(gridscale) / high
+    ix0, ix1, iy0, iy1, iz0, iz1 = (v * high for v in (ix0, ix1, iy0, iy1, iz0, iz1))
+    axes = grid_axes((ix1 - ix0) // 2, (iy1 - iy0) // 2, (iz1 - iz0) // 2)
+    local_scale = scale * float(ce[3]) if ce_scale else scale
+    G = np.eye(4) if geotran is None else np.asarray(geotran, np.float64).reshape(4, 4)
+    out = []
+    for ip, off in enumerate(ce_offset):
+        for ix in range(ix0, ix1 + 1):
+            for iy in range(iy0, iy1 + 1):
+                for iz in range(iz0, iz1 + 1):
+                    S = np.eye(4)
+                    S[3, :3] = (ix * local_scale, iy * local_scale, iz * local_scale)
+                    T = (S @ G).astype(np.float32)
+                    g = np.zeros(24, np.float32)
+                    u = g.view(np.uint32)
+                    u[0] = OpticksGenstep_FRAME
+                    u[1] = axes
+                    u[2] = np.array([ix, iy, iz, ip], np.int8).view(np.uint32)[0]   # SGenstep::GenstepID
+                    u[3] = abs(ppg)
+                    g[4:7] = off
+                    g[7] = 1.0
+                    g[8:24] = T.reshape(-1)
+                    out.append(g.reshape(6, 4))
+    return np.stack(out) if out else np.zeros((0, 6, 4), np.float32)
+
+
 class Workload:
     def __init__(self, name, geometry, physics, gensteps, seed=42, events=None, meta=None):
         self.name = name
@@ -121,6 +211,27 @@ def detector_tree():
     neck = Node.cylinder(127.0, -250.0, -40.0)
     bore = Node.cylinder(120.0, -260.0, -60.0)
     return Node.op(CSG_DIFFERENCE, Node.op(CSG_UNION, ell, neck), bore)
+
+
+def cerenkov_water(num_photon=200_000, seed=42, n_gs=20):
+    """config0's water sphere with Cerenkov gensteps of 10 mm track steps inside the
+    water (beta 0.99 -> 0.985, 200-600 nm, mean photon yield falling 100 -> 80 along
+    each step), from numpy.random.default_rng(seed) positions and directions."""
+    w = config0(num_photon=1, seed=seed)
+    water_line = w.physics["boundary_names"].index("Vacuum///Water") * 4 + 3
+    rng = np.random.default_rng(seed)
+    per = num_photon // n_gs
+    gs = []
+    for k in range(n_gs):
+        pos = rng.uniform(-400.0, 400.0, 3)
+        d = rng.normal(size=3)
+        d = 10.0 * d / np.linalg.norm(d)
+        gs.append(cerenkov_genstep(per, water_line, pos, float(rng.uniform(0, 1)), d, 0.99,
+                                   n1=100.0, n2=80.0, post_beta=0.985))
+    w.gensteps = np.stack(gs).astype(np.float32)
+    w.name = "cerenkov_water"
+    w.meta.update(water_line=water_line)
+    return w
 
 
 def config0(num_photon=1_000_000, seed=42, water_absorption=20000.0, water_rayleigh=30000.0, sensor=False):

@@ -800,6 +800,30 @@ int mi_sim_reset(mi_sim* sim, int32_t event_id)
     return MI_OK;
 }
 
+int mi_sim_simtrace_device(mi_sim* sim, const void* d_gensteps, int64_t n_gs, int64_t n_photon, void* d_simtrace,
+                           void* stream)
+{
+    if (!sim) return fail(MI_ERR_ARG, "null sim");
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    if (!d_gensteps || !d_simtrace) return fail(MI_ERR_ARG, "null simtrace buffers");
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    HIPCHECK(sim->prefix.reserve((size_t)n_gs + 1));
+    HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, sim->prefix.p, s));
+    int64_t total = 0;
+    HIPCHECK(hipMemcpyAsync(&total, sim->prefix.p + n_gs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (n_photon >= 0 && n_photon != total) return fail(MI_ERR_ARG, "n_photon does not match the gensteps");
+    if (total == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    SimParams P = base_params(sim);
+    P.gensteps = (const float*)d_gensteps;
+    P.gs_prefix = sim->prefix.p;
+    P.n_gs = n_gs;
+    P.n_photon = total;
+    HIPCHECK(oa_launch::simtrace(P, (float4*)d_simtrace, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
 int mi_sim_trace_device(mi_sim* sim, const void* d_rays, int64_t n, void* d_isect, void* stream)
 {
     if (!sim || !d_rays || !d_isect || n < 0) return fail(MI_ERR_ARG, "bad trace arguments");

@@ -127,6 +127,7 @@ struct Prd {
     v3 normal; float t;
     float lposcost;
     float lx, ly;                 /* local hit position x, y (azimuth for sphotonlite); miss: 0, NaN */
+    uint32_t gprim;               /* globalPrimIdx of the hit prim (simtrace records) */
     uint32_t iindex_identity, boundary;
 };
 

@@ -15,6 +15,7 @@ hipError_t compact_hits(const unsigned long long* d_words, int64_t n_words, int3
 hipError_t event_hit_offsets(const unsigned long long* d_words, const int32_t* d_word_prefix,
                              const int64_t* d_block_total, const int64_t* d_ev_ph_begin, int32_t n_event,
                              int64_t n_words, int64_t* d_hit_offsets, hipStream_t s);
+hipError_t simtrace(const SimParams& P, float4* out, hipStream_t s);
 hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out, hipStream_t s);
 hipError_t refit(const float* d_o2w, const BvhNode* d_blas, const uint32_t* d_inst_blas_root, int64_t n_inst,
                  float* d_inst_box, BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth,

@@ -109,7 +109,9 @@ OA_D void lane_start(const SimParams& P, Lane& L, int64_t g)
     }
     const float4* gq = reinterpret_cast<const float4*>(P.gensteps + 24 * gsi);
     const int gencode = (int)__float_as_uint(gq[0].x);
+    /* qsim::generate_photon dispatch (qudarap/qsim.h:2532-2561) */
     if (gencode == 6) gen_torch(L.p, L.rng, gq, photon_idx);
+    else if (gencode == 15 || gencode == 18) gen_cerenkov(P, L.p, L.rng, gq);
     else gen_scint(P, L.p, L.rng, gq);
     /* sphoton::set_index (sysrap/sphoton.h:230-237) */
     L.p.index = (uint32_t)(photon_idx & 0xffffffffu);
@@ -242,6 +244,51 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
             }
         }
     }
+}
+
+/* ------------------------------------------------------------ simtrace */
+
+/* CSGOptiX7.cu:563-604 simtrace: one ray per photon of OpticksGenstep_FRAME
+   gensteps, generated by qsim::generate_photon_simtrace_frame
+   (qudarap/qsim.h:2460-2522) on RNG stream (event 0, photon index), recorded as
+   sevent::add_simtrace (sysrap/sevent.h:670-696); a miss writes q0 = (0.6,0.6,0.6,1),
+   the miss program's midgrey background (CSGOptiX7.cu:683-700, CSGOptiX/SBT.cc:185-191) */
+template <int BLOCK, bool EXT>
+__global__ void __launch_bounds__(BLOCK) OA_OCC k_simtrace(SimParams P, float4* out)
+{
+    __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
+    const int64_t idx = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= P.n_photon) return;
+    const int64_t gsi = upper_index(P.gs_prefix, 0, P.n_gs, idx);
+    const float4* gq = reinterpret_cast<const float4*>(P.gensteps + 24 * gsi);
+    Rng rng;
+    rng.kind = 0;
+    oa_rng_init(&rng.ph, P.seed, P.rng_offset, P.skipahead, 0ull, (uint64_t)idx);
+    const uint32_t gridaxes = __float_as_uint(gq[0].y);
+    const float u0 = uni(rng);
+    float sinPhi, cosPhi;
+    oa_sincosf(2.f * OA_PI_F * u0, &sinPhi, &cosPhi);
+    const float u1 = uni(rng);
+    const float cosTheta = 2.f * u1 - 1.f;
+    const float sinTheta = sqrtf(1.f - cosTheta * cosTheta);
+    v3 dir = gridaxes == 1u ? mk(0.f, cosPhi, sinPhi)
+           : gridaxes == 2u ? mk(cosPhi, 0.f, sinPhi)
+           : gridaxes == 3u ? mk(cosPhi, sinPhi, 0.f)
+           : mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
+    const float4 q1 = gq[1];
+    const v3 pos = q_right(gq + 2, mk(q1.x, q1.y, q1.z), 1.f);
+    dir = q_right(gq + 2, dir, 0.f);
+    Prd prd;
+    trace<BLOCK, EXT>(P, s_stack + threadIdx.x, pos, dir, P.tmin, P.tmax, prd);
+    const bool miss = prd.boundary == 0xffffu;
+    const float t = miss ? 1.f : prd.t;
+    const uint32_t gpb = miss ? 0xffffffffu : (((prd.gprim & 0xffffu) << 16) | (prd.boundary & 0xffffu));
+    const uint32_t ii = miss ? 0xffffffffu : prd.iindex_identity;
+    float4* o = out + 4 * idx;
+    o[0] = miss ? make_float4(0.6f, 0.6f, 0.6f, t) : make_float4(prd.normal.x, prd.normal.y, prd.normal.z, t);
+    o[1] = make_float4(pos.x + t * dir.x, pos.y + t * dir.y, pos.z + t * dir.z, P.tmin);
+    o[2] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(gpb));
+    o[3] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(ii));
 }
 
 /* ---------------------------------------------------- hit compaction */
@@ -545,6 +592,14 @@ hipError_t prefix_check(const int64_t* a, const int64_t* b, int64_t n, unsigned 
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t simtrace(const SimParams& P, float4* out, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((P.n_photon + 127) / 128);
+    if (P.ext_leaves) hipLaunchKernelGGL((k_simtrace<128, true>), dim3(nb), dim3(128), 0, s, P, out);
+    else hipLaunchKernelGGL((k_simtrace<128, false>), dim3(nb), dim3(128), 0, s, P, out);
     return hipGetLastError();
 }
 

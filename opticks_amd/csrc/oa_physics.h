@@ -451,6 +451,59 @@ OA_D void gen_scint(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
     zero_flags_set(p, F_SCINTILLATION);
 }
 
+/* qcerenkov::generate (qudarap/qcerenkov.h): wavelength by reciprocal sampling
+   flat in energy with rejection on sin^2(theta) against the medium's RINDEX
+   from the bnd table (line = genstep matline), cone about the normalized step,
+   emission point along the step by rejection on the linear N1 -> N2 yield */
+OA_D void gen_cerenkov(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
+{
+    const float4 q0 = gq[0], q1 = gq[1], q2 = gq[2], q3 = gq[3], q4 = gq[4], q5 = gq[5];
+    const int matline = (int)__float_as_uint(q0.z);
+    const v3 dpos = mk(q2.x, q2.y, q2.z);
+    const float step_length = q2.w, preVelocity = q3.w;
+    const float BetaInverse = q4.x, Wmin = q4.y, Wmax = q4.z;
+    const float maxSin2 = q5.x, N1 = q5.y, N2 = q5.z, postVelocity = q5.w;
+    const v3 p0 = normalize(dpos);
+    float wavelength, cosTheta, sin2Theta, u_maxSin2;
+    unsigned count = 0;
+    do {
+        const float u0 = uni(rng);
+        const float w = Wmin + u0 * (Wmax - Wmin);
+        wavelength = Wmin * Wmax / w;
+        const float4 props = bnd_lookup(P, wavelength, matline, 0);
+        const float sampledRI = props.x;
+        cosTheta = BetaInverse / sampledRI;
+        sin2Theta = fmaxf(0.f, (1.f - cosTheta) * (1.f + cosTheta));
+        const float u1 = uni(rng);
+        u_maxSin2 = u1 * maxSin2;
+        count += 1;
+    } while (u_maxSin2 > sin2Theta && count < 100);
+    const float sinTheta = sqrtf(sin2Theta);
+    const float u0 = uni(rng);
+    const float phi = 2.f * OA_PI_F * u0;
+    float sinPhi, cosPhi;
+    oa_sincosf(phi, &sinPhi, &cosPhi);
+    p.mom = mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
+    rotateUz(p.mom, p0);
+    p.pol = mk(cosTheta * cosPhi, cosTheta * sinPhi, -sinTheta);
+    rotateUz(p.pol, p0);
+    p.wavelength = wavelength;
+    float fraction, delta, NumberOfPhotons, N, u;
+    const float MeanNumberOfPhotonsMax = fmaxf(N1, N2);
+    const float DeltaN = (N1 - N2);
+    do {
+        fraction = uni(rng);
+        delta = fraction * step_length;
+        NumberOfPhotons = N1 - fraction * DeltaN;
+        u = uni(rng);
+        N = u * MeanNumberOfPhotonsMax;
+    } while (N > NumberOfPhotons);
+    const float midVelocity = preVelocity + fraction * (postVelocity - preVelocity) * 0.5f;
+    p.time = q1.w + delta / midVelocity;
+    p.pos = mk(q1.x, q1.y, q1.z) + fraction * dpos;
+    zero_flags_set(p, F_CERENKOV);
+}
+
 /* ------------------------------------------------------------- trace */
 
 /* slab test; on a hit tn is the entry distance clamped to tmin */
@@ -650,7 +703,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #endif
     if (best_inst == 0xffffffffu) {
         prd.normal = mk(0.f, 0.f, 0.f); prd.t = 0.f; prd.lposcost = 0.f;
-        prd.lx = 0.f; prd.ly = __uint_as_float(0x7fc00000u);
+        prd.lx = 0.f; prd.ly = __uint_as_float(0x7fc00000u); prd.gprim = 0xffffu;
         prd.iindex_identity = 0xffffffffu; prd.boundary = 0xffffu;
         return;
     }
@@ -669,6 +722,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     const uint32_t ident = __float_as_uint(r1.w);
     prd.iindex_identity = ((best_inst & 0xffffu) << 16) | (ident & 0xffffu);
     prd.boundary = P.prim[best_prim].boundary & 0xffffu;
+    prd.gprim = P.prim[best_prim].global_prim;
 }
 
 } // namespace oa

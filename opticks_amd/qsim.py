@@ -335,6 +335,20 @@ class QSim:
         L.check(L.lib().mi_sim_trace_device(self.handle, rays_t.data_ptr(), rays_t.shape[0], isect_t.data_ptr(),
                                             None), "mi_sim_trace_device")
 
+    def simtrace_device(self, gs_t, out_t=None, n_photon=-1):
+        """Simtrace of OpticksGenstep_FRAME gensteps (CSGOptiX7.cu:563-604): (n_photon,4,4)
+        f32 sevent::add_simtrace records, written into out_t (allocated when None)."""
+        import torch
+        if n_photon < 0:
+            n_photon = int(gs_t[:, 0, 3].view(torch.int32).to(torch.int64).sum().item())
+        if out_t is None:
+            out_t = torch.empty((n_photon, 4, 4), dtype=torch.float32, device=gs_t.device)
+        if n_photon == 0:
+            return out_t
+        L.check(L.lib().mi_sim_simtrace_device(self.handle, gs_t.data_ptr(), gs_t.shape[0], n_photon,
+                                               out_t.data_ptr(), None), "mi_sim_simtrace_device")
+        return out_t
+
     def refit_instances(self, inst):
         inst = np.ascontiguousarray(inst, np.float32).reshape(-1, 4, 4)
         L.check(L.lib().mi_sim_refit_instances(self.handle, inst.ctypes.data, len(inst)), "mi_sim_refit_instances")

@@ -60,7 +60,7 @@ enum { CSG_ZERO = 0, CSG_UNION = 1, CSG_INTERSECTION = 2, CSG_DIFFERENCE = 3, CS
        CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103, CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110,
        CSG_CONVEXPOLYHEDRON = 112, CSG_HYPERBOLOID = 117, CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
 /* sysrap/OpticksGenstep.h:20-45 */
-enum { GS_TORCH = 6, GS_SCINT_R4695 = 5, GS_SCINTILLATION = 16 };
+enum { GS_TORCH = 6, GS_SCINT_R4695 = 5, GS_CERENKOV = 15, GS_SCINTILLATION = 16, GS_G4CERENKOV_MODIFIED = 18 };
 /* sysrap/storchtype.h:10-18; T_POINT_ISO is this project's isotropic point source */
 enum { T_UNDEF, T_DISC, T_LINE, T_POINT, T_CIRCLE, T_RECTANGLE, T_SPHERE_MARSAGLIA, T_SPHERE,
        T_POINT_ISO = 100 };
@@ -1513,6 +1513,66 @@ static void scint_generate(oph* p, orng* rng, const float* gs, const oracle_phys
     ph_set_flag(p, F_SCINTILLATION);
 }
 
+/* qcerenkov::generate (qudarap/qcerenkov.h:generate, wavelength_sampled_bndtex,
+   fraction_sampled) on the scerenkov genstep layout (sysrap/scerenkov.h:29-58):
+   q1 pos,time  q2 DeltaPosition,step_length  q3 code,charge,weight,preVelocity
+   q4 BetaInverse,Wmin,Wmax,maxCos  q5 maxSin2,MeanNumberOfPhotons1,2,postVelocity */
+static void cerenkov_generate(oph* p, orng* rng, const float* gs, const oracle_phys* ph)
+{
+    const int matline = ((const int32_t*)gs)[2];
+    const f3 gpos = mk3(gs[4], gs[5], gs[6]);
+    const float gtime = gs[7];
+    const f3 dpos = mk3(gs[8], gs[9], gs[10]);
+    const float step_length = gs[11];
+    const float preVelocity = gs[15];
+    const float BetaInverse = gs[16], Wmin = gs[17], Wmax = gs[18];
+    const float maxSin2 = gs[20], N1 = gs[21], N2 = gs[22], postVelocity = gs[23];
+    memset(p, 0, sizeof(*p));
+    const f3 p0 = norm3(dpos);
+    /* wavelength_sampled_bndtex: reciprocal sampling flat in energy, rejection on sin^2 */
+    float wavelength, cosTheta, sin2Theta;
+    unsigned count = 0;
+    float u_maxSin2;
+    do {
+        float u0 = ou(rng);
+        float w = Wmin + u0 * (Wmax - Wmin);
+        wavelength = Wmin * Wmax / w;
+        f4 props = bnd_lookup(ph, wavelength, matline, 0);
+        float sampledRI = props.x;
+        cosTheta = BetaInverse / sampledRI;
+        sin2Theta = fmaxf(0.f, (1.f - cosTheta) * (1.f + cosTheta));
+        float u1 = ou(rng);
+        u_maxSin2 = u1 * maxSin2;
+        count += 1;
+    } while (u_maxSin2 > sin2Theta && count < 100);
+    float sinTheta = sqrtf(sin2Theta);
+    float u0 = ou(rng);
+    float phi = 2.f * PI_F * u0;
+    float sinPhi, cosPhi;
+    oa_sincosf(phi, &sinPhi, &cosPhi);
+    p->mom = mk3(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
+    rotateUz(&p->mom, p0);
+    p->pol = mk3(cosTheta * cosPhi, cosTheta * sinPhi, -sinTheta);
+    rotateUz(&p->pol, p0);
+    p->wavelength = wavelength;
+    /* fraction_sampled: position along the step with the linear N1 -> N2 yield */
+    float fraction, delta, NumberOfPhotons, N, u;
+    float MeanNumberOfPhotonsMax = fmaxf(N1, N2);
+    float DeltaN = (N1 - N2);
+    do {
+        fraction = ou(rng);
+        delta = fraction * step_length;
+        NumberOfPhotons = N1 - fraction * DeltaN;
+        u = ou(rng);
+        N = u * MeanNumberOfPhotonsMax;
+    } while (N > NumberOfPhotons);
+    float midVelocity = preVelocity + fraction * (postVelocity - preVelocity) * 0.5f;
+    p->time = gtime + delta / midVelocity;
+    p->pos = add3(gpos, smul3(fraction, dpos));
+    ph_zero_flags(p);
+    ph_set_flag(p, F_CERENKOV);
+}
+
 /* ---------------------------------------------------------- simulate */
 
 static inline void seq_add_nibble(uint64_t* seqhis, uint64_t* seqbnd, unsigned slot, unsigned flag, unsigned boundary)
@@ -1554,6 +1614,7 @@ static int simulate_one(const oworld* w, const oracle_phys* ph, const oracle_cfg
     int gencode = ((const int32_t*)gs)[0];
     if (gencode == GS_TORCH) torch_generate(p, &rng, gs, photon_idx);
     else if (gencode == GS_SCINTILLATION || gencode == GS_SCINT_R4695) scint_generate(p, &rng, gs, ph);
+    else if (gencode == GS_CERENKOV || gencode == GS_G4CERENKOV_MODIFIED) cerenkov_generate(p, &rng, gs, ph);
     else { memset(p, 0, sizeof(*p)); bad |= 2; }
     ph_set_index(p, photon_idx);
     seqhis[0] = seqhis[1] = seqbnd[0] = seqbnd[1] = 0;
@@ -1676,6 +1737,71 @@ int oracle_adjoint_rows(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
     }
     free(off);
     return bad ? -bad : 0;
+}
+
+/*
+ * Simtrace: qsim::generate_photon_simtrace_frame (qudarap/qsim.h:2460-2522) for
+ * OpticksGenstep_FRAME gensteps (q0 = gencode, gridaxes, gsid, numphoton; q1 =
+ * local position; q2..q5 = the frame transform as a qat4), the trace of
+ * CSGOptiX7.cu:563-604 on RNG stream (event 0, photon index), and the record
+ * of sevent::add_simtrace (sysrap/sevent.h:670-696):
+ *   q0 hit normal (world, not normalized) and t   q1 pos + t dir, tmin
+ *   q2 start position, globalPrimIdx<<16|boundary  q3 direction, iindex<<16|identity
+ * A miss writes q0 = (0.6,0.6,0.6,1): the miss program's background colour and
+ * t = 1 (CSGOptiX7.cu:683-700), so q1 = pos + dir.
+ */
+enum { GRID_XYZ = 0, GRID_YZ = 1, GRID_XZ = 2, GRID_XY = 3 };
+
+static void simtrace_photon(const oworld* w, const oracle_cfg* cfg, const float* gs, uint64_t photon_idx, float* rec)
+{
+    orng rng;
+    rng.kind = 0;
+    oa_rng_init(&rng.ph, cfg->seed, cfg->rng_offset, cfg->event_skipahead, 0, photon_idx);
+    const int gridaxes = ((const int32_t*)gs)[1];
+    const float* qt = gs + 8;                     /* q2..q5 */
+    float u0 = ou(&rng);
+    float sinPhi, cosPhi;
+    oa_sincosf(2.f * PI_F * u0, &sinPhi, &cosPhi);
+    float u1 = ou(&rng);
+    float cosTheta = 2.f * u1 - 1.f;
+    float sinTheta = sqrtf(1.f - cosTheta * cosTheta);
+    f3 dir;
+    switch (gridaxes) {
+        case GRID_YZ: dir = mk3(0.f, cosPhi, sinPhi); break;
+        case GRID_XZ: dir = mk3(cosPhi, 0.f, sinPhi); break;
+        case GRID_XY: dir = mk3(cosPhi, sinPhi, 0.f); break;
+        default:      dir = mk3(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta); break;
+    }
+    f3 pos = q_right_multiply(qt, mk3(gs[4], gs[5], gs[6]), 1.f);
+    dir = q_right_multiply(qt, dir, 0.f);
+    oprd prd;
+    world_trace(w, pos, dir, cfg->tmin, cfg->tmax, &prd, cfg->traverse);
+    const int miss = (prd.gp_boundary & 0xffffu) == 0xffffu;
+    const float t = miss ? 1.f : prd.t;
+    /* a miss carries the miss program's background colour, midgrey (CSGOptiX/SBT.cc:185-191) */
+    rec[0] = miss ? 0.6f : prd.normal.x; rec[1] = miss ? 0.6f : prd.normal.y; rec[2] = miss ? 0.6f : prd.normal.z; rec[3] = t;
+    rec[4] = pos.x + t * dir.x; rec[5] = pos.y + t * dir.y; rec[6] = pos.z + t * dir.z; rec[7] = cfg->tmin;
+    rec[8] = pos.x; rec[9] = pos.y; rec[10] = pos.z;
+    uint32_t gpb = miss ? 0xffffffffu : prd.gp_boundary, ii = miss ? 0xffffffffu : prd.iindex_identity;
+    memcpy(rec + 11, &gpb, 4);
+    rec[12] = dir.x; rec[13] = dir.y; rec[14] = dir.z;
+    memcpy(rec + 15, &ii, 4);
+}
+
+int64_t oracle_simtrace(void* wp, const oracle_cfg* cfg, const float* gensteps, int64_t n_gs, float* out)
+{
+    oworld* w = (oworld*)wp;
+    if (!w || !cfg || n_gs <= 0 || !gensteps || !out) return -1;
+    int64_t* off = genstep_offsets(gensteps, n_gs);
+    const int64_t total = off[n_gs];
+#ifdef _OPENMP
+    if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t idx = 0; idx < total; ++idx)
+        simtrace_photon(w, cfg, gensteps + 24 * genstep_of(off, n_gs, idx), (uint64_t)idx, out + 16 * idx);
+    free(off);
+    return total;
 }
 
 /* ------------------------------------------------------- test helpers */

@@ -103,6 +103,10 @@ int64_t oracle_simulate_ex(void* world, const oracle_phys* ph, const oracle_cfg*
 float oracle_atan2f(float y, float x);
 void  oracle_atan2f_n(const float* y, const float* x, float* out, int64_t n);
 
+/* Simtrace records (n,4,4) of OpticksGenstep_FRAME gensteps (layout as
+   mi_sim_simtrace_device). Returns the photon count or <0. */
+int64_t oracle_simtrace(void* world, const oracle_cfg* cfg, const float* gensteps, int64_t n_gs, float* out);
+
 /* Nearest hit for n rays; same record layout as mi_sim_trace_device. */
 int oracle_trace(void* world, const float* rays, int64_t n, float* isect, int32_t traverse);
 

@@ -78,6 +78,8 @@ def lib():
                                          C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                          C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_simulate_ex.restype = C.c_int64
+        L.oracle_simtrace.argtypes = [C.c_void_p, C.POINTER(OracleCfg), C.c_void_p, C.c_int64, C.c_void_p]
+        L.oracle_simtrace.restype = C.c_int64
         L.oracle_atan2f_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_adjoint_rows.argtypes = [C.c_void_p, C.POINTER(OraclePhys), C.POINTER(OracleCfg), C.c_int32,
                                           C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
@@ -220,6 +222,19 @@ class World:
         if rc < 0:
             raise RuntimeError(f"oracle_adjoint_rows failed rc={rc}")
         return rows
+
+    def simtrace(self, gensteps, traverse=1, **cfgkw):
+        """(n,4,4) f32 simtrace records of OpticksGenstep_FRAME gensteps (oracle_simtrace)."""
+        gs = np.ascontiguousarray(gensteps, np.float32).reshape(-1, 6, 4)
+        total = int(gs.view(np.uint32)[:, 0, 3].sum())
+        out = np.zeros((total, 4, 4), np.float32)
+        if total == 0:
+            return out
+        cfg = self.cfg(traverse=traverse, **cfgkw)
+        rc = lib().oracle_simtrace(self.handle, C.byref(cfg), gs.ctypes.data, len(gs), out.ctypes.data)
+        if rc != total:
+            raise RuntimeError(f"oracle_simtrace failed rc={rc}")
+        return out
 
     def trace(self, rays, traverse=1):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 2, 4)

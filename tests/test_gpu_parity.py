@@ -262,3 +262,45 @@ def test_leaf_zoo_trace_and_simulate_match_checker():
     s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
     assert s >= 0.999 and len(hits) > 1000
     sim.close()
+
+
+def test_cerenkov_gensteps_match_checker():
+    """Cerenkov generation (qcerenkov::generate) + propagation, bitwise with the checker,
+    for both the plain and the persistent kernel"""
+    w = configs.cerenkov_water(num_photon=200_000)
+    ref_p, ref_s = po.World(w.geometry, w.physics).simulate(w.gensteps, seed=w.seed)
+    for kernel in (PLAIN, WAVE):
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=kernel))
+        sim.simulate(w.gensteps, 0)
+        s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+        assert s >= 0.999
+        assert np.all((sim.photon[:, 3, 3].view(np.uint32) & 0x1) != 0)       # CK in every flagmask
+        sim.close()
+
+
+@pytest.mark.parametrize("which", ["config0", "config1"])
+def test_simtrace_matches_checker(which):
+    """Simtrace records (frame gensteps -> grid rays -> nearest hit, sevent::add_simtrace
+    layout) bitwise with the checker, planar and volumetric grids, including misses"""
+    if which == "config0":
+        w = configs.config0(num_photon=10)
+        grids = [((0, 0, 0, 1000.0), (-8, 8, 0, 0, -8, 8, 40, 1), 0.3),
+                 ((0, 0, 0, 1000.0), (-3, 3, -3, 3, -3, 3, 20, 1), 0.6)]
+    else:
+        w = configs.config1(num_gs=1, per_gs=10)
+        grids = [((0, 0, 0, 20000.0), (-16, 16, -16, 16, 0, 0, 30, 1), 0.06)]
+    W = po.World(w.geometry, w.physics)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    for ce, cegs, scale in grids:
+        gs = configs.frame_gensteps(ce, cegs, gridscale=scale)
+        ref = W.simtrace(gs, seed=w.seed)
+        print(which, cegs, len(ref), flush=True)
+        out = sim.simtrace_device(torch.from_numpy(gs).to(DEV)).cpu().numpy()
+        assert out.shape == ref.shape
+        same = np.all(out.view(np.uint32) == ref.view(np.uint32), axis=(1, 2))
+        assert same.mean() >= 0.999, same.mean()
+        miss = ref.view(np.uint32)[:, 2, 3] == 0xFFFFFFFF
+        assert np.array_equal(out.view(np.uint32)[:, 2, 3] == 0xFFFFFFFF, miss)
+    with pytest.raises(RuntimeError):
+        sim.simtrace_device(torch.from_numpy(gs).to(DEV), n_photon=len(ref) + 1)
+    sim.close()
