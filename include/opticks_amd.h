@@ -247,6 +247,12 @@ int  mi_sim_trace_device(mi_sim* sim, const void* d_rays, int64_t n,
 int  mi_sim_simtrace_device(mi_sim* sim, const void* d_gensteps, int64_t n_gs,
                             int64_t n_photon, void* d_simtrace, void* stream);
 
+/* Host form of the same (QSim::simtrace, qudarap/QSim.cc:876-910): gensteps in host
+   memory; *simtrace points at (n_simtrace,4,4) records owned by the handle
+   until the next simtrace/reset/destroy; *device_ms (may be NULL) the device time. */
+int  mi_sim_simtrace(mi_sim* sim, const float* gensteps, int64_t n_gs,
+                     const float** simtrace, int64_t* n_simtrace, double* device_ms);
+
 /* Replace instance transforms (n_inst,4,4) and refit the TLAS on the GPU. */
 int  mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst);
 

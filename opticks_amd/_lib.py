@@ -71,7 +71,7 @@ SYMBOLS = ["mi_config_default", "mi_last_error", "mi_version", "mi_sim_create", 
            "mi_sim_last_adjoint_timing", "mi_batch_create", "mi_batch_launch", "mi_batch_wait",
            "mi_batch_num_photon", "mi_batch_destroy", "mi_sim_autotune", "mi_batch_create_adjoint",
            "mi_batch_last_timing", "mi_sim_merge_hits_device", "mi_sim_simulate_lite_device",
-           "mi_sim_simtrace_device"]
+           "mi_sim_simtrace_device", "mi_sim_simtrace"]
 
 _lib = None
 
@@ -101,6 +101,8 @@ def lib():
                                                C.c_void_p, C.c_void_p]
     L.mi_sim_trace_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
     L.mi_sim_simtrace_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]
+    L.mi_sim_simtrace.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.POINTER(C.c_float)),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.mi_sim_refit_instances.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     L.mi_sim_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.mi_sim_set_bnd.argtypes = [C.c_void_p, C.c_void_p]

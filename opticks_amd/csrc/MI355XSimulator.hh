@@ -7,7 +7,11 @@
  *   SSimulator method            here
  *   simulate_launch / launch     propagate the collected gensteps of the current event
  *   simulate(eventID, reset)     same, returns device ms; reset=true releases the event
- *   simtrace / simtrace_launch   not part of this path: return -1 (unsupported)
+ *   simtrace(eventID) / simtrace_launch
+ *                                the collected OpticksGenstep_FRAME gensteps traced
+ *                                as a simtrace grid (QSim::simtrace,
+ *                                qudarap/QSim.cc:876-910), returns device ms;
+ *                                records (n,4,4) via simtrace_records()
  *   render / render_launch       not part of this path: return -1 (unsupported)
  *   reset(eventID)               drop gensteps and the event's outputs
  *   desc()                       one-line description
@@ -94,7 +98,7 @@ public:
 
     /* SSimulator protocol */
     virtual double render_launch() { return -1.0; }
-    virtual double simtrace_launch() { return -1.0; }
+    virtual double simtrace_launch() { return run_simtrace(); }
     virtual double simulate_launch() { return run(event_id_); }
     virtual double launch() { return run(event_id_); }
     virtual const char* desc() const { return desc_.c_str(); }
@@ -105,13 +109,19 @@ public:
         if (reset_) reset(eventID);
         return ms;
     }
-    virtual double simtrace(int) { return -1.0; }
+    virtual double simtrace(int eventID)
+    {
+        event_id_ = eventID;
+        return run_simtrace();
+    }
     virtual double render(const char* = nullptr) { return -1.0; }
     virtual void reset(int eventID)
     {
         mi_sim_reset(sim_, eventID);
         gensteps_.clear();
         out_ = mi_event_out();
+        simtrace_ = nullptr;
+        n_simtrace_ = 0;
     }
 
     /* event outputs (valid until reset) */
@@ -120,6 +130,8 @@ public:
     const float* photon() const { return out_.photon; }      /* (n,4,4) sphoton */
     const uint64_t* seq() const { return out_.seq; }         /* (n,2,2) seqhis, seqbnd */
     const float* hit() const { return out_.hit; }            /* (m,4,4) */
+    int64_t num_simtrace() const { return n_simtrace_; }
+    const float* simtrace_records() const { return simtrace_; }   /* (n,4,4) sevent simtrace layout */
     mi_sim* handle() { return sim_; }
 
 private:
@@ -137,7 +149,21 @@ private:
         return out_.device_ms;
     }
 
+    double run_simtrace()
+    {
+        simtrace_ = nullptr;
+        n_simtrace_ = 0;
+        double ms = 0.0;
+        int rc = mi_sim_simtrace(sim_, gensteps_.empty() ? nullptr : gensteps_.data(), num_genstep(), &simtrace_,
+                                 &n_simtrace_, &ms);
+        if (rc == MI_NOTHING_TO_DO) return -1.0;                /* the reference's dt = -1 without gensteps */
+        if (rc != MI_OK) throw std::runtime_error(std::string("mi_sim_simtrace: ") + mi_last_error());
+        return ms;
+    }
+
     mi_sim* sim_ = nullptr;
+    const float* simtrace_ = nullptr;
+    int64_t n_simtrace_ = 0;
     std::vector<float> gensteps_;
     mi_event_out out_ = mi_event_out();
     int event_id_ = 0;

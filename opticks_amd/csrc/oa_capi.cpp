@@ -147,7 +147,7 @@ struct mi_sim {
     /* outputs of the host-array entry point */
     DBuf<float4> photon, hit;
     DBuf<uint64_t> seq;
-    std::vector<float> h_photon, h_hit;
+    std::vector<float> h_photon, h_hit, h_simtrace;
     std::vector<uint64_t> h_seq;
     /* adjoint */
     DBuf<int32_t> adj_mat_block, adj_sample_bin;
@@ -796,6 +796,7 @@ int mi_sim_reset(mi_sim* sim, int32_t event_id)
     if (!sim) return fail(MI_ERR_ARG, "null sim");
     std::vector<float>().swap(sim->h_photon);
     std::vector<float>().swap(sim->h_hit);
+    std::vector<float>().swap(sim->h_simtrace);
     std::vector<uint64_t>().swap(sim->h_seq);
     return MI_OK;
 }
@@ -821,6 +822,40 @@ int mi_sim_simtrace_device(mi_sim* sim, const void* d_gensteps, int64_t n_gs, in
     P.n_photon = total;
     HIPCHECK(oa_launch::simtrace(P, (float4*)d_simtrace, s));
     HIPCHECK(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+int mi_sim_simtrace(mi_sim* sim, const float* gensteps, int64_t n_gs, const float** simtrace, int64_t* n_simtrace,
+                    double* device_ms)
+{
+    if (!sim || !simtrace || !n_simtrace) return fail(MI_ERR_ARG, "null");
+    *simtrace = nullptr;
+    *n_simtrace = 0;
+    if (device_ms) *device_ms = 0.0;
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    if (!gensteps) return fail(MI_ERR_ARG, "null gensteps");
+    int64_t n = 0;
+    for (int64_t g = 0; g < n_gs; ++g) {
+        uint32_t np;
+        std::memcpy(&np, gensteps + 24 * g + 3, 4);
+        n += np;
+    }
+    if (n == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    hipStream_t s = sim->stream;
+    HIPCHECK(upload(sim->gs, gensteps, (size_t)n_gs * 24, s));
+    HIPCHECK(sim->photon.reserve((size_t)n * 4));
+    HIPCHECK(hipEventRecord(sim->ev[0], s));
+    int rc = mi_sim_simtrace_device(sim, sim->gs.p, n_gs, n, sim->photon.p, s);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(sim->ev[3], s));
+    HIPCHECK(hipEventSynchronize(sim->ev[3]));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, sim->ev[0], sim->ev[3]));
+    sim->h_simtrace.resize((size_t)n * 16);
+    HIPCHECK(hipMemcpy(sim->h_simtrace.data(), sim->photon.p, sizeof(float) * 16 * n, hipMemcpyDeviceToHost));
+    *simtrace = sim->h_simtrace.data();
+    *n_simtrace = n;
+    if (device_ms) *device_ms = ms;
     return MI_OK;
 }
 

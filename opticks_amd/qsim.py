@@ -12,6 +12,8 @@ Host-side mirror of the reference's simulate surface, over the C-ABI.
   QSim.simulate_device / simulate_batch
                  device-resident forms for torch callers (torch is only the
                  device-memory and stream plumbing here)
+  QSim.simtrace / simtrace_device
+                 simtrace of OpticksGenstep_FRAME grids (qudarap/QSim.cc:876-910)
 
 Errors follow the C-ABI: RuntimeError with mi_last_error() text; zero gensteps
 returns None like the reference's rc=1 "nothing to do" (qudarap/QEvt.cc:436).
@@ -334,6 +336,20 @@ class QSim:
     def trace_device(self, rays_t, isect_t):
         L.check(L.lib().mi_sim_trace_device(self.handle, rays_t.data_ptr(), rays_t.shape[0], isect_t.data_ptr(),
                                             None), "mi_sim_trace_device")
+
+    def simtrace(self, gs: np.ndarray):
+        """QSim::simtrace (qudarap/QSim.cc:876-910) over host gensteps: (n,4,4) f32 simtrace
+        records, or None without gensteps."""
+        gs = np.ascontiguousarray(gs, np.float32).reshape(-1, 6, 4)
+        ptr = C.POINTER(C.c_float)()
+        n = C.c_int64(0)
+        ms = C.c_double(0.0)
+        rc = L.lib().mi_sim_simtrace(self.handle, gs.ctypes.data, len(gs), C.byref(ptr), C.byref(n), C.byref(ms))
+        if rc == L.MI_NOTHING_TO_DO:
+            return None
+        L.check(rc, "mi_sim_simtrace")
+        self.simtrace_ms = ms.value
+        return np.ctypeslib.as_array(ptr, shape=(n.value * 16,)).reshape(n.value, 4, 4).copy()
 
     def simtrace_device(self, gs_t, out_t=None, n_photon=-1):
         """Simtrace of OpticksGenstep_FRAME gensteps (CSGOptiX7.cu:563-604): (n_photon,4,4)

@@ -301,6 +301,9 @@ def test_simtrace_matches_checker(which):
         assert same.mean() >= 0.999, same.mean()
         miss = ref.view(np.uint32)[:, 2, 3] == 0xFFFFFFFF
         assert np.array_equal(out.view(np.uint32)[:, 2, 3] == 0xFFFFFFFF, miss)
+        host = sim.simtrace(gs)                                    # QSim::simtrace host form
+        assert np.array_equal(host.view(np.uint32), out.view(np.uint32))
     with pytest.raises(RuntimeError):
         sim.simtrace_device(torch.from_numpy(gs).to(DEV), n_photon=len(ref) + 1)
+    assert sim.simtrace(gs[:0]) is None
     sim.close()
