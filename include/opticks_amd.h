@@ -37,7 +37,13 @@
  *                              sysrap/SEvt.cc:3754-3756)
  *   hit      (M,4,4)  f32/u32  photons with flagmask & hit_mask, photon order
  *                              (qudarap/QEvt.cc:1113-1132, sysrap/SU.cu:145-160)
- *   node     (n,4,4)  CSGNode  (CSG/CSGNode.h:196)
+ *   node     (n,4,4)  CSGNode  (CSG/CSGNode.h:196); prims are a leaf, a complete
+ *                     Boolean tree, or a list node (CSG_CONTIGUOUS <= 8 leaves,
+ *                     CSG_DISCONTIGUOUS, CSG_OVERLAP) whose leaves sit at
+ *                     root + subOffset; list nodes may also sit in a tree.
+ *                     Leaves: sphere, zsphere, cylinder, box3, cone,
+ *                     convexpolyhedron, hyperboloid, phicut, halfspace; other
+ *                     typecodes fail create with MI_ERR_UNSUPPORTED
  *   prim     (n,4,4)  CSGPrim  (CSG/CSGPrim.h:75)
  *   solid    (n) 48 B CSGSolid (CSG/CSGSolid.h:37)
  *   itra     (n,4,4)  qat4 inverse node transforms (sysrap/sqat4.h)

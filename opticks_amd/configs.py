@@ -310,10 +310,33 @@ def leaf_zoo_nodes():
     ]
 
 
-def leaf_zoo(num_photon=200_000, seed=42, spacing=400.0):
-    """The leaf_zoo_nodes solids as acrylic pieces on a 3x3 grid in a water box
-    with an absorbing, partly detecting wall; photons from a torch point source
-    at the centre of each cell's corner region."""
+def list_zoo_nodes():
+    """One prim per entry: list nodes at the prim root and inside Boolean trees
+    (the subOffset of each list counts from the prim's root node)."""
+    from .foundry import Node, CSG_CONTIGUOUS, CSG_DISCONTIGUOUS, CSG_OVERLAP, CSG_DIFFERENCE, CSG_INTERSECTION, CSG_UNION
+    beads = [Node.sphere(60.0, (x, 0.0, 0.0)) for x in (-80.0, 0.0, 80.0)]
+    bx = Node.box3(50.0, 50.0, 50.0)
+    bx.transform = np.eye(4)
+    bx.transform[3, :3] = (0.0, 90.0, 30.0)
+    islands = [Node.sphere(35.0, (-100.0, -60.0, 0.0)), Node.sphere(35.0, (100.0, -60.0, 0.0)), bx]
+    lens = [Node.sphere(120.0, (0.0, 0.0, -70.0)), Node.sphere(120.0, (0.0, 0.0, 70.0)), Node.cylinder(90.0, -60.0, 60.0)]
+    cross = [Node.box3(200.0, 60.0, 60.0), Node.box3(60.0, 200.0, 60.0), Node.box3(60.0, 60.0, 200.0)]
+    return [
+        Node.list(CSG_CONTIGUOUS, beads),
+        Node.list(CSG_DISCONTIGUOUS, islands),
+        Node.list(CSG_OVERLAP, lens),
+        Node.op(CSG_DIFFERENCE, Node.list(CSG_CONTIGUOUS, [Node.sphere(60.0, (x, 0.0, 0.0)) for x in (-80.0, 0.0, 80.0)]),
+                Node.cylinder(25.0, -200.0, 200.0)),
+        Node.op(CSG_INTERSECTION, Node.list(CSG_CONTIGUOUS, cross), Node.sphere(90.0)),
+        Node.op(CSG_UNION, Node.list(CSG_DISCONTIGUOUS, [Node.sphere(30.0, (-90.0, 0.0, 0.0)), Node.sphere(30.0, (90.0, 0.0, 0.0))]),
+                Node.list(CSG_OVERLAP, [Node.sphere(70.0, (0.0, 0.0, -30.0)), Node.sphere(70.0, (0.0, 0.0, 30.0))])),
+    ]
+
+
+def leaf_zoo(num_photon=200_000, seed=42, spacing=400.0, nodes=None, name="leaf_zoo"):
+    """The leaf_zoo_nodes solids (or `nodes`) as acrylic pieces on a 3x3 grid in a
+    water box with an absorbing, partly detecting wall; photons from a torch point
+    source at the centre of each cell's corner region."""
     lib = BoundaryLib()
     lib.add_material(Material("Water", 1.33, 20000.0, 30000.0))
     lib.add_material(Material("Acrylic", 1.50, 1e6, 1e6))
@@ -325,7 +348,7 @@ def leaf_zoo(num_photon=200_000, seed=42, spacing=400.0):
     p0 = fd.addPrim(Node.box3(2 * half, 2 * half, 2 * half), b_world)
     s0 = fd.addSolid("world", [p0])
     fd.addInstance(np.eye(4), s0)
-    nodes = leaf_zoo_nodes()
+    nodes = leaf_zoo_nodes() if nodes is None else nodes
     for k, nd in enumerate(nodes):
         pk = fd.addPrim(nd, b_obj)
         sk = fd.addSolid(f"leaf{k}", [pk])
@@ -338,7 +361,12 @@ def leaf_zoo(num_photon=200_000, seed=42, spacing=400.0):
     gs = np.stack([torch_genstep(num_photon // 4, T_POINT_ISO, pos=p, wavelength=300.0, weight=300.0)
                    for p in ((200.0, 200.0, 0.0), (-200.0, 200.0, 50.0), (200.0, -200.0, -50.0), (-200.0, -200.0, 0.0))]
                   ).astype(np.float32)
-    return Workload("leaf_zoo", fd.arrays(), phys, gs, seed=seed, meta=dict(n_leaf_prims=len(nodes)))
+    return Workload(name, fd.arrays(), phys, gs, seed=seed, meta=dict(n_leaf_prims=len(nodes)))
+
+
+def list_zoo(num_photon=200_000, seed=42, spacing=400.0):
+    """leaf_zoo world with the list_zoo_nodes solids"""
+    return leaf_zoo(num_photon, seed, spacing, nodes=list_zoo_nodes(), name="list_zoo")
 
 
 def uniform_ball(rng, n, r):

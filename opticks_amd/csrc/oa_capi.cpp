@@ -232,18 +232,29 @@ static int validate_geometry(const mi_geometry* g, const mi_physics* ph)
             if (num != 1) return fail(MI_ERR_GEOMETRY, "leaf prim with numNode != 1");
         } else if (tc >= 1 && tc <= 3) {
             int h = tree_height(root[0]);
-            if (h < 0 || (int)root[0] != num) return fail(MI_ERR_GEOMETRY, "tree subNum is not a complete tree size");
+            if (h < 0 || (int)root[0] > num) return fail(MI_ERR_GEOMETRY, "tree subNum is not a complete tree size");
             if (h > OA_CSG_MAX_HEIGHT) return fail(MI_ERR_UNSUPPORTED, "CSG tree deeper than the device stacks");
-        } else {
-            return fail(MI_ERR_UNSUPPORTED, "list-node prims (CSG_CONTIGUOUS/DISCONTIGUOUS/OVERLAP) not supported");
+        } else if (!(tc >= 11 && tc <= 13)) {
+            return fail(MI_ERR_UNSUPPORTED, "unsupported CSG prim root typecode " + std::to_string(tc));
         }
+        const int tree_nodes = (tc >= 1 && tc <= 3) ? (int)root[0] : 1;
         for (int k = 0; k < num; ++k) {
             const uint32_t* nd = (const uint32_t*)(g->node + 16 * (off + k));
             uint32_t t = nd[14];
             /* sphere, zsphere, cylinder, cone, box3, convexpolyhedron, hyperboloid, phicut, halfspace */
-            bool ok = t == 0 || (t >= 1 && t <= 3) || t == 101 || t == 103 || t == 105 || t == 108 || t == 110 ||
-                      t == 112 || t == 117 || t == 121 || t == 125;
+            const bool leaf = t == 101 || t == 103 || t == 105 || t == 108 || t == 110 || t == 112 || t == 117 ||
+                              t == 121 || t == 125;
+            const bool list = t >= 11 && t <= 13;
+            bool ok = k < tree_nodes ? (t == 0 || (t >= 1 && t <= 3) || leaf || list) : leaf;
             if (!ok) return fail(MI_ERR_UNSUPPORTED, "unsupported CSG node typecode " + std::to_string(t));
+            if (list) {
+                /* sub leaves at root + subOffset (CSG/CSGNode.h:187-211), after the tree nodes */
+                const uint64_t sn = nd[0], so = nd[1];
+                if (sn < 1 || so < (uint64_t)tree_nodes || so + sn > (uint64_t)num)
+                    return fail(MI_ERR_GEOMETRY, "list node sub range outside its prim");
+                if (t == 11 && sn > 8)
+                    return fail(MI_ERR_UNSUPPORTED, "CSG_CONTIGUOUS with more than 8 leaves (the reference's enter arrays)");
+            }
             if (t == 112) {
                 const uint64_t pidx = nd[0], pnum = nd[1];
                 if (!g->plan || pidx + pnum > (uint64_t)g->n_plan)
@@ -376,7 +387,7 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     sim->ext_leaves = 0;
     for (int64_t k = 0; k < g->n_node; ++k) {
         const uint32_t t = ((const uint32_t*)(g->node + 16 * k))[14];
-        if (t == 108 || t == 112 || t == 117 || t == 121 || t == 125) sim->ext_leaves = 1;
+        if (t == 108 || t == 112 || t == 117 || t == 121 || t == 125 || (t >= 11 && t <= 13)) sim->ext_leaves = 1;
     }
     HIPCHECK(upload(sim->node, (const float4*)g->node, (size_t)g->n_node * 4, s));
     if (g->n_tran > 0) HIPCHECK(upload(sim->itra, (const float4*)g->itra, (size_t)g->n_tran * 4, s));

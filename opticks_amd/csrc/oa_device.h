@@ -82,6 +82,7 @@ enum : uint32_t {
 enum : int { CMD_BREAK = 1, CMD_CONTINUE = 2, CMD_BOUNDARY = 3 };
 enum : int { EMS_NOSURFACE = 1, EMS_SURFACE = 2, EMS_ZPLUS_SENSOR_A = 3 };
 enum : uint32_t { TC_ZERO = 0, TC_UNION = 1, TC_INTERSECTION = 2, TC_DIFFERENCE = 3, TC_NODE = 11,
+                  TC_CONTIGUOUS = 11, TC_DISCONTIGUOUS = 12, TC_OVERLAP = 13,
                   TC_LEAF = 101, TC_SPHERE = 101, TC_ZSPHERE = 103, TC_CYLINDER = 105, TC_CONE = 108,
                   TC_BOX3 = 110, TC_CONVEXPOLYHEDRON = 112, TC_HYPERBOLOID = 117, TC_PHICUT = 121,
                   TC_HALFSPACE = 125 };
@@ -556,6 +557,149 @@ OA_D int classify(const Isect& e, v3 dir, float tmin)
 OA_D uint32_t tdepth(uint32_t i) { return 31u - (uint32_t)__builtin_clz(i); }
 OA_D uint32_t postorder_next(uint32_t i, uint32_t elev) { return (i & 1u) ? i >> 1 : (i << elev) + (1u << elev); }
 
+/* ------------------------------------------------------ CSG list nodes */
+
+/* List nodes keep subNum (q0.u.x) leaves at root + subOffset (q0.u.y) + i,
+   counted from the prim's root node (CSG/CSGNode.h:187-211). They occur only in
+   the EXT instantiation and run out of line: the contiguous case keeps small
+   per-lane arrays of enter distances. An invalid result leaves the intersect
+   zero, as the reference's list functions leave it untouched. */
+#define OA_LIST_EPSILON 0.0001f
+#define OA_LIST_MAX_ENTER 8
+
+struct ListHit { float x, y, z, w; int valid; };
+
+OA_D ListHit list_hit(const Isect& e, bool valid)
+{
+    ListHit h;
+    h.x = valid ? e.x : 0.f; h.y = valid ? e.y : 0.f; h.z = valid ? e.z : 0.f; h.w = valid ? e.w : 0.f;
+    h.valid = valid ? 1 : 0;
+    return h;
+}
+
+/* intersect_node_contiguous (CSG/csg_intersect_node.h:392-650) */
+__device__ static __attribute__((noinline)) ListHit list_contiguous(const float4* nd, const float4* root,
+                                                                    const float4* itra, const float4* plan,
+                                                                    float t_min, v3 ro, v3 rd)
+{
+    const int num_sub = (int)__float_as_uint(nd[0].x);
+    const int off = (int)__float_as_uint(nd[0].y);
+    Isect nearest_enter; nearest_enter.x = 0.f; nearest_enter.y = 0.f; nearest_enter.z = 0.f; nearest_enter.w = OA_RT_DEFAULT_MAX;
+    Isect farthest_exit; farthest_exit.x = 0.f; farthest_exit.y = 0.f; farthest_exit.z = 0.f; farthest_exit.w = t_min;
+    Isect sub;
+    int exit_count = 0;
+    for (int i = 0; i < num_sub; ++i) {
+        if (!intersect_leaf<true>(sub, root + 4 * (off + i), itra, plan, t_min, ro, rd)) continue;
+        const int st = classify(sub, rd, t_min);
+        if (st == ST_ENTER) { if (sub.w < nearest_enter.w) nearest_enter = sub; }
+        else if (st == ST_EXIT) exit_count += 1;
+    }
+    if (exit_count == 0)
+        return list_hit(nearest_enter, nearest_enter.w > t_min && nearest_enter.w < OA_RT_DEFAULT_MAX);
+    int enter_count = 0;
+    float enter[OA_LIST_MAX_ENTER];
+    int aux[OA_LIST_MAX_ENTER], idx[OA_LIST_MAX_ENTER];
+    for (int isub = 0; isub < num_sub; ++isub) {
+        if (!intersect_leaf<true>(sub, root + 4 * (off + isub), itra, plan, t_min, ro, rd)) continue;
+        const int st = classify(sub, rd, t_min);
+        if (st == ST_ENTER) {
+            if (enter_count < OA_LIST_MAX_ENTER) {        /* num_sub <= 8 is checked at create */
+                aux[enter_count] = isub; idx[enter_count] = enter_count; enter[enter_count] = sub.w;
+                enter_count += 1;
+            }
+        } else if (st == ST_EXIT) {
+            exit_count += 1;
+            if (sub.w > farthest_exit.w) farthest_exit = sub;
+        }
+    }
+    for (int i = 1; i < enter_count; ++i) {             /* indirect insertion sort by enter distance */
+        const int key = idx[i];
+        int j = i - 1;
+        while (j >= 0 && enter[idx[j]] > enter[key]) { idx[j + 1] = idx[j]; j -= 1; }
+        idx[j + 1] = key;
+    }
+    for (int i = 0; i < enter_count; ++i) {
+        const float tminAdvanced = enter[idx[i]] + OA_LIST_EPSILON;
+        const int isub = aux[idx[i]];
+        if (tminAdvanced < farthest_exit.w) {
+            if (intersect_leaf<true>(sub, root + 4 * (off + isub), itra, plan, tminAdvanced, ro, rd) &&
+                classify(sub, rd, tminAdvanced) == ST_EXIT) {
+                exit_count += 1;
+                if (sub.w > farthest_exit.w) farthest_exit = sub;
+            }
+        }
+    }
+    return list_hit(farthest_exit, exit_count > 0 && farthest_exit.w > t_min);
+}
+
+/* intersect_node_discontiguous (CSG/csg_intersect_node.h:654-690) */
+__device__ static __attribute__((noinline)) ListHit list_discontiguous(const float4* nd, const float4* root,
+                                                                       const float4* itra, const float4* plan,
+                                                                       float t_min, v3 ro, v3 rd)
+{
+    const int num_sub = (int)__float_as_uint(nd[0].x);
+    const int off = (int)__float_as_uint(nd[0].y);
+    Isect closest; closest.x = 0.f; closest.y = 0.f; closest.z = 0.f; closest.w = OA_RT_DEFAULT_MAX;
+    Isect sub;
+    for (int i = 0; i < num_sub; ++i)
+        if (intersect_leaf<true>(sub, root + 4 * (off + i), itra, plan, t_min, ro, rd) && sub.w < closest.w)
+            closest = sub;
+    return list_hit(closest, closest.w < OA_RT_DEFAULT_MAX);
+}
+
+/* intersect_node_overlap (CSG/csg_intersect_node.h:819-905) */
+__device__ static __attribute__((noinline)) ListHit list_overlap(const float4* nd, const float4* root,
+                                                                 const float4* itra, const float4* plan,
+                                                                 float t_min, v3 ro, v3 rd)
+{
+    const uint32_t num_sub = __float_as_uint(nd[0].x);
+    const uint32_t off = __float_as_uint(nd[0].y);
+    Isect farthest_enter; farthest_enter.x = 0.f; farthest_enter.y = 0.f; farthest_enter.z = 0.f; farthest_enter.w = t_min;
+    Isect nearest_exit; nearest_exit.x = 0.f; nearest_exit.y = 0.f; nearest_exit.z = 0.f; nearest_exit.w = OA_RT_DEFAULT_MAX;
+    Isect sub;
+    uint32_t enter_count = 0u, exit_count = 0u;
+    for (uint32_t isub = 0; isub < num_sub; ++isub) {
+        const float4* sn = root + 4u * (off + isub);
+        if (!intersect_leaf<true>(sub, sn, itra, plan, t_min, ro, rd)) continue;
+        const int st = classify(sub, rd, t_min);
+        if (st == ST_ENTER) {
+            enter_count += 1u;
+            if (sub.w > farthest_enter.w) farthest_enter = sub;
+            const float tminAdvanced = sub.w + OA_LIST_EPSILON;
+            if (intersect_leaf<true>(sub, sn, itra, plan, tminAdvanced, ro, rd) &&
+                classify(sub, rd, tminAdvanced) == ST_EXIT) {
+                exit_count += 1u;
+                if (sub.w < nearest_exit.w) nearest_exit = sub;
+            }
+        } else if (st == ST_EXIT) {
+            exit_count += 1u;
+            if (sub.w < nearest_exit.w) nearest_exit = sub;
+        }
+    }
+    const bool overlap_all = farthest_enter.w < nearest_exit.w && max(enter_count, exit_count) == num_sub;
+    if (overlap_all && farthest_enter.w > t_min && farthest_enter.w < OA_RT_DEFAULT_MAX) return list_hit(farthest_enter, true);
+    if (overlap_all && nearest_exit.w > t_min && nearest_exit.w < OA_RT_DEFAULT_MAX) return list_hit(nearest_exit, true);
+    return list_hit(sub, false);
+}
+
+/* intersect_node (CSG/csg_intersect_node.h:913-937): a list node (EXT only) or a leaf */
+template <bool EXT>
+OA_D bool intersect_node(Isect& is, const float4* nd, const float4* root, const float4* itra, const float4* plan,
+                         float t_min, v3 ro, v3 rd)
+{
+    if constexpr (EXT) {
+        const uint32_t typecode = __float_as_uint(nd[3].z);
+        if (typecode >= TC_CONTIGUOUS && typecode <= TC_OVERLAP) {
+            const ListHit h = typecode == TC_CONTIGUOUS ? list_contiguous(nd, root, itra, plan, t_min, ro, rd)
+                            : typecode == TC_DISCONTIGUOUS ? list_discontiguous(nd, root, itra, plan, t_min, ro, rd)
+                            : list_overlap(nd, root, itra, plan, t_min, ro, rd);
+            is.x = h.x; is.y = h.y; is.z = h.z; is.w = h.w;
+            return h.valid != 0;
+        }
+    }
+    return intersect_leaf<EXT>(is, nd, itra, plan, t_min, ro, rd);
+}
+
 /*
  * Device stacks. The evaluation of a complete tree of height h never holds
  * more than h+1 intersects (one pending left result per ancestor, plus the
@@ -607,7 +751,7 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, c
             if (typecode >= TC_NODE) {
                 Isect e;
                 OA_COUNT(OA_ST_LEAF);
-                intersect_leaf<EXT>(e, nd, itra, plan, tmin, ro, rd);
+                intersect_node<EXT>(e, nd, root, itra, plan, tmin, ro, rd);
                 e.w = copysignf(e.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
                 if (sn >= D) { ierr = true; break; }
 #pragma unroll
@@ -698,6 +842,7 @@ OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, cons
         is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
         return is.w > 0.f;
     }
+    if constexpr (EXT) return intersect_node<true>(is, root, root, itra, plan, t_min, ro, rd);
     return false;
 }
 

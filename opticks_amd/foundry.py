@@ -19,6 +19,8 @@ The arrays are exactly the ones the reference uploads (CSG/CSGFoundry.h:458-470)
 
 Boolean trees are complete binary trees in level order padded with CSG_ZERO
 (CSG/csg_intersect_tree.h:199-274); gtransformIdx is 1-based into tran/itra.
+List nodes (CSG_CONTIGUOUS / DISCONTIGUOUS / OVERLAP) hold subNum leaves stored
+after the tree at root + subOffset (CSG/CSGMaker.cc:805-870).
 Method names follow CSGFoundry (addSolid, addPrim, addNode, addTran, addInstance).
 """
 from __future__ import annotations
@@ -30,6 +32,9 @@ CSG_ZERO = 0
 CSG_UNION = 1
 CSG_INTERSECTION = 2
 CSG_DIFFERENCE = 3
+CSG_CONTIGUOUS = 11
+CSG_DISCONTIGUOUS = 12
+CSG_OVERLAP = 13
 CSG_SPHERE = 101
 CSG_ZSPHERE = 103
 CSG_CYLINDER = 105
@@ -52,7 +57,7 @@ class Node:
     """One CSG node: a leaf with parameters or a Boolean operator."""
 
     def __init__(self, typecode, q0=(0, 0, 0, 0), q1=(0, 0), aabb=None, transform=None,
-                 complement=False, left=None, right=None, planes=None):
+                 complement=False, left=None, right=None, planes=None, subs=None):
         self.typecode = typecode
         self.planes = None if planes is None else np.asarray(planes, np.float32).reshape(-1, 4)
         self.q0 = tuple(float(v) for v in q0)
@@ -62,6 +67,7 @@ class Node:
         self.complement = complement
         self.left = left
         self.right = right
+        self.subs = subs                # leaves of a list node
 
     # leaf constructors (parameters as CSGNode.h:84-100)
     @staticmethod
@@ -123,11 +129,22 @@ class Node:
     def op(typecode, left, right):
         return Node(typecode, left=left, right=right)
 
+    @staticmethod
+    def list(typecode, subs):
+        """list node over leaves: CSG_CONTIGUOUS (touching union), CSG_DISCONTIGUOUS (disjoint
+        union) or CSG_OVERLAP (intersection)"""
+        assert typecode in (CSG_CONTIGUOUS, CSG_DISCONTIGUOUS, CSG_OVERLAP)
+        assert subs and all(n.is_leaf() for n in subs)
+        return Node(typecode, subs=list(subs))
+
     def is_leaf(self):
         return self.typecode >= 101
 
+    def is_list(self):
+        return CSG_CONTIGUOUS <= self.typecode <= CSG_OVERLAP
+
     def height(self):
-        if self.is_leaf():
+        if self.is_leaf() or self.is_list():
             return 0
         return 1 + max(self.left.height(), self.right.height())
 
@@ -141,6 +158,12 @@ class Node:
                                 for z in (lo[2], hi[2])])
             w = corners @ self.transform
             return w[:, :3].min(0), w[:, :3].max(0)
+        if self.is_list():
+            boxes = [n.world_aabb() for n in self.subs]
+            los, his = np.array([b[0] for b in boxes]), np.array([b[1] for b in boxes])
+            if self.typecode == CSG_OVERLAP:
+                return los.max(0), his.min(0)
+            return los.min(0), his.max(0)
         llo, lhi = self.left.world_aabb()
         rlo, rhi = self.right.world_aabb()
         if self.typecode == CSG_DIFFERENCE:
@@ -187,7 +210,7 @@ class Foundry:
             self.plans.extend(list(nd.planes))
         u[6] = boundary
         u[7] = index
-        if nd.is_leaf():
+        if nd.is_leaf() or nd.is_list():
             lo, hi = nd.world_aabb()
             rec[8:11] = lo
             rec[11] = hi[0]
@@ -207,7 +230,7 @@ class Foundry:
 
         def place(nd, i):
             slots[i - 1] = nd
-            if not nd.is_leaf():
+            if not (nd.is_leaf() or nd.is_list()):
                 place(nd.left, 2 * i)
                 place(nd.right, 2 * i + 1)
 
@@ -215,9 +238,18 @@ class Foundry:
         node_offset = len(self.nodes)
         for k, nd in enumerate(slots):
             self.nodes.append(self._node_record(nd, boundary, k))
-        if not root.is_leaf():
+        if not (root.is_leaf() or root.is_list()):
             self.nodes[node_offset].view(np.uint32)[0] = num      # subNum
             self.nodes[node_offset].view(np.uint32)[1] = 0        # subOffset
+        sub_offset = num                                          # list leaves follow the tree
+        for k, nd in enumerate(slots):
+            if nd is not None and nd.is_list():
+                hdr = self.nodes[node_offset + k].view(np.uint32)
+                hdr[0], hdr[1] = len(nd.subs), sub_offset
+                for j, leaf in enumerate(nd.subs):
+                    self.nodes.append(self._node_record(leaf, boundary, sub_offset + j))
+                sub_offset += len(nd.subs)
+        num = sub_offset
         lo, hi = root.world_aabb()
         ext = hi - lo
         lo = lo - pad * ext - 1e-2

@@ -57,6 +57,7 @@ enum { EMS_MATERIAL = 0, EMS_NOSURFACE = 1, EMS_SURFACE = 2, EMS_ZPLUS_SENSOR_A 
        EMS_ZPLUS_SENSOR_CUSTOMART = 4, EMS_ZMINUS = 5 };
 /* sysrap/OpticksCSG.h */
 enum { CSG_ZERO = 0, CSG_UNION = 1, CSG_INTERSECTION = 2, CSG_DIFFERENCE = 3, CSG_NODE = 11,
+       CSG_CONTIGUOUS = 11, CSG_DISCONTIGUOUS = 12, CSG_OVERLAP = 13,
        CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103, CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110,
        CSG_CONVEXPOLYHEDRON = 112, CSG_HYPERBOLOID = 117, CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
 /* sysrap/OpticksGenstep.h:20-45 */
@@ -160,6 +161,7 @@ static inline uint32_t nd_gtransformIdx(const float* nd) { return nodeu(nd)[15] 
 static inline int nd_complement(const float* nd) { return (nodeu(nd)[15] & 0x80000000u) != 0u; }
 static inline uint32_t nd_boundary(const float* nd) { return nodeu(nd)[6]; }
 static inline uint32_t nd_subNum(const float* nd) { return nodeu(nd)[0]; }
+static inline uint32_t nd_subOffset(const float* nd) { return nodeu(nd)[1]; }
 
 /* qat4::right_multiply (sysrap/sqat4.h:45-51): v^T M with w */
 static inline f3 q_right_multiply(const float* q, f3 v, float w)
@@ -604,6 +606,152 @@ static inline unsigned tree_height(unsigned n) { return (unsigned)(__builtin_ffs
 static inline unsigned postorder_next(unsigned i, unsigned elev) { return (i & 1u) ? i >> 1 : (i << elev) + (1u << elev); }
 #define PACK_SLICE(b, e) ((((b) & 0xffu) << 16) | (((e) & 0xffu) << 24))
 
+/* ------------------------------------------------------ CSG list nodes */
+/* Compound nodes whose subNum leaves sit at root + subOffset + i (offsets from
+   the prim's root node, CSG/CSGNode.h:187-211). */
+#define LIST_PROPAGATE_EPSILON 0.0001f
+#define LIST_MAX_ENTER 8
+
+/* intersect_node_contiguous (CSG/csg_intersect_node.h:392-650): union of
+   touching leaves. Zeroth pass: nearest enter and a count of exits; with no
+   exit the ray starts outside and the nearest enter is the answer. Otherwise
+   collect the enters, order them by t (indirect insertion sort), and march
+   through them extending the farthest exit with each leaf entered before it. */
+static int list_contiguous(int* valid, f4* isect, const float* nd, const float* root, const float* itra,
+                           const float* plan, float t_min, f3 ro, f3 rd)
+{
+    const int num_sub = (int)nd_subNum(nd);
+    const int off = (int)nd_subOffset(nd);
+    f4 nearest_enter = mk4(0.f, 0.f, 0.f, RT_DEFAULT_MAX_F);
+    f4 farthest_exit = mk4(0.f, 0.f, 0.f, t_min);
+    f4 sub = mk4(0.f, 0.f, 0.f, 0.f);
+    int exit_count = 0;
+    for (int i = 0; i < num_sub; ++i) {
+        int v = 0;
+        if (intersect_leaf(&v, &sub, root + 16 * (off + i), itra, plan, t_min, ro, rd) != 0) return -1;
+        if (!v) continue;
+        int st = csg_classify(sub, rd, t_min);
+        if (st == S_ENTER) { if (sub.w < nearest_enter.w) nearest_enter = sub; }
+        else if (st == S_EXIT) exit_count += 1;
+    }
+    if (exit_count == 0) {
+        *valid = nearest_enter.w > t_min && nearest_enter.w < RT_DEFAULT_MAX_F;
+        if (*valid) *isect = nearest_enter;
+        return 0;
+    }
+    int enter_count = 0;
+    float enter[LIST_MAX_ENTER];
+    int aux[LIST_MAX_ENTER], idx[LIST_MAX_ENTER];
+    for (int isub = 0; isub < num_sub; ++isub) {
+        int v = 0;
+        if (intersect_leaf(&v, &sub, root + 16 * (off + isub), itra, plan, t_min, ro, rd) != 0) return -1;
+        if (!v) continue;
+        int st = csg_classify(sub, rd, t_min);
+        if (st == S_ENTER) {
+            if (enter_count >= LIST_MAX_ENTER) return -1;
+            aux[enter_count] = isub; idx[enter_count] = enter_count; enter[enter_count] = sub.w;
+            enter_count += 1;
+        } else if (st == S_EXIT) {
+            exit_count += 1;
+            if (sub.w > farthest_exit.w) farthest_exit = sub;
+        }
+    }
+    for (int i = 1; i < enter_count; ++i) {
+        int key = idx[i];
+        int j = i - 1;
+        while (j >= 0 && enter[idx[j]] > enter[key]) { idx[j + 1] = idx[j]; j -= 1; }
+        idx[j + 1] = key;
+    }
+    for (int i = 0; i < enter_count; ++i) {
+        float tminAdvanced = enter[idx[i]] + LIST_PROPAGATE_EPSILON;
+        int isub = aux[idx[i]];
+        if (tminAdvanced < farthest_exit.w) {
+            int v = 0;
+            if (intersect_leaf(&v, &sub, root + 16 * (off + isub), itra, plan, tminAdvanced, ro, rd) != 0) return -1;
+            if (v && csg_classify(sub, rd, tminAdvanced) == S_EXIT) {
+                exit_count += 1;
+                if (sub.w > farthest_exit.w) farthest_exit = sub;
+            }
+        }
+    }
+    *valid = exit_count > 0 && farthest_exit.w > t_min;
+    if (*valid) *isect = farthest_exit;
+    return 0;
+}
+
+/* intersect_node_discontiguous (CSG/csg_intersect_node.h:654-690): disjoint
+   leaves, the closest valid leaf intersect */
+static int list_discontiguous(int* valid, f4* isect, const float* nd, const float* root, const float* itra,
+                              const float* plan, float t_min, f3 ro, f3 rd)
+{
+    const int num_sub = (int)nd_subNum(nd);
+    const int off = (int)nd_subOffset(nd);
+    f4 closest = mk4(0.f, 0.f, 0.f, RT_DEFAULT_MAX_F);
+    f4 sub = mk4(0.f, 0.f, 0.f, 0.f);
+    for (int i = 0; i < num_sub; ++i) {
+        int v = 0;
+        if (intersect_leaf(&v, &sub, root + 16 * (off + i), itra, plan, t_min, ro, rd) != 0) return -1;
+        if (v && sub.w < closest.w) closest = sub;
+    }
+    *valid = closest.w < RT_DEFAULT_MAX_F;
+    if (*valid) *isect = closest;
+    return 0;
+}
+
+/* intersect_node_overlap (CSG/csg_intersect_node.h:819-905): intersection of
+   leaves; each entered leaf is re-shot past its enter for its exit; a hit needs
+   farthest enter < nearest exit with every leaf entered or exited */
+static int list_overlap(int* valid, f4* isect, const float* nd, const float* root, const float* itra,
+                        const float* plan, float t_min, f3 ro, f3 rd)
+{
+    const unsigned num_sub = nd_subNum(nd);
+    const unsigned off = nd_subOffset(nd);
+    f4 farthest_enter = mk4(0.f, 0.f, 0.f, t_min);
+    f4 nearest_exit = mk4(0.f, 0.f, 0.f, RT_DEFAULT_MAX_F);
+    f4 sub = mk4(0.f, 0.f, 0.f, 0.f);
+    unsigned enter_count = 0, exit_count = 0;
+    for (unsigned isub = 0; isub < num_sub; ++isub) {
+        const float* sn = root + 16 * (off + isub);
+        int v = 0;
+        if (intersect_leaf(&v, &sub, sn, itra, plan, t_min, ro, rd) != 0) return -1;
+        if (!v) continue;
+        int st = csg_classify(sub, rd, t_min);
+        if (st == S_ENTER) {
+            enter_count += 1;
+            if (sub.w > farthest_enter.w) farthest_enter = sub;
+            float tminAdvanced = sub.w + LIST_PROPAGATE_EPSILON;
+            if (intersect_leaf(&v, &sub, sn, itra, plan, tminAdvanced, ro, rd) != 0) return -1;
+            if (v && csg_classify(sub, rd, tminAdvanced) == S_EXIT) {
+                exit_count += 1;
+                if (sub.w < nearest_exit.w) nearest_exit = sub;
+            }
+        } else if (st == S_EXIT) {
+            exit_count += 1;
+            if (sub.w < nearest_exit.w) nearest_exit = sub;
+        }
+    }
+    *valid = 0;
+    unsigned mx = enter_count > exit_count ? enter_count : exit_count;
+    if (farthest_enter.w < nearest_exit.w && mx == num_sub) {
+        if (farthest_enter.w > t_min && farthest_enter.w < RT_DEFAULT_MAX_F) { *valid = 1; *isect = farthest_enter; }
+        else if (nearest_exit.w > t_min && nearest_exit.w < RT_DEFAULT_MAX_F) { *valid = 1; *isect = nearest_exit; }
+    }
+    return 0;
+}
+
+/* intersect_node (CSG/csg_intersect_node.h:913-937): list nodes or a leaf */
+static int intersect_node(int* valid, f4* isect, const float* nd, const float* root, const float* itra,
+                          const float* plan, float t_min, f3 ro, f3 rd)
+{
+    *valid = 0;
+    switch (nd_typecode(nd)) {
+        case CSG_CONTIGUOUS:    return list_contiguous(valid, isect, nd, root, itra, plan, t_min, ro, rd);
+        case CSG_OVERLAP:       return list_overlap(valid, isect, nd, root, itra, plan, t_min, ro, rd);
+        case CSG_DISCONTIGUOUS: return list_discontiguous(valid, isect, nd, root, itra, plan, t_min, ro, rd);
+        default:                return intersect_leaf(valid, isect, nd, itra, plan, t_min, ro, rd);
+    }
+}
+
 /* intersect_tree (CSG/csg_intersect_tree.h:277-672) */
 static int intersect_tree(int* valid, f4* isect, const float* node, const float* itra, const float* plan, float t_min,
                           f3 ro, f3 rd)
@@ -633,7 +781,7 @@ static int intersect_tree(int* valid, f4* isect, const float* node, const float*
             if (node_or_leaf) {
                 f4 nd_isect = mk4(0.f, 0.f, 0.f, 0.f);
                 int nd_valid = 0;
-                if (intersect_leaf(&nd_valid, &nd_isect, nd, itra, plan, tmin, ro, rd) != 0) return -1;
+                if (intersect_node(&nd_valid, &nd_isect, nd, node, itra, plan, tmin, ro, rd) != 0) return -1;
                 nd_isect.w = copysignf(nd_isect.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
                 if (csg_curr >= CSG_STACK_SIZE - 1) { ierr = ERR_OVERFLOW; break; }
                 csg_curr++; csg[csg_curr] = nd_isect;
@@ -700,7 +848,7 @@ static int intersect_tree(int* valid, f4* isect, const float* node, const float*
     return 0;
 }
 
-/* intersect_prim (CSG/csg_intersect_tree.h:684-719); list nodes not restated */
+/* intersect_prim (CSG/csg_intersect_tree.h:684-719) */
 static int intersect_prim(int* valid, f4* isect, const float* node, const float* itra, const float* plan, float t_min,
                           f3 ro, f3 rd)
 {
@@ -708,6 +856,9 @@ static int intersect_prim(int* valid, f4* isect, const float* node, const float*
     *valid = 0;
     if (typecode >= CSG_LEAF) return intersect_leaf(valid, isect, node, itra, plan, t_min, ro, rd);
     if (typecode < CSG_NODE) return intersect_tree(valid, isect, node, itra, plan, t_min, ro, rd);
+    if (typecode == CSG_CONTIGUOUS) return list_contiguous(valid, isect, node, node, itra, plan, t_min, ro, rd);
+    if (typecode == CSG_DISCONTIGUOUS) return list_discontiguous(valid, isect, node, node, itra, plan, t_min, ro, rd);
+    if (typecode == CSG_OVERLAP) return list_overlap(valid, isect, node, node, itra, plan, t_min, ro, rd);
     return -1;
 }
 

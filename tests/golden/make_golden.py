@@ -10,6 +10,10 @@ Writes the golden fixtures of tests/golden from the reference checkout:
   csg_golden_leaves.npz  rays against the leaf-zoo prims (cone, hyperboloid, convex
                   polyhedra, phicut and halfspace inside Boolean trees, a
                   transformed cone), intersected by the same reference code.
+  csg_golden_lists.npz  rays against the list-zoo prims (CSG_CONTIGUOUS,
+                  CSG_DISCONTIGUOUS and CSG_OVERLAP list nodes at the prim root and
+                  inside Boolean trees), intersected by the same reference code built
+                  with WITH_CONTIGUOUS as the reference builds its CSG library.
   csg_lut.json    the packed Boolean LUT words parsed from CSG/csg_classify.h.
 
 Run in the container that holds the reference (python tests/golden/make_golden.py);
@@ -72,14 +76,21 @@ def csg():
 
 
 def leaves():
-    import pyoracle as po
     from opticks_amd import configs
+    return zoo_vectors(configs.leaf_zoo(num_photon=4).geometry, 20261203)
+
+
+def lists():
+    from opticks_amd import configs
+    return zoo_vectors(configs.list_zoo(num_photon=4).geometry, 20261204)
+
+
+def zoo_vectors(g, seed):
+    import pyoracle as po
     po.build()
     assert po.reflib() is not None, "oracle/_ref/libref_csg.so missing"
-    w = configs.leaf_zoo(num_photon=4)
-    g = w.geometry
-    # prim k+1 is leaf_zoo_nodes()[k], in its own solid frame (instance transforms not applied)
-    rng = np.random.default_rng(20261203)
+    # prim k+1 is the zoo's k-th node, in its own solid frame (instance transforms not applied)
+    rng = np.random.default_rng(seed)
     rows = []
     for prim in range(1, len(g["prim"])):
         for k in range(700):
@@ -106,4 +117,5 @@ if __name__ == "__main__":
         json.dump(lut(), f, indent=1)
     np.savez_compressed(os.path.join(HERE, "csg_golden.npz"), **csg())
     np.savez_compressed(os.path.join(HERE, "csg_golden_leaves.npz"), **leaves())
+    np.savez_compressed(os.path.join(HERE, "csg_golden_lists.npz"), **lists())
     print("wrote", os.listdir(HERE))

@@ -203,3 +203,63 @@ def test_leaf_closed_forms(zoo):
     # sphere r 140 minus the halfspace z <= 40 (prim 7): from below along +z the first surface is z = 40
     v, i = po.intersect_prim(zoo, 7, 0.0, (0.0, 0.0, -500.0), (0.0, 0.0, 1.0))
     assert v and abs(i[3] - 540.0) < 1e-3
+
+
+@pytest.fixture(scope="module")
+def lzoo():
+    return configs.list_zoo(num_photon=4).geometry
+
+
+def test_list_golden_vectors_bitwise(lzoo):
+    """CSG_CONTIGUOUS / DISCONTIGUOUS / OVERLAP list nodes at the prim root and inside
+    trees: oracle == reference CSG code (built WITH_CONTIGUOUS) on committed vectors"""
+    z = np.load(os.path.join(GOLD, "csg_golden_lists.npz"))
+    bad = []
+    for i in range(len(z["prim"])):
+        v, isect = po.intersect_prim(lzoo, int(z["prim"][i]), z["tmin"][i], z["origin"][i], z["direction"][i])
+        if int(v) != int(z["valid"][i]) or not np.array_equal(isect.view(np.uint32), z["isect"][i].view(np.uint32)):
+            bad.append(i)
+    assert not bad, (len(bad), bad[:5])
+    for p in range(1, 7):
+        assert z["valid"][z["prim"] == p].sum() > 20
+
+
+@pytest.mark.skipif(po.reflib() is None, reason="oracle/_ref not built (reference checkout absent)")
+def test_list_random_rays_vs_reference_build(lzoo):
+    rng = np.random.default_rng(9)
+    bad = 0
+    for k in range(6000):
+        prim = 1 + k % 6
+        o = rng.uniform(-250, 250, 3).astype(np.float32)
+        d = rng.normal(size=3); d = (d / np.linalg.norm(d)).astype(np.float32)
+        t = np.float32([0.0, 0.05, 2.0][k % 3])
+        v1, i1 = po.intersect_prim(lzoo, prim, t, o, d)
+        v2, i2 = po.ref_intersect_prim(lzoo, prim, t, o, d)
+        bad += (v1 != v2) or not np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
+    assert bad == 0
+
+
+def test_list_closed_forms(lzoo):
+    """bead chain (contiguous spheres r 60 at x = -80, 0, 80): first surface from outside
+    at x = -140, exit from inside the middle bead at the far end x = 140 (not at the bead's
+    own surface x = 60); islands (discontiguous): nearest island only; lens (overlap of two
+    r 120 spheres at z = +-70 and a cylinder r 90, |z| <= 60): z in [-50, 50] on the axis,
+    and a ray through the cylinder side outside the spheres' overlap misses"""
+    v, i = po.intersect_prim(lzoo, 1, 0.0, (-500.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 360.0) < 1e-3 and abs(i[0] + 1.0) < 1e-6
+    v, i = po.intersect_prim(lzoo, 1, 0.0, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 140.0) < 1e-3 and abs(i[0] - 1.0) < 1e-6
+    v, i = po.intersect_prim(lzoo, 2, 0.0, (-500.0, -60.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 365.0) < 1e-3
+    v, i = po.intersect_prim(lzoo, 2, 0.0, (0.0, -60.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 65.0) < 1e-3                   # to the right island, not back
+    v, i = po.intersect_prim(lzoo, 3, 0.0, (0.0, 0.0, -500.0), (0.0, 0.0, 1.0))
+    assert v and abs(i[3] - 450.0) < 1e-3 and abs(i[2] + 1.0) < 1e-6
+    v, i = po.intersect_prim(lzoo, 3, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    assert v and abs(i[3] - 50.0) < 1e-3 and abs(i[2] - 1.0) < 1e-6
+    # the sphere overlap at z = 0 reaches x = sqrt(120^2 - 70^2) = 97.5 > 90: the cylinder side is the surface
+    v, i = po.intersect_prim(lzoo, 3, 0.0, (-500.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 410.0) < 1e-3
+    # bead chain with a bore (difference with cylinder r 25 along z): along x at y = 0 the bore splits it
+    v, i = po.intersect_prim(lzoo, 4, 0.0, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    assert v and abs(i[3] - 25.0) < 1e-3

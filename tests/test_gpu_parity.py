@@ -307,3 +307,34 @@ def test_simtrace_matches_checker(which):
         sim.simtrace_device(torch.from_numpy(gs).to(DEV), n_photon=len(ref) + 1)
     assert sim.simtrace(gs[:0]) is None
     sim.close()
+
+
+def test_list_zoo_trace_and_simulate_match_checker():
+    """CSG_CONTIGUOUS / DISCONTIGUOUS / OVERLAP list nodes at prim roots and inside trees:
+    nearest hits against the checker's brute force, then full propagation"""
+    w = configs.list_zoo(num_photon=200_000)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    rng = np.random.default_rng(13)
+    n = 60000
+    rays = np.zeros((n, 2, 4), np.float32)
+    rays[:, 0, :3] = rng.uniform(-600, 600, (n, 3))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 1, :3] = d
+    rays[:, 0, 3] = rng.choice([0.0, 0.05, 1.0], n)
+    rays[:, 1, 3] = 1e6
+    print("list_zoo trace", n, flush=True)
+    rt = torch.from_numpy(rays).to(DEV)
+    it = torch.zeros((n, 2, 4), dtype=torch.float32, device=DEV)
+    sim.trace_device(rt, it)
+    g = it.cpu().numpy()
+    W = po.World(w.geometry, w.physics)
+    r = W.trace(rays, traverse=0)
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+    inst = g[:, 1, 0].view(np.uint32) >> 16
+    assert len(np.unique(inst[g[:, 1, 1].view(np.uint32) != 0xFFFF])) == 7      # world + six list solids
+    print("list_zoo simulate", flush=True)
+    hits = sim.simulate(w.gensteps, 0)
+    ref_p, ref_s = W.simulate(w.gensteps, seed=w.seed)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s >= 0.999 and len(hits) > 1000
+    sim.close()
