@@ -323,7 +323,7 @@ OA_D int propagate(const SimParams& P, Rng& rng, Photon& p, const Prd& prd, Tape
 
 /* -------------------------------------------------------- generation */
 
-enum : uint32_t { T_DISC = 1, T_LINE = 2, T_POINT = 3, T_CIRCLE = 4, T_SPHERE_MARSAGLIA = 6, T_SPHERE = 7,
+enum : uint32_t { T_DISC = 1, T_LINE = 2, T_POINT = 3, T_CIRCLE = 4, T_RECTANGLE = 5, T_SPHERE_MARSAGLIA = 6, T_SPHERE = 7,
                   T_POINT_ISO = 100 };
 
 OA_D void zero_flags_set(Photon& p, uint32_t flag)
@@ -421,6 +421,27 @@ OA_D void gen_torch(Photon& p, Rng& rng, const float4* gq, uint64_t photon_id)
         float r = radius < 0.f ? -radius : radius;
         p.mom = mk(radius < 0.f ? -cosPhi : cosPhi, 0.f, radius < 0.f ? -sinPhi : sinPhi);
         p.pos = mk(r * cosPhi, 0.f, r * sinPhi);
+        p.pos = p.pos + gpos;
+        p.pol = mk(0.f, -1.f, 0.f);
+        rotateUz(p.pol, p.mom);
+    } else if (type == T_RECTANGLE) {
+        /* storch.h:463-516: photons split over the four sides of the rectangle
+           x in [azimuth.x, azimuth.y], z in [zenith.x, zenith.y] of the y = 0 plane,
+           each pointing inwards; left, right, bottom, top. Fewer than 4 photons
+           (side size 0, a division by zero in the reference) all go to side 0. */
+        p.wavelength = gwl; p.time = gtime;
+        const int side_size = (int)(numphoton / 4u);
+        const uint64_t ss = side_size > 0 ? (uint64_t)side_size : 1ull;
+        const int side = (int)(photon_id / ss);
+        const int side_index = (int)(photon_id - (uint64_t)side * ss);
+        const float frac = (float)side_index / (float)(side_size > 0 ? side_size : 1);
+        if (side == 0 || side == 1) {
+            p.pos = mk(side == 0 ? ax : ay, 0.f, (1.f - frac) * zx + frac * zy);
+            p.mom = mk(side == 0 ? 1.f : -1.f, 0.f, 0.f);
+        } else if (side == 2 || side == 3) {
+            p.pos = mk((1.f - frac) * ax + frac * ay, 0.f, side == 2 ? zx : zy);
+            p.mom = mk(0.f, 0.f, side == 2 ? 1.f : -1.f);
+        }
         p.pos = p.pos + gpos;
         p.pol = mk(0.f, -1.f, 0.f);
         rotateUz(p.pol, p.mom);

@@ -1628,6 +1628,27 @@ static void torch_generate(oph* p, orng* rng, const float* gs, uint64_t photon_i
         p->pos = add3(p->pos, gpos);
         p->pol = mk3(0.f, -1.f, 0.f);
         rotateUz(&p->pol, p->mom);
+    } else if (type == T_RECTANGLE) {
+        /* storch.h:463-516: four sides (left, right, bottom, top) of the rectangle
+           x in [azimuth.x, azimuth.y], z in [zenith.x, zenith.y], momenta inwards;
+           photons past 4*(numphoton/4) keep zero position offset and momentum;
+           with fewer than 4 photons (a zero side size) all go to side 0 */
+        p->wavelength = gwl; p->time = gtime;
+        int side_size = (int)(numphoton / 4u);
+        uint64_t ss = side_size > 0 ? (uint64_t)side_size : 1u;
+        int side = (int)(photon_id / ss);
+        int side_index = (int)(photon_id - (uint64_t)side * ss);
+        float frac = (float)side_index / (float)(side_size > 0 ? side_size : 1);
+        if (side == 0 || side == 1) {
+            p->pos = mk3(side == 0 ? azi_x : azi_y, 0.f, (1.f - frac) * zen_x + frac * zen_y);
+            p->mom = mk3(side == 0 ? 1.f : -1.f, 0.f, 0.f);
+        } else if (side == 2 || side == 3) {
+            p->pos = mk3((1.f - frac) * azi_x + frac * azi_y, 0.f, side == 2 ? zen_x : zen_y);
+            p->mom = mk3(0.f, 0.f, side == 2 ? 1.f : -1.f);
+        }
+        p->pos = add3(p->pos, gpos);
+        p->pol = mk3(0.f, -1.f, 0.f);
+        rotateUz(&p->pol, p->mom);
     }
     ph_zero_flags(p);
     ph_set_flag(p, F_TORCH);

@@ -338,3 +338,26 @@ def test_list_zoo_trace_and_simulate_match_checker():
     s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
     assert s >= 0.999 and len(hits) > 1000
     sim.close()
+
+
+def test_torch_types_match_checker():
+    """every storch type (point, line, disc, circle, rectangle, sphere, Marsaglia sphere)
+    generated and propagated in the config0 world, bitwise with the checker"""
+    w = configs.config0(num_photon=10)
+    T = configs
+    # the rectangle's sides come from the absolute photon index (storch.h:484-487), so it goes first
+    gs = np.stack([
+        T.torch_genstep(20000, T.T_RECTANGLE, zenith=(-300.0, 500.0), azimuth=(-200.0, 400.0)),
+        T.torch_genstep(20000, T.T_POINT, pos=(1.0, 2.0, 3.0), mom=(0.0, 0.6, 0.8)),
+        T.torch_genstep(20000, T.T_LINE, pos=(0.0, 0.0, 5.0), mom=(0.0, 0.0, 1.0), radius=300.0),
+        T.torch_genstep(20000, T.T_DISC, pos=(10.0, 0.0, 0.0), mom=(0.0, 0.6, 0.8), radius=500.0),
+        T.torch_genstep(20000, T.T_CIRCLE, radius=-900.0),
+        T.torch_genstep(20000, T.T_SPHERE, radius=-300.0),
+        T.torch_genstep(20000, T.T_SPHERE_MARSAGLIA, radius=200.0),
+    ]).astype(np.float32)
+    ref_p, ref_s = po.World(w.geometry, w.physics).simulate(gs, seed=w.seed)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    sim.simulate(gs, 0)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s >= 0.999 and b >= 0.999
+    sim.close()
