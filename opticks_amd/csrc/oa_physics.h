@@ -612,7 +612,6 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     uint32_t n1 = 0, n2 = 0;
 #endif
     while (!done) {
-#ifndef OA_SPECULATIVE
         /* phase 1: walk nodes until this lane has a leaf or nothing left */
         while (!done && leaf == OA_BVH_EMPTY) {
             OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
@@ -631,36 +630,6 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
             }
         }
-#else
-        /* phase 1, speculative: a lane that already holds a leaf keeps walking
-           internal nodes until every lane of the wave holds a leaf or is done */
-        for (;;) {
-            const bool walk = !done && (leaf == OA_BVH_EMPTY || (cur != OA_BVH_EMPTY && !(cur & OA_BVH_LEAF_BIT)));
-            if (walk) {
-                OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
-#ifdef OA_STATS
-                n1++;
-#endif
-                if (cur == OA_BVH_EMPTY) {
-                    if (sp == 0) done = true;
-                    else { sp--; cur = stk[sp * STRIDE]; }
-                } else if (cur & OA_BVH_LEAF_BIT) {
-                    leaf = cur;
-                    cur = OA_BVH_EMPTY;
-                } else {
-                    OA_COUNT(OA_ST_TLAS);
-                    cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
-                }
-            }
-#ifndef OA_SPEC_READY_QUARTERS
-            if (__all(done || leaf != OA_BVH_EMPTY)) break;
-#else
-            /* start the leaf phase once this many quarters of the lanes hold a leaf or are done */
-            const unsigned long long ready = __ballot(done || leaf != OA_BVH_EMPTY), act = __ballot(1);
-            if (ready == act || 4 * __popcll(ready) >= OA_SPEC_READY_QUARTERS * __popcll(act)) break;
-#endif
-        }
-#endif
         if (leaf == OA_BVH_EMPTY) continue;
         /* phase 2: the pending instance leaf */
         const uint32_t first = leaf & 0x00ffffffu, count = (leaf >> 24) & 0x7fu;
