@@ -1,14 +1,22 @@
 /*
- * oa_bvh.h : host-side BVH builder (binned SAH, 16 bins) producing the
- * wide-2 BvhNode layout of oa_layout.h. Used for the TLAS over instance world
- * boxes and for one BLAS per solid over its CSGPrim boxes. This replaces the
- * OptiX GAS/IAS builds of CSGOptiX/SBT.cc:277-375,495-553.
+ * oa_bvh.h : host-side BVH builder producing the 4-wide BvhNode layout of
+ * oa_layout.h. Binary binned-SAH splits (16 bins, plus the "largest item
+ * alone" candidate) are collapsed four-wide: a node starts from the two halves
+ * of its range and splits the child of largest surface area until it has four
+ * children. Used for the TLAS over instance world boxes and for one BLAS per
+ * solid over its CSGPrim boxes. This replaces the OptiX GAS/IAS builds of
+ * CSGOptiX/SBT.cc:277-375,495-553.
+ *
+ * stack_need: the deepest traversal stack a nearest-first walk can reach below
+ * the root, sum over a root-to-leaf path of (children - 1), each node pushing
+ * all hit children but the nearest.
  */
 #pragma once
 #include <stdint.h>
 #include <vector>
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include "oa_layout.h"
 
 namespace oa_bvh {
@@ -24,6 +32,7 @@ struct Builder {
     std::vector<uint32_t>& items;
     int leaf_size;
     uint32_t max_depth = 0;
+    uint32_t stack_need = 0;
 
     Builder(std::vector<BvhNode>& n, std::vector<uint32_t>& it, int leaf) : nodes(n), items(it), leaf_size(leaf) {}
 
@@ -126,28 +135,78 @@ struct Builder {
         return mid;
     }
 
-    /* returns a child link for range [b,e) and its box */
-    uint32_t link(std::vector<Item>& v, int b, int e, uint32_t depth, double* lo, double* hi)
+    /* returns a child link for range [b,e), its box and the stack need below it */
+    uint32_t link(std::vector<Item>& v, int b, int e, uint32_t depth, double* lo, double* hi, uint32_t& need)
     {
         box_of(v, b, e, lo, hi);
+        need = 0;
         if (e - b <= leaf_size) return leaf_link(v, b, e);
-        return node(v, b, e, depth);
+        return node(v, b, e, depth, need);
     }
 
-    uint32_t node(std::vector<Item>& v, int b, int e, uint32_t depth)
+    uint32_t node(std::vector<Item>& v, int b, int e, uint32_t depth, uint32_t& need)
     {
         uint32_t idx = (uint32_t)nodes.size();
         nodes.push_back(BvhNode());
         max_depth = std::max(max_depth, depth);
-        int mid = split(v, b, e);
-        double lo0[3], hi0[3], lo1[3], hi1[3];
-        uint32_t c0 = link(v, b, mid, depth + 1, lo0, hi0);
-        uint32_t c1 = link(v, mid, e, depth + 1, lo1, hi1);
+        /* ranges of the children: split the largest splittable one until four */
+        int rb[OA_BVH_WIDTH], re[OA_BVH_WIDTH], nc = 1;
+        rb[0] = b; re[0] = e;
+        while (nc < OA_BVH_WIDTH) {
+            int pick = -1;
+            double best = -1.0;
+            for (int c = 0; c < nc; ++c) {
+                if (re[c] - rb[c] <= leaf_size) continue;
+                double lo[3], hi[3];
+                box_of(v, rb[c], re[c], lo, hi);
+                const double a = area(lo, hi);
+                if (a > best) { best = a; pick = c; }
+            }
+            if (pick < 0) break;
+            const int mid = split(v, rb[pick], re[pick]);
+            rb[nc] = mid; re[nc] = re[pick];
+            re[pick] = mid;
+            ++nc;
+        }
+        /* leaves first: on equal entry distance the traversal keeps this order,
+           so a large enclosing leaf (a world volume) is tested before subtrees */
+        int order[OA_BVH_WIDTH];
+        for (int c = 0; c < nc; ++c) order[c] = c;
+        std::stable_sort(order, order + nc, [&](int x, int y) {
+            return (re[x] - rb[x] <= leaf_size) > (re[y] - rb[y] <= leaf_size);
+        });
+        uint32_t links[OA_BVH_WIDTH];
+        double clo[OA_BVH_WIDTH][3], chi[OA_BVH_WIDTH][3];
+        uint32_t below = 0;
+        for (int k = 0; k < nc; ++k) {
+            uint32_t n_below = 0;
+            const int c = order[k];
+            links[k] = link(v, rb[c], re[c], depth + 1, clo[k], chi[k], n_below);
+            below = std::max(below, n_below);
+        }
+        need = (uint32_t)(nc - 1) + below;
         BvhNode& n = nodes[idx];
-        store(n.lo0, n.hi0, lo0, hi0);
-        store(n.lo1, n.hi1, lo1, hi1);
-        n.c0 = c0; n.c1 = c1; n.depth = depth; n.pad = 0;
+        fill(n, nc, links, clo, chi);
+        n.depth = depth;
         return idx;
+    }
+
+    static void fill(BvhNode& n, int nc, const uint32_t* links, const double (*lo)[3], const double (*hi)[3])
+    {
+        for (int k = 0; k < OA_BVH_WIDTH; ++k) {
+            if (k < nc) {
+                float flo[3], fhi[3];
+                store(flo, fhi, lo[k], hi[k]);
+                n.lox[k] = flo[0]; n.loy[k] = flo[1]; n.loz[k] = flo[2];
+                n.hix[k] = fhi[0]; n.hiy[k] = fhi[1]; n.hiz[k] = fhi[2];
+                n.child[k] = links[k];
+            } else {                    /* empty slot: a box at +inf, which every slab test misses */
+                n.lox[k] = n.loy[k] = n.loz[k] = INFINITY;
+                n.hix[k] = n.hiy[k] = n.hiz[k] = INFINITY;
+                n.child[k] = OA_BVH_EMPTY;
+            }
+        }
+        n.pad[0] = n.pad[1] = n.pad[2] = 0;
     }
 
     /* build a tree; returns the root node index (always an internal node record) */
@@ -157,16 +216,17 @@ struct Builder {
         if (n <= leaf_size) {
             uint32_t idx = (uint32_t)nodes.size();
             nodes.push_back(BvhNode());
-            double lo[3], hi[3];
-            box_of(v, 0, n, lo, hi);
+            double lo[1][3], hi[1][3];
+            box_of(v, 0, n, lo[0], hi[0]);
             uint32_t c0 = leaf_link(v, 0, n);
-            BvhNode& r = nodes[idx];
-            store(r.lo0, r.hi0, lo, hi);
-            for (int k = 0; k < 3; ++k) { r.lo1[k] = 1.f; r.hi1[k] = -1.f; }
-            r.c0 = c0; r.c1 = OA_BVH_EMPTY; r.depth = 0; r.pad = 0;
+            fill(nodes[idx], 1, &c0, lo, hi);
+            nodes[idx].depth = 0;
             return idx;
         }
-        return node(v, 0, n, 0);
+        uint32_t need = 0;
+        const uint32_t root = node(v, 0, n, 0, need);
+        stack_need = std::max(stack_need, need);
+        return root;
     }
 };
 

@@ -117,7 +117,7 @@ struct mi_sim {
 
     /* geometry */
     int64_t n_inst = 0, n_prim = 0, n_tlas = 0;
-    uint32_t tlas_depth = 0, blas_depth = 0;
+    uint32_t tlas_depth = 0, blas_depth = 0, blas_stack = 0;
     std::vector<uint32_t> solid_blas_root;
     std::vector<uint32_t> h_inst_blas_root;
     std::vector<std::array<double, 6>> solid_box;   /* padded solid-frame prim box per solid */
@@ -308,7 +308,8 @@ static int build_tlas(mi_sim* sim, int leaf)
     oa_bvh::Builder tb(tlas, tlas_items, leaf);
     uint32_t troot = tb.build(titems);
     if (troot != 0) return fail(MI_ERR_GEOMETRY, "TLAS root is not node 0");
-    if (tb.max_depth + sim->blas_depth + 2 > OA_STACK_DEPTH)
+    /* deepest stack a walk can reach: TLAS pushes, then the BLAS walk on top */
+    if (tb.stack_need + sim->blas_stack + 1 > OA_STACK_DEPTH)
         return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
     HIPCHECK(hipStreamSynchronize(sim->stream));
     HIPCHECK(upload(sim->tlas, tlas.data(), tlas.size(), sim->stream));
@@ -328,7 +329,7 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     std::vector<uint32_t> blas_items;
     const unsigned char* sb = (const unsigned char*)g->solid;
     sim->solid_blas_root.resize(g->n_solid);
-    uint32_t bdepth = 0;
+    uint32_t bdepth = 0, bneed = 0;
     std::vector<std::array<double, 6>> solid_box(g->n_solid);
     for (int64_t so = 0; so < g->n_solid; ++so) {
         int32_t np, po;
@@ -350,6 +351,7 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
         oa_bvh::Builder b(blas, blas_items, 1);
         sim->solid_blas_root[so] = b.build(items);
         bdepth = std::max(bdepth, b.max_depth);
+        bneed = std::max(bneed, b.stack_need);
         solid_box[so] = bx;
     }
     /* instances: world->object transforms */
@@ -370,6 +372,7 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
         sim->h_inst_blas_root[i] = sim->solid_blas_root[gas];
     }
     sim->blas_depth = bdepth;
+    sim->blas_stack = bneed;
     sim->n_inst = ni;
     sim->n_prim = g->n_prim;
     int trc = build_tlas(sim, sim->cfg.bvh_leaf_size);

@@ -27,6 +27,15 @@
 __device__ unsigned long long oa_stats[OA_ST_N];
 #define OA_COUNT(i) atomicAdd(&oa_stats[(i)], 1ull)
 #define OA_COUNT_N(i, n) atomicAdd(&oa_stats[(i)], (unsigned long long)(n))
+/* cycles between a mark and an add, both at points where the wave's active lanes are
+   converged; counted once per wave */
+#define OA_CYC_MARK(v) const unsigned long long v = (unsigned long long)__builtin_readcyclecounter()
+#define OA_CYC_ADD(i, v)                                                                      \
+    do {                                                                                      \
+        const unsigned long long t_ = (unsigned long long)__builtin_readcyclecounter();       \
+        const unsigned long long m_ = __ballot(1);                                            \
+        if ((int)__lane_id() == __ffsll((long long)m_) - 1) atomicAdd(&oa_stats[(i)], t_ - (v)); \
+    } while (0)
 #define OA_WAVE_COUNT(w, l)                                                                   \
     do {                                                                                      \
         const unsigned long long m_ = __ballot(1);                                            \
@@ -45,6 +54,8 @@ __device__ unsigned long long oa_slow_n;
 __device__ float4 oa_slow[OA_SLOW_MAX * 3];
 #else
 #define OA_WAVE_COUNT(w, l) ((void)0)
+#define OA_CYC_MARK(v) ((void)0)
+#define OA_CYC_ADD(i, v) ((void)0)
 #define OA_COUNT(i) ((void)0)
 #define OA_COUNT_N(i, n) ((void)0)
 #endif

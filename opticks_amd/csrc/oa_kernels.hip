@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
     bool exhausted = false;
     const int period = P.refill_period > 0 ? P.refill_period : 1;
     for (int iter = 0;; ++iter) {
+        OA_CYC_MARK(c_iter);
         const unsigned long long idle = __ballot(!active);
         const unsigned long long all = __ballot(true);
         if (!exhausted && idle != 0ull && (idle == all || (iter % period) == 0)) {
@@ -235,6 +236,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
 #ifdef OA_STATS
         if (lane == 0) { OA_COUNT(OA_ST_ITER); OA_COUNT_N(OA_ST_ACTIVE, __popcll(act)); }
 #endif
+        OA_CYC_ADD(OA_ST_C_REFILL, c_iter);
         if (active) {
             if (lane_step<BLOCK, NoTape, (F & 1) != 0, (F & 2) != 0>(P, L, stk, tape)) {
                 lane_finish<(F & 1) != 0>(P, L);
@@ -243,6 +245,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
                 active = false;
             }
         }
+        OA_CYC_ADD(OA_ST_C_ITER, c_iter);
     }
 }
 
@@ -389,18 +392,25 @@ __global__ void __launch_bounds__(BLOCK) k_trace(SimParams P, const float4* rays
 
 /* --------------------------------------------------------------- refit */
 
+/* union of a node's non-empty child boxes */
+__device__ static void node_box(const BvhNode& n, float* lo, float* hi)
+{
+    for (int k = 0; k < 3; ++k) { lo[k] = 1e30f; hi[k] = -1e30f; }
+    for (int c = 0; c < OA_BVH_WIDTH; ++c) {
+        if (n.child[c] == OA_BVH_EMPTY) continue;
+        lo[0] = fminf(lo[0], n.lox[c]); lo[1] = fminf(lo[1], n.loy[c]); lo[2] = fminf(lo[2], n.loz[c]);
+        hi[0] = fmaxf(hi[0], n.hix[c]); hi[1] = fmaxf(hi[1], n.hiy[c]); hi[2] = fmaxf(hi[2], n.hiz[c]);
+    }
+}
+
 /* world box of each instance: transform the 8 corners of its solid's BLAS root box */
 __global__ void k_inst_boxes(const float* o2w, const BvhNode* blas, const uint32_t* inst_blas_root,
                              int64_t n_inst, float* inst_box, float pad_rel, float pad_abs)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_inst) return;
-    const BvhNode& r = blas[inst_blas_root[i]];
     float lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = fminf(r.c0 != OA_BVH_EMPTY ? r.lo0[k] : 1e30f, r.c1 != OA_BVH_EMPTY ? r.lo1[k] : 1e30f);
-        hi[k] = fmaxf(r.c0 != OA_BVH_EMPTY ? r.hi0[k] : -1e30f, r.c1 != OA_BVH_EMPTY ? r.hi1[k] : -1e30f);
-    }
+    node_box(blas[inst_blas_root[i]], lo, hi);
     const float* m = o2w + 16 * i;
     float wl[3] = { 1e30f, 1e30f, 1e30f }, wh[3] = { -1e30f, -1e30f, -1e30f };
     for (int c = 0; c < 8; ++c) {
@@ -429,11 +439,7 @@ __device__ static void link_box(const BvhNode* nodes, const uint32_t* items, con
             for (int k = 0; k < 3; ++k) { lo[k] = fminf(lo[k], b[k]); hi[k] = fmaxf(hi[k], b[3 + k]); }
         }
     } else {
-        const BvhNode& n = nodes[c];
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(n.c0 != OA_BVH_EMPTY ? n.lo0[k] : 1e30f, n.c1 != OA_BVH_EMPTY ? n.lo1[k] : 1e30f);
-            hi[k] = fmaxf(n.c0 != OA_BVH_EMPTY ? n.hi0[k] : -1e30f, n.c1 != OA_BVH_EMPTY ? n.hi1[k] : -1e30f);
-        }
+        node_box(nodes[c], lo, hi);
     }
 }
 
@@ -445,8 +451,13 @@ __global__ void k_refit_level(BvhNode* nodes, int64_t n_nodes, uint32_t depth, c
     if (i >= n_nodes) return;
     BvhNode& n = nodes[i];
     if (n.depth != depth) return;
-    link_box(nodes, items, inst_box, n.c0, n.lo0, n.hi0);
-    link_box(nodes, items, inst_box, n.c1, n.lo1, n.hi1);
+    for (int c = 0; c < OA_BVH_WIDTH; ++c) {
+        if (n.child[c] == OA_BVH_EMPTY) continue;
+        float lo[3], hi[3];
+        link_box(nodes, items, inst_box, n.child[c], lo, hi);
+        n.lox[c] = lo[0]; n.loy[c] = lo[1]; n.loz[c] = lo[2];
+        n.hix[c] = hi[0]; n.hiy[c] = hi[1]; n.hiz[c] = hi[2];
+    }
 }
 
 /* ------------------------------------------------------ host launchers */

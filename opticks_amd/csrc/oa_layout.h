@@ -2,10 +2,11 @@
  * oa_layout.h : records shared by the host runtime and the gfx950 kernels.
  *
  * HBM layout (all 16-byte aligned, read with dwordx4 loads):
- *   BvhNode   64 B  two child boxes + two child links ("wide-2" node: one
- *                   fetch tests both children). Link: bit31 set = leaf with
- *                   items [link & 0xffffff, +((link>>24)&0x7f)); 0xffffffff =
- *                   empty; otherwise the child node index.
+ *   BvhNode  128 B  four child boxes in SoA order (lo.x[4], lo.y[4], lo.z[4],
+ *                   hi.x[4], hi.y[4], hi.z[4]) + four child links: one node
+ *                   is seven dwordx4 loads and tests four children. Link: bit31
+ *                   set = leaf with items [link & 0xffffff, +((link>>24)&0x7f));
+ *                   0xffffffff = empty slot; otherwise the child node index.
  *   InstRec   64 B  world->object 3x4 (rows of qat4::right_multiply: m[0..2],
  *                   m[4..6], m[8..10] linear, m[12..14] translation), then
  *                   gas (solid) index, identity (sensor_identifier+1), blas root.
@@ -20,7 +21,8 @@
 
 #define OA_BVH_LEAF_BIT 0x80000000u
 #define OA_BVH_EMPTY    0xffffffffu
-#define OA_STACK_DEPTH  32
+#define OA_STACK_DEPTH  48
+#define OA_BVH_WIDTH    4
 #define OA_CSG_MAX_HEIGHT 7
 
 /* Optional work counters (build with -DOA_STATS): traces, TLAS node visits,
@@ -33,13 +35,15 @@ enum { OA_ST_TRACE, OA_ST_TLAS, OA_ST_INST, OA_ST_BLAS, OA_ST_PRIM, OA_ST_TREE, 
        OA_ST_ITER, OA_ST_ACTIVE,
        /* wave-level pairs: loop iterations executed by the wave, and lanes active in them */
        OA_ST_W_TRACE, OA_ST_L_TRACE, OA_ST_W_P1, OA_ST_L_P1, OA_ST_W_P2, OA_ST_L_P2,
-       OA_ST_W_PRIM, OA_ST_L_PRIM, OA_ST_W_PHYS, OA_ST_L_PHYS, OA_ST_N };
+       OA_ST_W_PRIM, OA_ST_L_PRIM, OA_ST_W_PHYS, OA_ST_L_PHYS,
+       /* wave-level shader-clock cycles spent in each part (first active lane adds) */
+       OA_ST_C_ITER, OA_ST_C_REFILL, OA_ST_C_TRACE, OA_ST_C_P1, OA_ST_C_P2, OA_ST_C_PRIM, OA_ST_N };
 
 struct BvhNode {
-    float lo0[3], hi0[3];
-    float lo1[3], hi1[3];
-    uint32_t c0, c1;
-    uint32_t depth, pad;
+    float lox[4], loy[4], loz[4];
+    float hix[4], hiy[4], hiz[4];
+    uint32_t child[4];
+    uint32_t depth, pad[3];
 };
 
 struct InstRec {

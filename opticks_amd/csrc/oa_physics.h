@@ -528,48 +528,68 @@ OA_D void gen_cerenkov(const SimParams& P, Photon& p, Rng& rng, const float4* gq
 /* ------------------------------------------------------------- trace */
 
 /* slab test; on a hit tn is the entry distance clamped to tmin */
-OA_D bool box_enter(const float* lo, const float* hi, v3 o, v3 inv, float tmin, float tmax, float& tn)
+/* Ray data for slab tests: t = fma(plane, inv, -oi) with inv = 1/d and oi = o*inv.
+   A direction component below 1e-20 in magnitude is replaced by +-1e-20 so inv
+   and oi stay finite and the slab of a parallel ray gives huge t of the right
+   sign. The FMA form rounds differently from (plane - o)*inv by ~ulp(o); box
+   tests only prune (every box is padded by >= 1e-2 mm around what it holds),
+   so the hits found do not depend on it. */
+OA_D void slab_ray(v3 o, v3 d, v3& inv, v3& oi)
 {
-    float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
-    float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
-    float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
-    tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    return tn <= tf;
+    const float e = 1e-20f;
+    inv = mk(1.f / (fabsf(d.x) < e ? copysignf(e, d.x) : d.x),
+             1.f / (fabsf(d.y) < e ? copysignf(e, d.y) : d.y),
+             1.f / (fabsf(d.z) < e ? copysignf(e, d.z) : d.z));
+    oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
 }
 
-OA_D void load_node(const BvhNode* nodes, uint32_t idx, float* lo0, float* hi0, float* lo1, float* hi1,
-                    uint32_t& c0, uint32_t& c1)
+/* entry distance into one child box, or +inf when the ray misses it in [tmin, tmax] */
+OA_D float box_enter(float lx, float ly, float lz, float hx, float hy, float hz, v3 inv, v3 oi, float tmin,
+                     float tmax)
 {
-    const float4* q = reinterpret_cast<const float4*>(nodes + idx);
-    float4 a = q[0], b = q[1], c = q[2], d = q[3];
-    lo0[0] = a.x; lo0[1] = a.y; lo0[2] = a.z; hi0[0] = a.w;
-    hi0[1] = b.x; hi0[2] = b.y; lo1[0] = b.z; lo1[1] = b.w;
-    lo1[2] = c.x; hi1[0] = c.y; hi1[1] = c.z; hi1[2] = c.w;
-    c0 = __float_as_uint(d.x); c1 = __float_as_uint(d.y);
+    const float x0 = fmaf(lx, inv.x, -oi.x), x1 = fmaf(hx, inv.x, -oi.x);
+    const float y0 = fmaf(ly, inv.y, -oi.y), y1 = fmaf(hy, inv.y, -oi.y);
+    const float z0 = fmaf(lz, inv.z, -oi.z), z1 = fmaf(hz, inv.z, -oi.z);
+    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    return tn <= tf ? tn : INFINITY;
 }
 
-/* Visit one wide-2 node: returns the nearer hit child link (or EMPTY) and
-   pushes the farther one. Leaf links go on the stack like node links, so
-   both levels are traversed strictly nearest-first and a shrinking tmax
-   culls what is still pending. */
+OA_D void cswap(float& ta, uint32_t& ca, float& tb, uint32_t& cb)
+{
+    const bool s = tb < ta;
+    const float t = s ? tb : ta;
+    const uint32_t c = s ? cb : ca;
+    tb = s ? ta : tb; cb = s ? ca : cb;
+    ta = t; ca = c;
+}
+
+/* Visit one 4-wide node: returns the nearest hit child link (or EMPTY) and
+   pushes the other hit children farthest first, so they pop nearest first.
+   Leaf links go on the stack like node links, so both levels are traversed
+   nearest-first and a shrinking tmax culls what is still pending. Equal entry
+   distances keep the node's child order (leaves first). */
 template <int STRIDE>
-OA_D uint32_t visit(const BvhNode* nodes, uint32_t idx, v3 o, v3 inv, float tmin, float tmax,
+OA_D uint32_t visit(const BvhNode* nodes, uint32_t idx, v3 inv, v3 oi, float tmin, float tmax,
                     uint32_t* stk, int& sp)
 {
-    float lo0[3], hi0[3], lo1[3], hi1[3];
-    uint32_t c0, c1;
-    load_node(nodes, idx, lo0, hi0, lo1, hi1, c0, c1);
-    float tn0, tn1;
-    const bool h0 = c0 != OA_BVH_EMPTY && box_enter(lo0, hi0, o, inv, tmin, tmax, tn0);
-    const bool h1 = c1 != OA_BVH_EMPTY && box_enter(lo1, hi1, o, inv, tmin, tmax, tn1);
-    if (h0 && h1) {
-        const bool first0 = tn0 <= tn1;
-        stk[sp * STRIDE] = first0 ? c1 : c0;
-        sp++;
-        return first0 ? c0 : c1;
-    }
-    return h0 ? c0 : (h1 ? c1 : OA_BVH_EMPTY);
+    const float4* q = reinterpret_cast<const float4*>(nodes + idx);
+    const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+    const uint4 ch = *reinterpret_cast<const uint4*>(q + 6);
+    float t0 = box_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, inv, oi, tmin, tmax);
+    float t1 = box_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, inv, oi, tmin, tmax);
+    float t2 = box_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, inv, oi, tmin, tmax);
+    float t3 = box_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, inv, oi, tmin, tmax);
+    uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+    cswap(t0, c0, t1, c1);
+    cswap(t2, c2, t3, c3);
+    cswap(t0, c0, t2, c2);
+    cswap(t1, c1, t3, c3);
+    cswap(t1, c1, t2, c2);
+    if (t3 < INFINITY) { stk[sp * STRIDE] = c3; sp++; }
+    if (t2 < INFINITY) { stk[sp * STRIDE] = c2; sp++; }
+    if (t1 < INFINITY) { stk[sp * STRIDE] = c1; sp++; }
+    return t0 < INFINITY ? c0 : OA_BVH_EMPTY;
 }
 
 OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
@@ -599,7 +619,9 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 {
     OA_COUNT(OA_ST_TRACE);
     OA_WAVE_COUNT(OA_ST_W_TRACE, OA_ST_L_TRACE);
-    const v3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    OA_CYC_MARK(c_trace);
+    v3 inv, oi;
+    slab_ray(o, d, inv, oi);
     float best_t = tmax;
     uint64_t best_key = ~0ull;
     uint32_t best_inst = 0xffffffffu, best_prim = 0u;
@@ -613,6 +635,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #endif
     while (!done) {
         /* phase 1: walk nodes until this lane has a leaf or nothing left */
+        OA_CYC_MARK(c_p1);
         while (!done && leaf == OA_BVH_EMPTY) {
             OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
 #ifdef OA_STATS
@@ -627,11 +650,13 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 cur = OA_BVH_EMPTY;
             } else {
                 OA_COUNT(OA_ST_TLAS);
-                cur = visit<STRIDE>(P.tlas, cur, o, inv, tmin, best_t, stk, sp);
+                cur = visit<STRIDE>(P.tlas, cur, inv, oi, tmin, best_t, stk, sp);
             }
         }
+        OA_CYC_ADD(OA_ST_C_P1, c_p1);
         if (leaf == OA_BVH_EMPTY) continue;
         /* phase 2: the pending instance leaf */
+        OA_CYC_MARK(c_p2);
         const uint32_t first = leaf & 0x00ffffffu, count = (leaf >> 24) & 0x7fu;
         leaf = OA_BVH_EMPTY;
         for (uint32_t k = 0; k < count; ++k) {
@@ -640,7 +665,8 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             const float4* m = reinterpret_cast<const float4*>(P.inst + ii);
             v3 oo, od;
             inst_ray(m, o, d, oo, od);
-            const v3 inv2 = mk(1.f / od.x, 1.f / od.y, 1.f / od.z);
+            v3 inv2, oi2;
+            slab_ray(oo, od, inv2, oi2);
             const int sp_base = sp;
             uint32_t bc = __float_as_uint(m[2].w);     /* BLAS root node of this solid */
             for (;;) {
@@ -656,10 +682,11 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 }
                 if (!(bc & OA_BVH_LEAF_BIT)) {
                     OA_COUNT(OA_ST_BLAS);
-                    bc = visit<STRIDE>(P.blas, bc, oo, inv2, tmin, best_t, stk, sp);
+                    bc = visit<STRIDE>(P.blas, bc, inv2, oi2, tmin, best_t, stk, sp);
                     continue;
                 }
                 const uint32_t bf = bc & 0x00ffffffu, bn = (bc >> 24) & 0x7fu;
+                OA_CYC_MARK(c_prim);
                 for (uint32_t j = 0; j < bn; ++j) {
                     const uint32_t pi = P.blas_items[bf + j];
                     const PrimRec pr = P.prim[pi];
@@ -675,10 +702,13 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                         best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;
                     }
                 }
+                OA_CYC_ADD(OA_ST_C_PRIM, c_prim);
                 bc = OA_BVH_EMPTY;
             }
         }
+        OA_CYC_ADD(OA_ST_C_P2, c_p2);
     }
+    OA_CYC_ADD(OA_ST_C_TRACE, c_trace);
 #ifdef OA_STATS
     atomicAdd(&oa_hist[0][n1 ? 32 - __clz(n1) : 0], 1ull);
     atomicAdd(&oa_hist[1][n2 ? 32 - __clz(n2) : 0], 1ull);
