@@ -603,6 +603,23 @@ OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
             r0.z * d.x + r1.z * d.y + r2.z * d.z);
 }
 
+/* one prim of one instance against the nearest hit so far (t, then the (instance, prim) key) */
+template <bool EXT>
+OA_D void consider_prim(const SimParams& P, uint32_t ii, uint32_t pi, v3 oo, v3 od, float tmin, float& best_t,
+                        uint64_t& best_key, uint32_t& best_inst, uint32_t& best_prim, Isect& best_is)
+{
+    const PrimRec pr = P.prim[pi];
+    Isect is;
+    if (!intersect_prim<EXT>(is, P.node + 4u * pr.node_offset, P.itra, P.plan, tmin, oo, od)) return;
+    /* a reported hit outside [tmin, tmax] is ignored, as optixReportIntersection does
+       (a convex polyhedron behind the ray reports valid with t = 0) */
+    if (!(is.w >= tmin)) return;
+    const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
+    if (is.w < best_t || (is.w == best_t && key < best_key)) {
+        best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;
+    }
+}
+
 /*
  * Nearest hit over instances x prims with t in (tmin, tmax); exact ties go to
  * the smaller (instance, prim) key so the answer does not depend on traversal
@@ -689,18 +706,9 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 OA_CYC_MARK(c_prim);
                 for (uint32_t j = 0; j < bn; ++j) {
                     const uint32_t pi = P.blas_items[bf + j];
-                    const PrimRec pr = P.prim[pi];
-                    Isect is;
                     OA_COUNT(OA_ST_PRIM);
                     OA_WAVE_COUNT(OA_ST_W_PRIM, OA_ST_L_PRIM);
-                    if (!intersect_prim<EXT>(is, P.node + 4u * pr.node_offset, P.itra, P.plan, tmin, oo, od)) continue;
-                    /* a reported hit outside [tmin, tmax] is ignored, as optixReportIntersection does
-                       (a convex polyhedron behind the ray reports valid with t = 0) */
-                    if (!(is.w >= tmin)) continue;
-                    const uint64_t key = ((uint64_t)ii << 32) | (uint64_t)pi;
-                    if (is.w < best_t || (is.w == best_t && key < best_key)) {
-                        best_t = is.w; best_key = key; best_inst = ii; best_prim = pi; best_is = is;
-                    }
+                    consider_prim<EXT>(P, ii, pi, oo, od, tmin, best_t, best_key, best_inst, best_prim, best_is);
                 }
                 OA_CYC_ADD(OA_ST_C_PRIM, c_prim);
                 bc = OA_BVH_EMPTY;
