@@ -142,7 +142,8 @@ struct mi_sim {
     /* work */
     DBuf<float> gs;
     DBuf<int64_t> prefix, block_total, evbuf;
-    DBuf<int32_t> word_prefix, ev_id;
+    DBuf<int32_t> word_prefix, ev_id, gs_event;
+    DBuf<uint32_t> seed;                 /* genstep of each photon of the launch */
     DBuf<unsigned long long> words, next;
     /* outputs of the host-array entry point */
     DBuf<float4> photon, hit;
@@ -485,6 +486,7 @@ void mi_sim_destroy(mi_sim* sim)
     sim->bnd.release(); sim->optical.release(); sim->icdf.release(); sim->precooked.release();
     sim->gs.release(); sim->prefix.release(); sim->block_total.release(); sim->evbuf.release();
     sim->word_prefix.release(); sim->ev_id.release(); sim->words.release(); sim->next.release();
+    sim->seed.release(); sim->gs_event.release();
     sim->photon.release(); sim->hit.release(); sim->seq.release();
     sim->adj_mat_block.release(); sim->adj_sample_bin.release(); sim->adj_rows.release(); sim->adj_comp.release();
     sim->adj_key.release(); sim->adj_val.release(); sim->adj_key2.release(); sim->adj_val2.release();
@@ -556,6 +558,12 @@ static int launch(mi_sim* sim, SimParams& P, hipStream_t s)
     const int64_t n_words = (P.n_photon + 63) / 64;
     HIPCHECK(sim->words.reserve((size_t)n_words + 1));
     P.hitwords = sim->words.p;
+    HIPCHECK(sim->seed.reserve((size_t)P.n_photon));
+    HIPCHECK(sim->gs_event.reserve((size_t)std::max<int64_t>(P.n_gs, 1)));
+    HIPCHECK(oa_launch::seed_map(P.gs_prefix, P.n_gs, P.n_photon, sim->seed.p, P.ev_gs_begin, P.n_event,
+                                 sim->gs_event.p, s));
+    P.seed_map = sim->seed.p;
+    P.gs_event = sim->gs_event.p;
     if (sim->cfg.kernel == MI_KERNEL_WAVEFRONT) {
         HIPCHECK(oa_launch::zero_u64(sim->words.p, n_words, s));
         HIPCHECK(oa_launch::zero_u64(sim->next.p, 1, s));
@@ -1094,7 +1102,8 @@ struct mi_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = { nullptr, nullptr };
     DBuf<int64_t> prefix, saved_prefix, evbuf, block_total, ho;
-    DBuf<int32_t> ev_id, word_prefix;
+    DBuf<int32_t> ev_id, word_prefix, gs_event;
+    DBuf<uint32_t> seed;
     DBuf<unsigned long long> words, next, flag;
     int64_t* h_out = nullptr;   /* pinned: hit offsets (n_event+1), then the prefix-mismatch flag */
     bool launched = false;
@@ -1126,6 +1135,7 @@ int mi_batch_destroy(mi_batch* b)
     if (b->h_out) (void)hipHostFree(b->h_out);
     b->prefix.release(); b->saved_prefix.release(); b->evbuf.release(); b->block_total.release(); b->ho.release();
     b->ev_id.release(); b->word_prefix.release(); b->words.release(); b->next.release(); b->flag.release();
+    b->seed.release(); b->gs_event.release();
     b->mat_block.release(); b->sample_bin.release(); b->rows.release(); b->comp.release();
     b->key.release(); b->val.release(); b->key2.release(); b->val2.release(); b->tmp.release(); b->seg.release();
     delete b;
@@ -1141,6 +1151,10 @@ static hipError_t batch_work(mi_batch* b, const SimParams& P, const void* d_gens
     if ((e = oa_launch::gs_prefix((const float*)d_gensteps, b->n_gs, b->prefix.p, s)) != hipSuccess) return e;
     if ((e = oa_launch::zero_u64(b->flag.p, 1, s)) != hipSuccess) return e;
     if ((e = oa_launch::prefix_check(b->prefix.p, b->saved_prefix.p, b->n_gs + 1, b->flag.p, s)) != hipSuccess) return e;
+    /* the seed map follows the captured prefix; a changed prefix is reported by the flag and the
+       replay fails on the host, so its photons are never used */
+    if ((e = oa_launch::seed_map(b->saved_prefix.p, b->n_gs, b->n_photon, b->seed.p, P.ev_gs_begin, b->n_event,
+                                 b->gs_event.p, s)) != hipSuccess) return e;
     if (sim->cfg.kernel == MI_KERNEL_WAVEFRONT) {
         if ((e = oa_launch::zero_u64(b->words.p, b->n_words, s)) != hipSuccess) return e;
         if ((e = oa_launch::zero_u64(b->next.p, 1, s)) != hipSuccess) return e;
@@ -1221,6 +1235,8 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
     BCHECK(upload(b->evbuf, evh.data(), evh.size(), s));
     BCHECK(upload(b->ev_id, event_ids, (size_t)n_event, s));
     BCHECK(b->words.reserve((size_t)b->n_words + 1));
+    BCHECK(b->seed.reserve((size_t)b->n_photon));
+    BCHECK(b->gs_event.reserve((size_t)n_gs));
     BCHECK(b->next.reserve(1));
     BCHECK(b->flag.reserve(1));
     BCHECK(b->word_prefix.reserve((size_t)b->n_words + 1));
@@ -1243,6 +1259,8 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
     P.seq = (uint64_t*)d_seq;
     P.hitwords = b->words.p;
     P.next = b->next.p;
+    P.seed_map = b->seed.p;
+    P.gs_event = b->gs_event.p;
     if (adj) {
         if (!d_hit || hit_capacity <= 0) return bail(fail(MI_ERR_ARG, "the batch adjoint needs the hit buffer"));
         if (!d_grad) return bail(fail(MI_ERR_ARG, "null gradient buffer"));

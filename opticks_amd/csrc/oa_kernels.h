@@ -4,6 +4,9 @@
 #include "oa_layout.h"
 
 namespace oa_launch {
+/* genstep of each photon (seed) and, for n_event > 1, event of each genstep */
+hipError_t seed_map(const int64_t* d_prefix, int64_t n_gs, int64_t n_photon, uint32_t* d_seed,
+                    const int64_t* d_ev_gs_begin, int32_t n_event, int32_t* d_gs_event, hipStream_t s);
 hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s);
 hipError_t simulate(const SimParams& P, int kernel, int block, int grid, hipStream_t s);
 int wave_blocks_per_cu(int block);   /* resident blocks of the persistent kernel per CU */

@@ -2,8 +2,11 @@
  * oa_kernels.hip : gfx950 kernels of the propagation path.
  *
  *   k_gs_prefix       exclusive prefix of genstep numphoton (QEvt.cu:181-226 seeding
- *                     reduce+expand; photons find their genstep by binary search
- *                     instead of a per-photon seed array)
+ *                     reduce)
+ *   k_seed_fill       the expand: genstep index of every photon, one block per
+ *                     genstep writing its photon range (QEvt.cu:181-226 seed
+ *                     buffer, sysrap/iexpand.h); k_gs_event the event of every
+ *                     genstep for batched launches
  *   k_sim_plain       one lane per photon, the CSGOptiX7.cu:432-530 shape
  *   k_sim_wave        persistent waves: lanes whose photon finished take a new
  *                     one from a global counter every refill_period bounces
@@ -32,6 +35,28 @@ using namespace oa;
 #endif
 
 /* ------------------------------------------------------------ seeding */
+
+/* seed[i] = genstep of photon i for i < n_photon: blocks stride over gensteps,
+   threads over each genstep's photon range [prefix[g], prefix[g+1]) (coalesced
+   writes). Photons past the gensteps' total (a caller's n_photon larger than
+   the gensteps hold) get the last genstep, as the binary search gave them. */
+__global__ void __launch_bounds__(256) k_seed_fill(const int64_t* prefix, int64_t n_gs, int64_t n_photon,
+                                                   uint32_t* seed)
+{
+    for (int64_t g = blockIdx.x; g < n_gs; g += gridDim.x) {
+        const int64_t b = prefix[g], e = prefix[g + 1] < n_photon ? prefix[g + 1] : n_photon;
+        for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) seed[i] = (uint32_t)g;
+    }
+    if (blockIdx.x == 0)
+        for (int64_t i = prefix[n_gs] + threadIdx.x; i < n_photon; i += blockDim.x) seed[i] = (uint32_t)(n_gs - 1);
+}
+
+/* gs_event[g] = event owning genstep g, from the events' genstep boundaries */
+__global__ void __launch_bounds__(256) k_gs_event(const int64_t* ev_gs_begin, int32_t n_event, int32_t* gs_event)
+{
+    for (int32_t e = blockIdx.x; e < n_event; e += gridDim.x)
+        for (int64_t g = ev_gs_begin[e] + threadIdx.x; g < ev_gs_begin[e + 1]; g += blockDim.x) gs_event[g] = e;
+}
 
 __global__ void __launch_bounds__(1024) k_gs_prefix(const float* gs, int64_t n_gs, int64_t* prefix)
 {
@@ -94,8 +119,14 @@ OA_D void lane_start(const SimParams& P, Lane& L, int64_t g)
 {
     L.g = g;
     int e = 0;
-    if (P.n_event > 1) e = (int)upper_index(P.ev_ph_begin, 0, P.n_event, g);
-    const int64_t gsi = upper_index(P.gs_prefix, P.ev_gs_begin[e], P.ev_gs_begin[e + 1], g);
+    int64_t gsi;
+    if (P.seed_map) {
+        gsi = (int64_t)P.seed_map[g];
+        if (P.n_event > 1) e = P.gs_event[gsi];
+    } else {
+        if (P.n_event > 1) e = (int)upper_index(P.ev_ph_begin, 0, P.n_event, g);
+        gsi = upper_index(P.gs_prefix, P.ev_gs_begin[e], P.ev_gs_begin[e + 1], g);
+    }
     const int64_t in_event = g - P.ev_ph_begin[e];
     const uint64_t photon_idx = (uint64_t)(P.photon_offset + in_event);
     L.rng.kind = P.rng_kind;
@@ -500,6 +531,18 @@ extern "C" int oa_stats_read(unsigned long long* out, int reset)
     (void)reset;
 #endif
     return 0;
+}
+
+hipError_t seed_map(const int64_t* d_prefix, int64_t n_gs, int64_t n_photon, uint32_t* d_seed,
+                    const int64_t* d_ev_gs_begin, int32_t n_event, int32_t* d_gs_event, hipStream_t s)
+{
+    const unsigned gb = (unsigned)std::min<int64_t>(std::max<int64_t>(n_gs, 1), 65535);
+    hipLaunchKernelGGL(k_seed_fill, dim3(gb), dim3(256), 0, s, d_prefix, n_gs, n_photon, d_seed);
+    if (n_event > 1) {
+        const unsigned eb = (unsigned)std::min<int32_t>(n_event, 65535);
+        hipLaunchKernelGGL(k_gs_event, dim3(eb), dim3(256), 0, s, d_ev_gs_begin, n_event, d_gs_event);
+    }
+    return hipGetLastError();
 }
 
 hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s)

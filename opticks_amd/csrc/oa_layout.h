@@ -91,6 +91,8 @@ struct SimParams {
     /* event(s): photons are numbered globally over all events of the launch */
     const float*   gensteps;      /* (n_gs,6,4) all events concatenated */
     const int64_t* gs_prefix;     /* (n_gs+1) exclusive prefix of numphoton */
+    const uint32_t* seed_map;     /* (n_photon) genstep of each photon, or null: binary search of gs_prefix */
+    const int32_t* gs_event;      /* (n_gs) event of each genstep (with seed_map and n_event > 1) */
     int64_t        n_gs;
     int32_t        n_event;
     const int64_t* ev_gs_begin;   /* (n_event+1) genstep index boundaries */
