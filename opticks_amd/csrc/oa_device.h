@@ -849,6 +849,14 @@ OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, cons
     if (typecode < TC_NODE) {
         OA_COUNT(OA_ST_TREE);
         OA_WAVE_COUNT(OA_ST_W_TREE, OA_ST_L_TREE);
+#ifdef OA_STATS
+        if (__float_as_uint(root[0].x) <= 15u) {
+            OA_CYC_MARK(c_tree);
+            const bool h = intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd);
+            OA_CYC_ADD(OA_ST_C_TREE, c_tree);
+            return h;
+        }
+#endif
         if (__float_as_uint(root[0].x) <= 15u) return intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd);
         const float4 r = intersect_tree_deep<EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
         is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
