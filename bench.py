@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kernel", default="wavefront", choices=["wavefront", "plain"])
-    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--refill", type=int, default=1)
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--waves-per-cu", type=int, default=0)

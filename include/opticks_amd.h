@@ -137,7 +137,7 @@ typedef struct mi_config {
     uint32_t output_mask;      /* MI_OUT_* bits                   */
     int64_t  max_slot;         /* photons per launch, 0 = from free VRAM */
     int32_t  kernel;           /* MI_KERNEL_*                     */
-    int32_t  block_size;       /* threads per block (64..1024, multiple of 64) */
+    int32_t  block_size;       /* threads per block: 64 (default), 128 or 256 */
     int32_t  refill_period;    /* wavefront kernel: bounces between lane refills */
     int32_t  bvh_leaf_size;    /* instances per TLAS leaf (1..8)  */
     int32_t  waves_per_cu;     /* wavefront kernel: resident waves per CU (0 = auto) */

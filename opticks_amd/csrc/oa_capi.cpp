@@ -183,7 +183,7 @@ void mi_config_default(mi_config* c)
     c->output_mask = MI_OUT_PHOTON | MI_OUT_SEQ | MI_OUT_HIT;
     c->max_slot = 0;
     c->kernel = MI_KERNEL_WAVEFRONT;
-    c->block_size = 128;
+    c->block_size = 64;
     c->refill_period = 1;
     c->bvh_leaf_size = 1;
     c->waves_per_cu = 0;

@@ -528,18 +528,19 @@ OA_D void gen_cerenkov(const SimParams& P, Photon& p, Rng& rng, const float4* gq
 /* ------------------------------------------------------------- trace */
 
 /* slab test; on a hit tn is the entry distance clamped to tmin */
-/* Ray data for slab tests: t = fma(plane, inv, -oi) with inv = 1/d and oi = o*inv.
-   A direction component below 1e-20 in magnitude is replaced by +-1e-20 so inv
-   and oi stay finite and the slab of a parallel ray gives huge t of the right
-   sign. The FMA form rounds differently from (plane - o)*inv by ~ulp(o); box
-   tests only prune (every box is padded by >= 1e-2 mm around what it holds),
-   so the hits found do not depend on it. */
+/* Ray data for slab tests: t = fma(plane, inv, -oi) with inv ~ 1/d (the hardware
+   reciprocal, 1 ulp) and oi = o*inv. A direction component below 1e-20 in
+   magnitude is replaced by +-1e-20 so inv and oi stay finite and the slab of a
+   parallel ray gives huge t of the right sign. The reciprocal and the FMA form
+   round differently from (plane - o)/d by a few ulp of o; box tests only prune
+   (every box is padded by >= 1e-2 mm around what it holds), so the hits found
+   do not depend on it. */
 OA_D void slab_ray(v3 o, v3 d, v3& inv, v3& oi)
 {
     const float e = 1e-20f;
-    inv = mk(1.f / (fabsf(d.x) < e ? copysignf(e, d.x) : d.x),
-             1.f / (fabsf(d.y) < e ? copysignf(e, d.y) : d.y),
-             1.f / (fabsf(d.z) < e ? copysignf(e, d.z) : d.z));
+    inv = mk(__builtin_amdgcn_rcpf(fabsf(d.x) < e ? copysignf(e, d.x) : d.x),
+             __builtin_amdgcn_rcpf(fabsf(d.y) < e ? copysignf(e, d.y) : d.y),
+             __builtin_amdgcn_rcpf(fabsf(d.z) < e ? copysignf(e, d.z) : d.z));
     oi = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
 }
 

@@ -71,7 +71,7 @@ class SEventConfig:
     output_mask: int = L.MI_OUT_PHOTON | L.MI_OUT_SEQ | L.MI_OUT_HIT
     max_slot: int = 0
     kernel: int = L.MI_KERNEL_WAVEFRONT
-    block_size: int = 128
+    block_size: int = 64
     refill_period: int = 1
     bvh_leaf_size: int = 1
     waves_per_cu: int = 0
