@@ -15,7 +15,9 @@ Also reported: the dominant kernel's roofline (HIP events on the kernel's
 stream), the CPU checker's rate on a bounded sample on this host, the
 adjoint/forward device-time ratio on the same geometry with 256 binned
 LS/water absorption and Rayleigh lengths (10M photons, 2,000 detectors), and
-the batched 64-event workload (autotuned, replayed from a captured hipGraph).
+the batched 64-event workload (autotuned, replayed from a captured hipGraph),
+and the rows around the core path (hit merge, sphotonlite output, Cerenkov
+generation, simtrace), each timed on the GPU beside its CPU counterpart.
 
 --copy-calibration-mib M appends one device copy of M MiB after the timed
 region: a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE counters.
@@ -145,6 +147,7 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-event batched workload")
     ap.add_argument("--no-full", action="store_true", help="skip the 256-event forward+adjoint graph")
+    ap.add_argument("--no-next", action="store_true", help="skip the hit merge / lite / Cerenkov / simtrace rows")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -248,12 +251,128 @@ def main():
                                 "value_1_thread": rate1,
                                 "sample_1_thread": f"first {sample1} photons, {cdt1:.1f} s on 1 thread",
                                 "host_cpu": cpu_model()}
+    if not args.no_next:
+        line["next_rows"] = bench_next_rows(torch, dev, sim, gs, photon, max(1, min(args.steps, 3)), not args.no_cpu)
     if not args.no_batched:
         line["batched"] = bench_batched(torch, dev, max(1, min(args.steps, 3)))
     if not args.no_full:
         line["full_batched_adjoint"] = bench_full(torch, dev, max(1, min(args.steps, 2)))
     print(json.dumps(line), flush=True)
     sim.close()
+
+
+def _timed(torch, fn, steps):
+    """mean wall seconds of fn() bracketed by device synchronizations (fn is synchronous)"""
+    fn()
+    ts = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.mean(ts)), r
+
+
+def bench_next_rows(torch, dev, sim, gs, photon, steps, cpu):
+    """Hit merge (SPM) of the forward run's 10M sphoton records, the sphotonlite
+    output form of the same forward, Cerenkov generation + propagation (10M photons
+    in a water sphere), and simtrace of a 201 x 201 XZ grid (10.1M rays) over the
+    2,000-detector geometry. GPU times are wall clock around synchronous calls
+    (merge, simtrace) or HIP events (propagation); CPU counterparts run the
+    checker's restatements on this host."""
+    from opticks_amd import configs
+    from opticks_amd.qsim import QSim, SEventConfig, flag_mask
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {}
+    N = photon.shape[0]
+    sim.simulate_device(gs, photon, None, None, n_photon=N)
+    mask = flag_mask("SD")
+    t, merged = _timed(torch, lambda: sim.merge_hits_device(photon, select_mask=mask, time_window=1.0), steps)
+    n_sel = int(((photon[:, 3, 3].view(torch.int32) & mask) != 0).sum().item())
+    row = {"workload": "SPM merge_partial_select of the 10M sphoton records of the forward run: select SD, "
+                       "key (identity, time / 1 ns), stable sort, run reduce",
+           "records": int(N), "selected": n_sel, "merged": int(merged.shape[0]), "ms": 1e3 * t,
+           "records_per_s": N / t, "selected_per_s": n_sel / t,
+           "input_gb_s": N * 64 / t / 1e9, "input_hbm_frac": N * 64 / t / 1e9 / HBM_PEAK_GBS}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import spm
+        host = photon.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = spm.merge_photons(host, mask, 1.0)
+        tc = time.perf_counter() - t0
+        row["cpu_baseline"] = {"value": N / tc, "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"all {N} records, numpy restatement of SPM (oracle/spm.py), {tc:.2f} s",
+                               "merged": int(len(ref))}
+    out["hit_merge"] = row
+
+    lite = torch.empty((N, 4), dtype=torch.int32, device=dev)
+    full_ms, lite_ms = [], []
+    for _ in range(steps + 1):
+        sim.simulate_device(gs, photon, None, None, n_photon=N)
+        full_ms.append(sim.last_timing()["total_ms"])
+        sim.simulate_lite_device(gs, lite, n_photon=N)
+        lite_ms.append(sim.last_timing()["total_ms"])
+    lm, fm = float(np.mean(lite_ms[1:])), float(np.mean(full_ms[1:]))
+    out["photonlite"] = {"workload": "the 10M forward with only sphotonlite (16 B) records out",
+                         "ms": lm, "photons_per_s": N / (lm * 1e-3),
+                         "photon_output_ms": fm, "photon_output_photons_per_s": N / (fm * 1e-3)}
+    del lite
+
+    wc = configs.cerenkov_water(num_photon=10_000_000, n_gs=1000)
+    sc = QSim(wc.geometry, wc.physics, SEventConfig(seed=wc.seed))
+    gsc = torch.from_numpy(wc.gensteps).to(dev)
+    Nc = wc.num_photon
+    pc = torch.empty((Nc, 4, 4), dtype=torch.float32, device=dev)
+    ms = []
+    for _ in range(steps + 1):
+        sc.simulate_device(gsc, pc, None, None, n_photon=Nc)
+        ms.append(sc.last_timing()["total_ms"])
+    cm = float(np.mean(ms[1:]))
+    row = {"workload": "Cerenkov gensteps (1000 x 10000 photons, 10 mm steps at beta 0.99 -> 0.985, 200-600 nm) "
+                       "in the water sphere of the 1M-photon geometry (vacuum box with a detecting inner "
+                       "surface): generation + propagation, photon records out",
+           "n_photon": int(Nc), "ms": cm, "photons_per_s": Nc / (cm * 1e-3)}
+    if cpu:
+        import pyoracle
+        W = pyoracle.World(wc.geometry, wc.physics)
+        sample = Nc
+        t0 = time.perf_counter()
+        W.simulate(wc.gensteps, seed=wc.seed, p_end=sample, nthreads=threads)
+        tc = time.perf_counter() - t0
+        row["cpu_baseline"] = {"value": sample / tc, "unit": "photons/s", "cores": threads, "kind": "port",
+                               "sample": f"first {sample} photons, {tc:.2f} s on {threads} threads"}
+    out["cerenkov"] = row
+    sc.close()
+    del pc
+
+    fg = configs.frame_gensteps((0.0, 0.0, 0.0, 6500.0), (-100, 100, 0, 0, -100, 100, 250, 1), gridscale=0.01)
+    fgt = torch.from_numpy(fg).to(dev)
+    n_ray = int(fg.view(np.uint32)[:, 0, 3].sum())
+    rec = torch.empty((n_ray, 4, 4), dtype=torch.float32, device=dev)
+    t, _ = _timed(torch, lambda: sim.simtrace_device(fgt, rec, n_photon=n_ray), steps)
+    row = {"workload": "simtrace of a 201 x 201 XZ grid x 250 rays (65 mm pitch over +-6.5 m) through the "
+                       "2000-detector geometry, sevent simtrace records out",
+           "rays": n_ray, "ms": 1e3 * t, "rays_per_s": n_ray / t}
+    if cpu:
+        import pyoracle
+        W = pyoracle.World(*_config1_arrays())
+        sample = fg
+        ns = int(sample.view(np.uint32)[:, 0, 3].sum())
+        t0 = time.perf_counter()
+        W.simtrace(sample, nthreads=threads, seed=42)
+        tc = time.perf_counter() - t0
+        row["cpu_baseline"] = {"value": ns / tc, "unit": "rays/s", "cores": threads, "kind": "port",
+                               "sample": f"first {ns} rays, {tc:.2f} s on {threads} threads"}
+    out["simtrace"] = row
+    return out
+
+
+def _config1_arrays():
+    from opticks_amd import configs
+    w = configs.config1(num_gs=1, per_gs=1)
+    return w.geometry, w.physics
 
 
 def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
