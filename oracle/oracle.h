@@ -7,21 +7,35 @@
  *
  * It restates, in plain C99 + OpenMP, one photon per iteration:
  *   generation        qudarap/qsim.h:2532-2561, sysrap/storch.h:192-520,
- *                     qudarap/qscint.h:66-200
+ *                     qudarap/qscint.h:66-200, qudarap/qcerenkov.h
  *   bounce loop       CSGOptiX/CSGOptiX7.cu:432-530
  *   world trace       optixTrace + __intersection__is/__closesthit__ch/__miss__ms
  *                     (CSGOptiX7.cu:113-218,682-967): nearest hit over all
  *                     instances x prims, here by brute force or a plain BVH
- *   CSG intersect     CSG/csg_intersect_tree.h:277-719,
- *                     CSG/csg_intersect_leaf.h:174-324 and leaf headers
+ *   CSG intersect     CSG/csg_intersect_tree.h:277-719, csg_intersect_node.h:392-937
+ *                     (list nodes), CSG/csg_intersect_leaf.h:174-324 and leaf headers
  *   physics           qudarap/qsim.h:180-201,433-575,604-866,1001-1208,
  *                     1680-1758,1988-2093,2229-2338; qudarap/qbnd.h:108-223
  *   history           sysrap/sseq.h:174-183, sysrap/sctx.h:134-182
+ *   outputs           sphotonlite (sysrap/sphotonlite.h:41-60), simtrace records
+ *                     (qudarap/qsim.h:2460-2522, sysrap/sevent.h:670-696)
+ *   adjoint rows      score-function terms of the binned lengths along each path
  *
  * The bnd/icdf "textures" are read with FP32 linear interpolation and wrap
  * addressing (qudarap/QTex.cc:293-305) instead of the GPU texture unit.
  * Transcendentals and the Philox streams come from oa_portable.h, which is
  * pinned separately against glibc and the Philox known-answer vectors.
+ *
+ * How each part is pinned (tests/): CSG intersect by bitwise equality with the
+ * reference's own CSG headers compiled under oracle/_ref and by committed
+ * golden vectors; generation by the geometry of each torch type, the
+ * scintillation and Cerenkov distributions (cone angle, spectrum, yield,
+ * polarization, time); boundary/surface/bulk physics by closed-form
+ * probabilities, Fresnel coefficients and Rayleigh invariants; simtrace by the
+ * grid rules and agreement with trace; adjoint rows by closed forms on a water
+ * sphere. Parity unpinned: curand's stream values (this checker and the
+ * kernels share their own Philox convention) and the GPU texture unit's 8-bit
+ * filtering weights, neither of which can be reproduced offline.
  */
 #ifndef OA_ORACLE_H
 #define OA_ORACLE_H
