@@ -197,7 +197,7 @@ def main():
     # SQ_INSTS_VALU, SQ_THREAD_CYCLES_VALU, SQ_ACTIVE_INST_VALU)
     traffic, valu = None, None
     pj = profile_json("pmc_k_sim_wave_csg10M.json")
-    if pj and pj.get("n_photon") == N and pj.get("kernel") == kname:
+    if pj and pj.get("n_photon") == N and str(pj.get("kernel", "")).split("<")[0] == kname:
         traffic = pj.get("hbm_bytes_per_launch")
         if pj.get("valu_insts_per_launch"):
             rate = pj["valu_insts_per_launch"] / (kernel_ms * 1e-3) / 1e9
