@@ -361,3 +361,22 @@ def test_torch_types_match_checker():
     s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
     assert s >= 0.999 and b >= 0.999
     sim.close()
+
+
+@pytest.mark.parametrize("kernel", [PLAIN, WAVE])
+def test_partial_photon_count_is_a_prefix(w1, kernel):
+    """a caller's n_photon below the gensteps' total simulates exactly the first n_photon
+    photons of the full launch (the per-photon genstep map is written only that far)"""
+    N = w1.num_photon
+    n = N - 3777
+    sim = QSim(w1.geometry, w1.physics, SEventConfig(seed=w1.seed, kernel=kernel))
+    gs = torch.from_numpy(w1.gensteps).to(DEV)
+    full_p = torch.zeros((N, 4, 4), dtype=torch.float32, device=DEV)
+    full_s = torch.zeros((N, 2, 2), dtype=torch.int64, device=DEV)
+    part_p = torch.zeros((n, 4, 4), dtype=torch.float32, device=DEV)
+    part_s = torch.zeros((n, 2, 2), dtype=torch.int64, device=DEV)
+    sim.simulate_device(gs, full_p, full_s, None, n_photon=N)
+    sim.simulate_device(gs, part_p, part_s, None, n_photon=n)
+    assert np.array_equal(part_p.cpu().numpy().view(np.uint32), full_p[:n].cpu().numpy().view(np.uint32))
+    assert np.array_equal(part_s.cpu().numpy(), full_s[:n].cpu().numpy())
+    sim.close()
