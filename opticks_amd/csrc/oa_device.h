@@ -762,7 +762,9 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, c
             if (typecode >= TC_NODE) {
                 Isect e;
                 OA_COUNT(OA_ST_LEAF);
+                OA_CYC_MARK(c_tleaf);
                 intersect_node<EXT>(e, nd, root, itra, plan, tmin, ro, rd);
+                OA_CYC_ADD(OA_ST_C_TLEAF, c_tleaf);
                 e.w = copysignf(e.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
                 if (sn >= D) { ierr = true; break; }
 #pragma unroll
