@@ -668,7 +668,13 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 cur = OA_BVH_EMPTY;
             } else {
                 OA_COUNT(OA_ST_TLAS);
+#ifdef OA_STATS
+                const int sp0 = sp;
+#endif
                 cur = visit<STRIDE>(P.tlas, cur, inv, oi, tmin, best_t, stk, sp);
+#ifdef OA_STATS
+                if (cur == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_TLAS_DEAD);
+#endif
             }
         }
         OA_CYC_ADD(OA_ST_C_P1, c_p1);
@@ -700,7 +706,13 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 }
                 if (!(bc & OA_BVH_LEAF_BIT)) {
                     OA_COUNT(OA_ST_BLAS);
+#ifdef OA_STATS
+                    const int sp0 = sp;
+#endif
                     bc = visit<STRIDE>(P.blas, bc, inv2, oi2, tmin, best_t, stk, sp);
+#ifdef OA_STATS
+                    if (bc == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_BLAS_DEAD);
+#endif
                     continue;
                 }
                 const uint32_t bf = bc & 0x00ffffffu, bn = (bc >> 24) & 0x7fu;
