@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--refill", type=int, default=1)
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--regroup", type=int, default=25,
+                    help="trace_regroup_pct of the 10M forward/adjoint legs (best of 0/10/25/50 there)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="photons for the CPU baseline (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -166,7 +168,7 @@ def main():
     G = len(w.gensteps)
     kernel = L.MI_KERNEL_WAVEFRONT if args.kernel == "wavefront" else L.MI_KERNEL_PLAIN
     cfg = SEventConfig(seed=w.seed, kernel=kernel, block_size=args.block, refill_period=args.refill,
-                       bvh_leaf_size=args.leaf, waves_per_cu=args.waves_per_cu)
+                       bvh_leaf_size=args.leaf, waves_per_cu=args.waves_per_cu, trace_regroup_pct=args.regroup)
     sim = QSim(w.geometry, w.physics, cfg)
     gs = torch.from_numpy(w.gensteps).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
@@ -221,7 +223,7 @@ def main():
         "config": {"workload": "csg-boolean-10M: 1000 torch gensteps x 10000 photons, LS sphere r5m + acrylic "
                                "+ water + 2000 instanced difference(union(zsphere,cylinder),cylinder) detectors",
                    "n_photon": N, "n_genstep": G, "kernel": args.kernel, "block": args.block,
-                   "refill_period": args.refill, "bvh_leaf": args.leaf,
+                   "refill_period": args.refill, "bvh_leaf": args.leaf, "trace_regroup_pct": args.regroup,
                    "outputs": "photon+seq+hit", "steps_per_photon": S, "hits": int(n_hit)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -377,7 +379,8 @@ def _config1_arrays():
 
 def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
     """64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons in
-    the 20,000-detector geometry: autotune (block, refill period, TLAS leaf) on
+    the 20,000-detector geometry: autotune (block, refill period, TLAS leaf,
+    trace regroup percent) on
     the first events, capture the whole batch into a hipGraph, time replays.
     History agreement: seq of the first events against the plain kernel run
     per event (one launch per event, one lane per photon)."""
@@ -430,7 +433,8 @@ def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
             "photons_per_s_wall": N / dt, "device_ms_per_step": mean_ms, "wall_ms_per_step": dt * 1e3,
             "hits": int(ho[-1]),
             "autotune": {"events": tune_events, "seconds": tune_s, "candidates": len(tune),
-                         "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size")},
+                         "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size",
+                                                         "trace_regroup_pct")},
                          "chosen_ms": best["ms"], "slowest_ms": max(r["ms"] for r in tune)},
             "seqhis_agreement_vs_plain_per_event": same / max(total, 1), "agreement_photons": total}
 

@@ -187,6 +187,7 @@ void mi_config_default(mi_config* c)
     c->refill_period = 1;
     c->bvh_leaf_size = 1;
     c->waves_per_cu = 0;
+    c->trace_regroup_pct = 0;
 }
 
 const char* mi_last_error(void) { return g_err.c_str(); }
@@ -197,6 +198,7 @@ static int check_config(const mi_config* c)
     if (c->block_size != 64 && c->block_size != 128 && c->block_size != 256)
         return fail(MI_ERR_ARG, "block_size must be 64, 128 or 256");
     if (c->bvh_leaf_size < 1 || c->bvh_leaf_size > 64) return fail(MI_ERR_ARG, "bvh_leaf_size must be 1..64");
+    if (c->trace_regroup_pct < 0 || c->trace_regroup_pct > 100) return fail(MI_ERR_ARG, "trace_regroup_pct must be 0..100");
     if (c->max_bounce < 0) return fail(MI_ERR_ARG, "max_bounce < 0");
     if (c->kernel != MI_KERNEL_PLAIN && c->kernel != MI_KERNEL_WAVEFRONT) return fail(MI_ERR_ARG, "unknown kernel");
     if (c->rng_kind == MI_RNG_PRECOOKED && (!c->precooked || c->precooked_ni <= 0 || c->precooked_nk <= 0))
@@ -534,6 +536,7 @@ static SimParams base_params(const mi_sim* sim)
     P.pre_ni = c.precooked_ni; P.pre_nk = c.precooked_nk;
     P.next = sim->next.p;
     P.refill_period = c.refill_period;
+    P.regroup_pct = c.trace_regroup_pct;
     return P;
 }
 
@@ -1408,6 +1411,7 @@ int mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids, cons
     static const int blocks[] = { 64, 128, 256 };
     static const int refills[] = { 1, 4, 16 };
     static const int leaves[] = { 1, 2, 4 };
+    static const int regroups[] = { 0, 25 };
     const mi_config base = sim->cfg;
     mi_config best = base;
     double best_ms = 1e300;
@@ -1416,27 +1420,31 @@ int mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids, cons
     for (int leaf : leaves) {
         for (int blk : blocks) {
             for (int rf : refills) {
-                mi_config c = base;
-                c.bvh_leaf_size = leaf; c.block_size = blk; c.refill_period = rf;
-                rc = mi_sim_set_config(sim, &c);
-                if (rc) break;
-                rc = mi_sim_simulate_batch_device(sim, n_event, event_ids, gs_offsets, d_gensteps, po.data(), d_photon,
-                                                  d_seq, d_hit, hit_capacity, ho.data(), nullptr);
-                if (rc) break;
-                double tot = 0.0;
-                for (int r = 0; r < reps && !rc; ++r) {
+                for (int rg : regroups) {
+                    mi_config c = base;
+                    c.bvh_leaf_size = leaf; c.block_size = blk; c.refill_period = rf; c.trace_regroup_pct = rg;
+                    rc = mi_sim_set_config(sim, &c);
+                    if (rc) break;
                     rc = mi_sim_simulate_batch_device(sim, n_event, event_ids, gs_offsets, d_gensteps, po.data(),
                                                       d_photon, d_seq, d_hit, hit_capacity, ho.data(), nullptr);
-                    tot += sim->last_ms[3];
+                    if (rc) break;
+                    double tot = 0.0;
+                    for (int r = 0; r < reps && !rc; ++r) {
+                        rc = mi_sim_simulate_batch_device(sim, n_event, event_ids, gs_offsets, d_gensteps, po.data(),
+                                                          d_photon, d_seq, d_hit, hit_capacity, ho.data(), nullptr);
+                        tot += sim->last_ms[3];
+                    }
+                    if (rc) break;
+                    const double ms = tot / reps;
+                    if (results && *n_results < max_results) {
+                        mi_tune_result& t = results[*n_results];
+                        t.block_size = blk; t.refill_period = rf; t.bvh_leaf_size = leaf; t.kernel = c.kernel;
+                        t.trace_regroup_pct = rg; t.ms = ms;
+                    }
+                    ++*n_results;
+                    if (ms < best_ms) { best_ms = ms; best = c; }
                 }
                 if (rc) break;
-                const double ms = tot / reps;
-                if (results && *n_results < max_results) {
-                    mi_tune_result& t = results[*n_results];
-                    t.block_size = blk; t.refill_period = rf; t.bvh_leaf_size = leaf; t.kernel = c.kernel; t.ms = ms;
-                }
-                ++*n_results;
-                if (ms < best_ms) { best_ms = ms; best = c; }
             }
             if (rc) break;
         }

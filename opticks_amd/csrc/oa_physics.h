@@ -654,7 +654,14 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     while (!done) {
         /* phase 1: walk nodes until this lane has a leaf or nothing left */
         OA_CYC_MARK(c_p1);
+        /* Once fewer than regroup_pct percent of the lanes that entered this walk
+           still walk, the whole wave leaves it: the lanes holding a leaf run the
+           leaf phase and the others resume from their own node and stack after it.
+           Each lane still visits the same nodes and leaves in the same order, so
+           only the wave's interleaving changes, never a result. 0 waits for all. */
+        const int walk_floor = P.regroup_pct > 0 ? __popcll(__ballot(1)) * P.regroup_pct : 0;
         while (!done && leaf == OA_BVH_EMPTY) {
+            if (__popcll(__ballot(1)) * 100 < walk_floor) break;
             OA_WAVE_COUNT(OA_ST_W_P1, OA_ST_L_P1);
 #ifdef OA_STATS
             n1++;

@@ -75,6 +75,7 @@ class SEventConfig:
     refill_period: int = 1
     bvh_leaf_size: int = 1
     waves_per_cu: int = 0
+    trace_regroup_pct: int = 0
 
     ENV = dict(max_bounce=("OPTICKS_MAX_BOUNCE", int), max_time=("OPTICKS_MAX_TIME", float),
                propagate_epsilon=("OPTICKS_PROPAGATE_EPSILON", float),
@@ -119,6 +120,7 @@ class SEventConfig:
         c.refill_period = self.refill_period
         c.bvh_leaf_size = self.bvh_leaf_size
         c.waves_per_cu = self.waves_per_cu
+        c.trace_regroup_pct = self.trace_regroup_pct
         return c
 
 
@@ -296,23 +298,25 @@ class QSim:
         return Batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t, hit_t, adjoint, grad_t)
 
     def autotune(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, reps=2):
-        """Pick block size, refill period and TLAS leaf size on these events; returns all candidates."""
+        """Pick block size, refill period, TLAS leaf size and trace regroup percent on these
+        events; returns all candidates."""
         ne = len(event_ids)
         ev = np.ascontiguousarray(event_ids, np.int32)
         go = np.ascontiguousarray(gs_offsets, np.int64)
-        res = (L.MiTuneResult * 64)()
+        res = (L.MiTuneResult * 128)()
         n = C.c_int32(0)
         cap = 0 if hit_t is None else hit_t.shape[0]
         L.check(L.lib().mi_sim_autotune(self.handle, ne, ev.ctypes.data, go.ctypes.data, gs_t.data_ptr(),
                                         photon_t.data_ptr(), None if seq_t is None else seq_t.data_ptr(),
-                                        None if hit_t is None else hit_t.data_ptr(), cap, reps, res, 64,
+                                        None if hit_t is None else hit_t.data_ptr(), cap, reps, res, 128,
                                         C.byref(n)), "mi_sim_autotune")
         cfg = L.MiConfig()
         L.check(L.lib().mi_sim_get_config(self.handle, C.byref(cfg)), "mi_sim_get_config")
         self.config.block_size, self.config.refill_period = cfg.block_size, cfg.refill_period
         self.config.bvh_leaf_size = cfg.bvh_leaf_size
+        self.config.trace_regroup_pct = cfg.trace_regroup_pct
         return [dict(block_size=r.block_size, refill_period=r.refill_period, bvh_leaf_size=r.bvh_leaf_size,
-                     ms=r.ms) for r in res[:min(n.value, 64)]]
+                     trace_regroup_pct=r.trace_regroup_pct, ms=r.ms) for r in res[:min(n.value, 128)]]
 
     def merge_hits_device(self, in_t, select_mask=None, time_window=1.0, out_t=None, stream=None):
         """SPM::merge_partial_select on the GPU. in_t: (n,4,4) f32 sphoton records or (n,4)

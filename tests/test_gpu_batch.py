@@ -112,12 +112,13 @@ def test_autotune_picks_a_candidate_and_keeps_results(w2):
     _, ho_ref, P, S, H = eager(sim, gs_np, offs, ev_ids, N)
     photon, seq, hit = buffers(N)
     res = sim.autotune(torch.from_numpy(gs_np).to(DEV), offs, ev_ids, photon, seq, hit, reps=1)
-    assert len(res) == 27
+    assert len(res) == 54
+    assert {r["trace_regroup_pct"] for r in res} == {0, 25}
     assert all(r["ms"] > 0 for r in res)
     best = min(res, key=lambda r: r["ms"])
     c = sim.config
-    assert (c.block_size, c.refill_period, c.bvh_leaf_size) == (best["block_size"], best["refill_period"],
-                                                                best["bvh_leaf_size"])
+    keys = ("block_size", "refill_period", "bvh_leaf_size", "trace_regroup_pct")
+    assert tuple(getattr(c, k) for k in keys) == tuple(best[k] for k in keys)
     _, ho, P2, S2, H2 = eager(sim, gs_np, offs, ev_ids, N)
     assert np.array_equal(ho, ho_ref)
     assert np.array_equal(P2.view(np.uint32), P.view(np.uint32))

@@ -113,20 +113,27 @@ def test_config1_full_size_properties_and_samples():
     sim.close()
 
 
-def test_plain_and_wavefront_identical_full_size():
+def test_plain_wavefront_and_regroup_identical_full_size():
+    """One lane per photon, the persistent wave kernel, and the wave kernel with its
+    node walk handing over to the leaf phase early (25% and 100% regroup) give
+    the same photons and histories bit for bit on the 10M workload."""
     w = configs.config1()
     N = w.num_photon
     gs = torch.from_numpy(w.gensteps).to(DEV)
-    out = []
-    for k in (PLAIN, WAVE):
-        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=k))
+    ref = None
+    for k, rg in ((PLAIN, 0), (WAVE, 0), (WAVE, 25), (WAVE, 100)):
+        print("kernel", k, "trace_regroup_pct", rg, flush=True)
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=k, trace_regroup_pct=rg))
         photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
         seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
         sim.simulate_device(gs, photon, seq, None, n_photon=N)
-        out.append((photon, seq))
         sim.close()
-    assert torch.equal(out[0][1], out[1][1])
-    assert torch.equal(out[0][0].view(torch.int32), out[1][0].view(torch.int32))
+        if ref is None:
+            ref = (photon, seq)
+            continue
+        assert torch.equal(ref[1], seq)
+        assert torch.equal(ref[0].view(torch.int32), photon.view(torch.int32))
+        del photon, seq
 
 
 def test_trace_matches_checker_brute_force():
