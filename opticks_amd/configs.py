@@ -364,6 +364,34 @@ def leaf_zoo(num_photon=200_000, seed=42, spacing=400.0, nodes=None, name="leaf_
     return Workload(name, fd.arrays(), phys, gs, seed=seed, meta=dict(n_leaf_prims=len(nodes)))
 
 
+def boolean_zoo_nodes():
+    """One prim per entry: Boolean trees of height 1 (each operator), every shape of
+    a height-2 tree (operators on both sides, a leaf on the right, a leaf on the
+    left, nested differences) and one tree of height 3."""
+    from .foundry import Node, CSG_DIFFERENCE, CSG_INTERSECTION, CSG_UNION
+    U, I, D = CSG_UNION, CSG_INTERSECTION, CSG_DIFFERENCE
+    op = Node.op
+    return [
+        op(D, Node.sphere(150.0), Node.sphere(100.0)),
+        op(I, Node.box3(240.0, 240.0, 240.0), Node.sphere(150.0)),
+        op(U, Node.sphere(100.0, (-50.0, 0.0, 0.0)), Node.sphere(100.0, (50.0, 0.0, 0.0))),
+        op(D, op(U, Node.box3(200.0, 200.0, 200.0), Node.sphere(100.0, (0.0, 0.0, 80.0))),
+           op(I, Node.cylinder(60.0, -150.0, 150.0), Node.sphere(130.0))),
+        op(D, op(U, Node.zsphere(140.0, -40.0, 140.0), Node.cylinder(70.0, -180.0, -30.0)),
+           Node.cylinder(60.0, -190.0, -50.0)),
+        op(I, Node.sphere(160.0), op(D, Node.box3(260.0, 260.0, 260.0), Node.cylinder(70.0, -200.0, 200.0))),
+        op(U, op(D, Node.sphere(150.0), Node.box3(150.0, 150.0, 150.0)), Node.sphere(60.0)),
+        op(D, op(D, Node.box3(250.0, 250.0, 250.0), Node.sphere(150.0)), Node.cylinder(50.0, -200.0, 200.0)),
+        op(U, op(D, op(I, Node.box3(200.0, 200.0, 200.0), Node.sphere(130.0)), Node.cylinder(40.0, -150.0, 150.0)),
+           Node.sphere(50.0)),
+    ]
+
+
+def boolean_zoo(num_photon=200_000, seed=42, spacing=400.0):
+    """leaf_zoo world with the boolean_zoo_nodes solids"""
+    return leaf_zoo(num_photon, seed, spacing, nodes=boolean_zoo_nodes(), name="boolean_zoo")
+
+
 def list_zoo(num_photon=200_000, seed=42, spacing=400.0):
     """leaf_zoo world with the list_zoo_nodes solids"""
     return leaf_zoo(num_photon, seed, spacing, nodes=list_zoo_nodes(), name="list_zoo")

@@ -565,6 +565,35 @@ OA_D int classify(const Isect& e, v3 dir, float tmin)
     return fabsf(e.w) > tmin ? (e.x * dir.x + e.y * dir.y + e.z * dir.z < 0.f ? ST_ENTER : ST_EXIT) : ST_MISS;
 }
 
+/* one operator node's decision from its left and right child results: a
+   return (miss, A, B, flipped B) or a loop (advance A or B and re-evaluate) */
+OA_D int csg_decide(uint32_t typecode, const Isect& L, const Isect& R, v3 rd, float tmin)
+{
+    int l_state = classify(L, rd, tmin);
+    int r_state = classify(R, rd, tmin);
+    const float t_left = fabsf(L.w);
+    const float t_right = fabsf(R.w);
+    bool leftIsCloser = t_left <= t_right;
+    const bool l_promote = l_state == ST_MISS && (sgnbit(L.x) || sgnbit(L.y));
+    const bool r_promote = r_state == ST_MISS && (sgnbit(R.x) || sgnbit(R.y));
+    if (r_promote) { r_state = ST_EXIT; leftIsCloser = true; }
+    if (l_promote) { l_state = ST_EXIT; leftIsCloser = false; }
+    const uint32_t off = 3u * (uint32_t)l_state + (uint32_t)r_state;
+    const uint32_t word = (typecode >= TC_UNION && typecode <= TC_DIFFERENCE)
+                        ? (leftIsCloser ? kLutA[typecode - 1u] : kLutB[typecode - 1u]) : 0u;
+    return off < 8u ? (int)((word >> (4u * off)) & 0xfu) : CT_MISS;
+}
+
+/* the intersect a return decision passes up (w keeps its magnitude) */
+OA_D Isect csg_result(int ctrl, const Isect& L, const Isect& R)
+{
+    Isect res;
+    if (ctrl == CT_MISS) { res.x = 0.f; res.y = 0.f; res.z = 0.f; res.w = 0.f; }
+    else res = pick(ctrl == CT_RETURN_A, L, R);
+    if (ctrl == CT_RETURN_FLIP_B) { res.x = -res.x; res.y = -res.y; res.z = -res.z; }
+    return res;
+}
+
 OA_D uint32_t tdepth(uint32_t i) { return 31u - (uint32_t)__builtin_clz(i); }
 OA_D uint32_t postorder_next(uint32_t i, uint32_t elev) { return (i & 1u) ? i >> 1 : (i << elev) + (1u << elev); }
 
@@ -778,28 +807,13 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, c
                 if (!(firstLeft ^ secondLeft)) { ierr = true; break; }
                 const Isect L = pick(firstLeft, st[0], st[1]);
                 const Isect R = pick(firstLeft, st[1], st[0]);
-                int l_state = classify(L, rd, tmin);
-                int r_state = classify(R, rd, tmin);
-                const float t_left = fabsf(L.w);
-                const float t_right = fabsf(R.w);
-                bool leftIsCloser = t_left <= t_right;
-                const bool l_promote = l_state == ST_MISS && (sgnbit(L.x) || sgnbit(L.y));
-                const bool r_promote = r_state == ST_MISS && (sgnbit(R.x) || sgnbit(R.y));
-                if (r_promote) { r_state = ST_EXIT; leftIsCloser = true; }
-                if (l_promote) { l_state = ST_EXIT; leftIsCloser = false; }
-                const uint32_t off = 3u * (uint32_t)l_state + (uint32_t)r_state;
-                const uint32_t word = (typecode >= TC_UNION && typecode <= TC_DIFFERENCE)
-                                    ? (leftIsCloser ? kLutA[typecode - 1u] : kLutB[typecode - 1u]) : 0u;
-                const int ctrl = off < 8u ? (int)((word >> (4u * off)) & 0xfu) : CT_MISS;
+                const int ctrl = csg_decide(typecode, L, R, rd, tmin);
                 /* pop two, push one: st[0] = result, entries below move up by one */
 #pragma unroll
                 for (int i = 1; i < D - 1; ++i) st[i] = st[i + 1];
                 sn--;
                 if (ctrl < CT_LOOP_A) {
-                    Isect res;
-                    if (ctrl == CT_MISS) { res.x = 0.f; res.y = 0.f; res.z = 0.f; res.w = 0.f; }
-                    else res = pick(ctrl == CT_RETURN_A, L, R);
-                    if (ctrl == CT_RETURN_FLIP_B) { res.x = -res.x; res.y = -res.y; res.z = -res.z; }
+                    Isect res = csg_result(ctrl, L, R);
                     res.w = copysignf(res.w, nodeIdx % 2u == 0u ? -1.f : 1.f);
                     st[0] = res;
                 } else {
@@ -829,6 +843,69 @@ OA_D bool intersect_tree_t(Isect& out, const float4* root, const float4* itra, c
     return out.w > 0.f;
 }
 
+/*
+ * Trees of 3 or 7 nodes whose evaluation never loops. Each child of the root is
+ * a leaf or an operator over two leaves, so the postorder algorithm above,
+ * when no node decides to loop, evaluates every leaf once at t_min and
+ * combines bottom-up in one tranche: this does the same in straight-line
+ * code, with one leaf call site and no stacks. The sign that the general
+ * algorithm puts on w to tell left from right is not needed here (every use
+ * of w takes its magnitude) and the root's result carries |w| in both.
+ * Returns -1 for any other shape or at the first loop decision: the caller
+ * then runs the general algorithm from the start.
+ */
+template <bool EXT>
+OA_D int intersect_tree_small(Isect& out, const float4* root, const float4* itra, const float4* plan, float t_min,
+                              v3 ro, v3 rd)
+{
+    const uint32_t numNode = __float_as_uint(root[0].x);
+    const uint32_t tc1 = __float_as_uint(root[3].z);
+    if ((numNode != 3u && numNode != 7u) || tc1 < TC_UNION || tc1 > TC_DIFFERENCE) return -1;
+    Isect sub[2];
+    sub[0].x = sub[0].y = sub[0].z = sub[0].w = 0.f;
+    sub[1] = sub[0];
+    bool general = false;
+#pragma unroll 1
+    for (uint32_t s = 2u; s <= 3u; ++s) {
+        const uint32_t tcs = __float_as_uint(root[4u * (s - 1u) + 3u].z);
+        const bool sub_leaf = tcs >= TC_NODE;
+        if (sub_leaf) {
+            /* a leaf child of a 7-node root stands over two zero nodes */
+            if (numNode == 7u && (__float_as_uint(root[4u * (2u * s - 1u) + 3u].z) != TC_ZERO ||
+                                  __float_as_uint(root[4u * (2u * s) + 3u].z) != TC_ZERO)) { general = true; break; }
+        } else if (numNode == 3u || tcs < TC_UNION || tcs > TC_DIFFERENCE) { general = true; break; }
+        Isect x[2];
+        x[0].x = x[0].y = x[0].z = x[0].w = 0.f;
+        x[1] = x[0];
+#pragma unroll 1
+        for (uint32_t j = 0u; j < (sub_leaf ? 1u : 2u); ++j) {
+            const uint32_t k = sub_leaf ? s : 2u * s + j;
+            const float4* nd = root + 4u * (k - 1u);
+            if (__float_as_uint(nd[3].z) < TC_NODE) { general = true; break; }
+            Isect e;
+            OA_COUNT(OA_ST_LEAF);
+            intersect_node<EXT>(e, nd, root, itra, plan, t_min, ro, rd);
+            x[1] = pick(j == 0u, x[1], e);
+            x[0] = pick(j == 0u, e, x[0]);
+        }
+        if (general) break;
+        Isect r = x[0];
+        if (!sub_leaf) {
+            const int ctrl = csg_decide(tcs, x[0], x[1], rd, t_min);
+            if (ctrl >= CT_LOOP_A) { general = true; break; }
+            r = csg_result(ctrl, x[0], x[1]);
+        }
+        sub[1] = pick(s == 2u, sub[1], r);
+        sub[0] = pick(s == 2u, r, sub[0]);
+    }
+    if (general) return -1;
+    const int ctrl = csg_decide(tc1, sub[0], sub[1], rd, t_min);
+    if (ctrl >= CT_LOOP_A) return -1;
+    out = csg_result(ctrl, sub[0], sub[1]);
+    out.w = fabsf(out.w);
+    return out.w > 0.f ? 1 : 0;
+}
+
 /* deep trees (height 4..OA_CSG_MAX_HEIGHT): out of line, result returned by value */
 template <bool EXT>
 __device__ static __attribute__((noinline)) float4 intersect_tree_deep(const float4* root, const float4* itra,
@@ -851,18 +928,21 @@ OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, cons
     if (typecode < TC_NODE) {
         OA_COUNT(OA_ST_TREE);
         OA_WAVE_COUNT(OA_ST_W_TREE, OA_ST_L_TREE);
-#ifdef OA_STATS
-        if (__float_as_uint(root[0].x) <= 15u) {
-            OA_CYC_MARK(c_tree);
-            const bool h = intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd);
-            OA_CYC_ADD(OA_ST_C_TREE, c_tree);
-            return h;
+        OA_CYC_MARK(c_tree);
+        const uint32_t numNode = __float_as_uint(root[0].x);
+        int h = numNode <= 7u ? intersect_tree_small<EXT>(is, root, itra, plan, t_min, ro, rd) : -1;
+        if (h < 0) {
+            OA_COUNT(OA_ST_TREE_GENERAL);
+            if (numNode <= 15u) {
+                h = intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd) ? 1 : 0;
+            } else {
+                const float4 r = intersect_tree_deep<EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
+                is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
+                h = is.w > 0.f ? 1 : 0;
+            }
         }
-#endif
-        if (__float_as_uint(root[0].x) <= 15u) return intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd);
-        const float4 r = intersect_tree_deep<EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
-        is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
-        return is.w > 0.f;
+        OA_CYC_ADD(OA_ST_C_TREE, c_tree);
+        return h != 0;
     }
     if constexpr (EXT) return intersect_node<true>(is, root, root, itra, plan, t_min, ro, rd);
     return false;

@@ -43,7 +43,9 @@ enum { OA_ST_TRACE, OA_ST_TLAS, OA_ST_INST, OA_ST_BLAS, OA_ST_PRIM, OA_ST_TREE, 
        /* cycles in the leaf intersections of tree evaluations */
        OA_ST_C_TLEAF,
        /* node visits whose children all miss or lie beyond the nearest hit */
-       OA_ST_TLAS_DEAD, OA_ST_BLAS_DEAD, OA_ST_N };
+       OA_ST_TLAS_DEAD, OA_ST_BLAS_DEAD,
+       /* tree evaluations run by the general postorder algorithm */
+       OA_ST_TREE_GENERAL, OA_ST_N };
 
 struct BvhNode {
     float lox[4], loy[4], loz[4];

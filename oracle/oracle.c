@@ -752,10 +752,29 @@ static int intersect_node(int* valid, f4* isect, const float* nd, const float* r
     }
 }
 
+/* tree evaluations and loop decisions since the last oracle_tree_counts(reset);
+   counted only once a reset has switched counting on (the shared counters would
+   otherwise slow the multi-threaded runs that time the checker) */
+static int g_tree_counting;
+static int64_t g_tree_evals, g_tree_loops;
+
+void oracle_tree_counts(int64_t* evals, int64_t* loops, int32_t reset)
+{
+    *evals = __atomic_load_n(&g_tree_evals, __ATOMIC_RELAXED);
+    *loops = __atomic_load_n(&g_tree_loops, __ATOMIC_RELAXED);
+    if (reset) {
+        __atomic_store_n(&g_tree_evals, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&g_tree_loops, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&g_tree_counting, 1, __ATOMIC_RELAXED);
+    }
+}
+
 /* intersect_tree (CSG/csg_intersect_tree.h:277-672) */
 static int intersect_tree(int* valid, f4* isect, const float* node, const float* itra, const float* plan, float t_min,
                           f3 ro, f3 rd)
 {
+    const int counting = __atomic_load_n(&g_tree_counting, __ATOMIC_RELAXED);
+    if (counting) __atomic_fetch_add(&g_tree_evals, 1, __ATOMIC_RELAXED);
     int numNode = (int)nd_subNum(node);
     unsigned height = tree_height((unsigned)numNode);
     const float propagate_epsilon = 0.0001f;
@@ -816,6 +835,7 @@ static int intersect_tree(int* valid, f4* isect, const float* node, const float*
                     if (csg_curr >= CSG_STACK_SIZE - 1) { ierr = ERR_OVERFLOW; break; }
                     csg_curr++; csg[csg_curr] = result;
                 } else {
+                    if (counting) __atomic_fetch_add(&g_tree_loops, 1, __ATOMIC_RELAXED);
                     int loopside = ctrl == A_LOOP_A ? left : right;
                     int otherside = ctrl == A_LOOP_A ? right : left;
                     unsigned leftIdx = 2 * nodeIdx;

@@ -132,6 +132,10 @@ int oracle_intersect_prim(const oracle_geom* g, int32_t prim_idx, float t_min,
 /* The packed Boolean LUT words, for comparison with the reference constants. */
 void oracle_packed_lut(uint32_t* acloser3, uint32_t* bcloser3);
 
+/* CSG tree evaluations and loop decisions taken since the last reset (test statistics;
+   the first reset switches counting on for the rest of the process). */
+void oracle_tree_counts(int64_t* evals, int64_t* loops, int32_t reset);
+
 /* Portable math entry points for the accuracy tests. */
 float oracle_logf(float x);
 void  oracle_logf_n(const float* x, float* y, int64_t n);

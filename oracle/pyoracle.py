@@ -89,6 +89,7 @@ def lib():
         L.oracle_intersect_prim.argtypes = [C.POINTER(OracleGeom), C.c_int32, C.c_float,
                                             C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_packed_lut.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_tree_counts.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int32]
         L.oracle_logf.restype = C.c_float
         L.oracle_logf.argtypes = [C.c_float]
         L.oracle_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -256,6 +257,14 @@ def intersect_prim(geometry, prim_idx, t_min, origin, direction):
     if rc < 0:
         raise RuntimeError("oracle_intersect_prim failed")
     return bool(rc), out
+
+
+def tree_counts(reset=True):
+    """(CSG tree evaluations, loop decisions) of the checker since the last reset; the
+    first reset switches counting on."""
+    e, l = C.c_int64(0), C.c_int64(0)
+    lib().oracle_tree_counts(C.byref(e), C.byref(l), 1 if reset else 0)
+    return e.value, l.value
 
 
 def packed_lut():

@@ -14,10 +14,14 @@ Writes the golden fixtures of tests/golden from the reference checkout:
                   CSG_DISCONTIGUOUS and CSG_OVERLAP list nodes at the prim root and
                   inside Boolean trees), intersected by the same reference code built
                   with WITH_CONTIGUOUS as the reference builds its CSG library.
+  csg_golden_booleans.npz  rays against the Boolean-zoo prims (trees of height 1,
+                  every shape of height 2, and height 3; many of these rays make
+                  the tree evaluation loop), intersected by the same reference code.
   csg_lut.json    the packed Boolean LUT words parsed from CSG/csg_classify.h.
 
-Run in the container that holds the reference (python tests/golden/make_golden.py);
-the outputs are data only and are what the CPU tests read.
+Run in the container that holds the reference (python tests/golden/make_golden.py
+[name ...], default all of lut csg leaves lists booleans); the outputs are data only
+and are what the CPU tests read.
 """
 import json
 import os
@@ -85,6 +89,11 @@ def lists():
     return zoo_vectors(configs.list_zoo(num_photon=4).geometry, 20261204)
 
 
+def booleans():
+    from opticks_amd import configs
+    return zoo_vectors(configs.boolean_zoo(num_photon=4).geometry, 20261205)
+
+
 def zoo_vectors(g, seed):
     import pyoracle as po
     po.build()
@@ -113,9 +122,13 @@ def zoo_vectors(g, seed):
 
 
 if __name__ == "__main__":
-    with open(os.path.join(HERE, "csg_lut.json"), "w") as f:
-        json.dump(lut(), f, indent=1)
-    np.savez_compressed(os.path.join(HERE, "csg_golden.npz"), **csg())
-    np.savez_compressed(os.path.join(HERE, "csg_golden_leaves.npz"), **leaves())
-    np.savez_compressed(os.path.join(HERE, "csg_golden_lists.npz"), **lists())
-    print("wrote", os.listdir(HERE))
+    which = sys.argv[1:] or ["lut", "csg", "leaves", "lists", "booleans"]
+    if "lut" in which:
+        with open(os.path.join(HERE, "csg_lut.json"), "w") as f:
+            json.dump(lut(), f, indent=1)
+    files = dict(csg="csg_golden.npz", leaves="csg_golden_leaves.npz", lists="csg_golden_lists.npz",
+                 booleans="csg_golden_booleans.npz")
+    for name, fn in files.items():
+        if name in which:
+            np.savez_compressed(os.path.join(HERE, fn), **globals()[name]())
+    print("wrote", which)

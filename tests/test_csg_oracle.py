@@ -263,3 +263,40 @@ def test_list_closed_forms(lzoo):
     # bead chain with a bore (difference with cylinder r 25 along z): along x at y = 0 the bore splits it
     v, i = po.intersect_prim(lzoo, 4, 0.0, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0))
     assert v and abs(i[3] - 25.0) < 1e-3
+
+
+@pytest.fixture(scope="module")
+def bzoo():
+    return configs.boolean_zoo(num_photon=4).geometry
+
+
+def test_boolean_golden_vectors_bitwise(bzoo):
+    """Boolean trees of height 1 (each operator), every height-2 shape and height 3:
+    oracle == reference CSG code on committed vectors, many of which loop"""
+    z = np.load(os.path.join(GOLD, "csg_golden_booleans.npz"))
+    po.tree_counts(reset=True)
+    bad = []
+    for i in range(len(z["prim"])):
+        v, isect = po.intersect_prim(bzoo, int(z["prim"][i]), z["tmin"][i], z["origin"][i], z["direction"][i])
+        if int(v) != int(z["valid"][i]) or not np.array_equal(isect.view(np.uint32), z["isect"][i].view(np.uint32)):
+            bad.append(i)
+    evals, loops = po.tree_counts(reset=True)
+    assert not bad, (len(bad), bad[:5])
+    assert evals == len(z["prim"]) and loops > 500        # the vectors take the loop decisions
+    for p in range(1, 10):
+        assert z["valid"][z["prim"] == p].sum() > 50
+
+
+@pytest.mark.skipif(po.reflib() is None, reason="oracle/_ref not built (reference checkout absent)")
+def test_boolean_random_rays_vs_reference_build(bzoo):
+    rng = np.random.default_rng(10)
+    bad = 0
+    for k in range(6000):
+        prim = 1 + k % 9
+        o = rng.uniform(-250, 250, 3).astype(np.float32)
+        d = rng.normal(size=3); d = (d / np.linalg.norm(d)).astype(np.float32)
+        t = np.float32([0.0, 0.05, 2.0][k % 3])
+        v1, i1 = po.intersect_prim(bzoo, prim, t, o, d)
+        v2, i2 = po.ref_intersect_prim(bzoo, prim, t, o, d)
+        bad += (v1 != v2) or not np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
+    assert bad == 0

@@ -316,6 +316,41 @@ def test_simtrace_matches_checker(which):
     sim.close()
 
 
+def test_boolean_zoo_trace_and_simulate_match_checker():
+    """Boolean trees of height 1, every height-2 shape and height 3, with ray origins
+    inside and outside the solids so that many evaluations loop: nearest hits bit
+    for bit against the checker's brute force (the straight-line evaluation of
+    small trees and its fall-back to the general algorithm), then full propagation."""
+    w = configs.boolean_zoo(num_photon=200_000)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    rng = np.random.default_rng(12)
+    n = 60000
+    rays = np.zeros((n, 2, 4), np.float32)
+    rays[:, 0, :3] = rng.uniform(-600, 600, (n, 3))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 1, :3] = d
+    rays[:, 0, 3] = rng.choice([0.0, 0.05, 1.0], n)
+    rays[:, 1, 3] = 1e6
+    rt = torch.from_numpy(rays).to(DEV)
+    it = torch.zeros((n, 2, 4), dtype=torch.float32, device=DEV)
+    sim.trace_device(rt, it)
+    g = it.cpu().numpy()
+    W = po.World(w.geometry, w.physics)
+    po.tree_counts(reset=True)
+    r = W.trace(rays, traverse=0)
+    evals, loops = po.tree_counts(reset=True)
+    assert loops > 0.1 * evals
+    assert np.array_equal(g[:, 0].view(np.uint32), r[:, 0].view(np.uint32))
+    assert np.array_equal(g[:, 1, :3].view(np.uint32), r[:, 1, :3].view(np.uint32))
+    inst = g[:, 1, 0].view(np.uint32) >> 16
+    assert len(np.unique(inst[g[:, 1, 1].view(np.uint32) != 0xFFFF])) == 10
+    hits = sim.simulate(w.gensteps, 0)
+    ref_p, ref_s = W.simulate(w.gensteps, seed=w.seed)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s >= 0.999 and len(hits) > 1000
+    sim.close()
+
+
 def test_list_zoo_trace_and_simulate_match_checker():
     """CSG_CONTIGUOUS / DISCONTIGUOUS / OVERLAP list nodes at prim roots and inside trees:
     nearest hits against the checker's brute force, then full propagation"""
