@@ -666,14 +666,14 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
             n1++;
 #endif
+            /* one iteration: pop when the lane holds nothing, visit a node, and
+               take a leaf (popped or the visit's nearest child) at once */
             if (cur == OA_BVH_EMPTY) {
                 if (sp == 0) { done = true; break; }
                 sp--;
                 cur = stk[sp * STRIDE];
-            } else if (cur & OA_BVH_LEAF_BIT) {
-                leaf = cur;
-                cur = OA_BVH_EMPTY;
-            } else {
+            }
+            if (!(cur & OA_BVH_LEAF_BIT)) {
                 OA_COUNT(OA_ST_TLAS);
 #ifdef OA_STATS
                 const int sp0 = sp;
@@ -682,6 +682,10 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
                 if (cur == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_TLAS_DEAD);
 #endif
+            }
+            if (cur != OA_BVH_EMPTY && (cur & OA_BVH_LEAF_BIT)) {
+                leaf = cur;
+                cur = OA_BVH_EMPTY;
             }
         }
         OA_CYC_ADD(OA_ST_C_P1, c_p1);
@@ -705,11 +709,11 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
                 n2++;
 #endif
+                /* as in the node walk: pop, visit, and test a leaf's prims in one iteration */
                 if (bc == OA_BVH_EMPTY) {
                     if (sp == sp_base) break;
                     sp--;
                     bc = stk[sp * STRIDE];
-                    continue;
                 }
                 if (!(bc & OA_BVH_LEAF_BIT)) {
                     OA_COUNT(OA_ST_BLAS);
@@ -720,7 +724,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
                     if (bc == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_BLAS_DEAD);
 #endif
-                    continue;
+                    if (bc == OA_BVH_EMPTY || !(bc & OA_BVH_LEAF_BIT)) continue;
                 }
                 const uint32_t bf = bc & 0x00ffffffu, bn = (bc >> 24) & 0x7fu;
                 OA_CYC_MARK(c_prim);
