@@ -97,7 +97,7 @@ def profile_json(name):
         return json.load(f)
 
 
-def bench_adjoint(torch, dev, steps):
+def bench_adjoint(torch, dev, steps, regroup=0):
     """Forward then path-replay adjoint on the binned-length workload: config1
     geometry and photons, LS and water absorption/Rayleigh lengths as 4 x 64
     bins (x(1+U(-0.2,0.2)), seed 42); gradient (2000 detectors, 256) f32."""
@@ -108,7 +108,7 @@ def bench_adjoint(torch, dev, steps):
     params = w.meta["params"]
     N = w.num_photon
     n_det = w.meta["num_det"]
-    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, trace_regroup_pct=regroup))
     gs = torch.from_numpy(w.gensteps).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
@@ -126,7 +126,7 @@ def bench_adjoint(torch, dev, steps):
     f, a = float(np.mean(fwd)), float(np.mean(adj))
     return {"workload": "csg-boolean-10M with LS/water absorption+Rayleigh lengths as 4x64 bins "
                         "(256 parameters), 2000 detectors",
-            "n_photon": int(N), "hits": int(n_hit), "n_param": int(params.n_param), "n_det": int(n_det),
+            "trace_regroup_pct": regroup, "n_photon": int(N), "hits": int(n_hit), "n_param": int(params.n_param), "n_det": int(n_det),
             "forward_ms": f, "adjoint_ms": a, "replay_ms": float(np.mean(rep)), "reduce_ms": float(np.mean(red)),
             "adjoint_forward_ratio": a / f}
 
@@ -237,7 +237,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if not args.no_adjoint:
-        a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)))
+        a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)), regroup=args.regroup)
         line["adjoint"] = a
         line["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
     if not args.no_cpu:

@@ -874,10 +874,15 @@ __global__ void __launch_bounds__(256) k_det_reduce(const float* S, const uint32
 
 namespace oa_launch {
 
+/* threads per block of the adjoint replay (64, 128 and 256 measure within 1%) */
+#ifndef OA_ADJ_BLOCK
+#define OA_ADJ_BLOCK 128
+#endif
+
 int adjoint_blocks_per_cu()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint_wave<128, false>, 128, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adjoint_wave<OA_ADJ_BLOCK, false>, OA_ADJ_BLOCK, 0) != hipSuccess) return 0;
     return nb;
 }
 
@@ -892,10 +897,10 @@ hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_
                        d_hit_off, n_ev, hit_cap);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, P.next, (int64_t)1);
     if (P.ext_leaves)
-        hipLaunchKernelGGL((k_adjoint_wave<128, true>), dim3((unsigned)std::max(1, grid)), dim3(128), 0, s, P, A, d_hit,
+        hipLaunchKernelGGL((k_adjoint_wave<OA_ADJ_BLOCK, true>), dim3((unsigned)std::max(1, grid)), dim3(OA_ADJ_BLOCK), 0, s, P, A, d_hit,
                            begin, n_max, d_hit_off, n_ev, hit_cap, d_S);
     else
-        hipLaunchKernelGGL((k_adjoint_wave<128, false>), dim3((unsigned)std::max(1, grid)), dim3(128), 0, s, P, A, d_hit,
+        hipLaunchKernelGGL((k_adjoint_wave<OA_ADJ_BLOCK, false>), dim3((unsigned)std::max(1, grid)), dim3(OA_ADJ_BLOCK), 0, s, P, A, d_hit,
                            begin, n_max, d_hit_off, n_ev, hit_cap, d_S);
     return hipGetLastError();
 }
