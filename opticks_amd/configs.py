@@ -436,6 +436,22 @@ def config2(num_events=64, mean_gs=200, mean_photons=5000, seed=42):
 CONFIG3_BASE_LENGTHS = {"LS": (25000.0, 40000.0), "Water": (20000.0, 30000.0)}
 
 
+def detector_shell(num_det=50_000, num_gs=8, per_gs=2500, seed=3):
+    """The config1 layout at the instance count of a large detector: num_det
+    detectors on a Fibonacci lattice at the radius that keeps config2's area per
+    detector (about 0.236 m^2; 30.6 m for 50,000), LS sphere 28 m, world half-side
+    32 m, and num_gs isotropic torch points in the LS (numpy default_rng(seed))."""
+    det_r = float(np.sqrt(num_det * 0.2365e6 / (4.0 * np.pi)))
+    ls_r, world_half = det_r - 2600.0, det_r + 1400.0
+    geo, phys = _scint_world(num_det, det_r, ls_r, 120.0, world_half)
+    rng = np.random.default_rng(seed)
+    pos = uniform_ball(rng, num_gs, ls_r)
+    gs = np.stack([torch_genstep(per_gs, T_POINT_ISO, pos=p, wavelength=200.0, weight=600.0)
+                   for p in pos]).astype(np.float32)
+    return Workload("detector_shell", geo, phys, gs, seed=seed,
+                    meta=dict(ls_r=ls_r, det_r=det_r, num_det=num_det, world_half=world_half))
+
+
 def config3(num_gs=1000, per_gs=10_000, seed=42, n_bin=64):
     """config1 geometry and photons; LS and water absorption and Rayleigh lengths
     as n_bin bins over 200-800 nm (theta, mm), each the config1 value times

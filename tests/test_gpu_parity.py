@@ -351,6 +351,20 @@ def test_boolean_zoo_trace_and_simulate_match_checker():
     sim.close()
 
 
+def test_fifty_thousand_detectors_match_checker():
+    """50,000 instanced CSG detectors (the deepest TLAS of the suite): the
+    traversal stack bound holds at create, and 20,000 photons give the checker's
+    photons, histories and hits bit for bit."""
+    w = configs.detector_shell(num_det=50_000)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    hits = sim.simulate(w.gensteps, 0)
+    ref_p, ref_s = po.World(w.geometry, w.physics).simulate(w.gensteps, seed=w.seed)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s == 1.0 and b == 1.0
+    assert len(hits) > 1000 and np.array_equal(hits.view(np.uint32), stable_hits(ref_p).view(np.uint32))
+    sim.close()
+
+
 def test_list_zoo_trace_and_simulate_match_checker():
     """CSG_CONTIGUOUS / DISCONTIGUOUS / OVERLAP list nodes at prim roots and inside trees:
     nearest hits against the checker's brute force, then full propagation"""
