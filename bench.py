@@ -97,7 +97,7 @@ def profile_json(name):
         return json.load(f)
 
 
-def bench_adjoint(torch, dev, steps, regroup=0):
+def bench_adjoint(torch, dev, steps, regroup=0, batch=1):
     """Forward then path-replay adjoint on the binned-length workload: config1
     geometry and photons, LS and water absorption/Rayleigh lengths as 4 x 64
     bins (x(1+U(-0.2,0.2)), seed 42); gradient (2000 detectors, 256) f32."""
@@ -108,7 +108,7 @@ def bench_adjoint(torch, dev, steps, regroup=0):
     params = w.meta["params"]
     N = w.num_photon
     n_det = w.meta["num_det"]
-    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, trace_regroup_pct=regroup))
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, trace_regroup_pct=regroup, refill_batch=batch))
     gs = torch.from_numpy(w.gensteps).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
@@ -126,7 +126,7 @@ def bench_adjoint(torch, dev, steps, regroup=0):
     f, a = float(np.mean(fwd)), float(np.mean(adj))
     return {"workload": "csg-boolean-10M with LS/water absorption+Rayleigh lengths as 4x64 bins "
                         "(256 parameters), 2000 detectors",
-            "trace_regroup_pct": regroup, "n_photon": int(N), "hits": int(n_hit), "n_param": int(params.n_param), "n_det": int(n_det),
+            "trace_regroup_pct": regroup, "refill_batch": batch, "n_photon": int(N), "hits": int(n_hit), "n_param": int(params.n_param), "n_det": int(n_det),
             "forward_ms": f, "adjoint_ms": a, "replay_ms": float(np.mean(rep)), "reduce_ms": float(np.mean(red)),
             "adjoint_forward_ratio": a / f}
 
@@ -142,7 +142,9 @@ def main():
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--regroup", type=int, default=25,
-                    help="trace_regroup_pct of the 10M forward/adjoint legs (best of 0/10/25/50 there)")
+                    help="trace_regroup_pct of the 10M forward/adjoint legs (best of 0-60 there)")
+    ap.add_argument("--batch", type=int, default=32,
+                    help="refill_batch of the 10M forward/adjoint legs (best of 1-64 there)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="photons for the CPU baseline (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -168,7 +170,8 @@ def main():
     G = len(w.gensteps)
     kernel = L.MI_KERNEL_WAVEFRONT if args.kernel == "wavefront" else L.MI_KERNEL_PLAIN
     cfg = SEventConfig(seed=w.seed, kernel=kernel, block_size=args.block, refill_period=args.refill,
-                       bvh_leaf_size=args.leaf, waves_per_cu=args.waves_per_cu, trace_regroup_pct=args.regroup)
+                       bvh_leaf_size=args.leaf, waves_per_cu=args.waves_per_cu, trace_regroup_pct=args.regroup,
+                       refill_batch=args.batch)
     sim = QSim(w.geometry, w.physics, cfg)
     gs = torch.from_numpy(w.gensteps).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
@@ -224,6 +227,7 @@ def main():
                                "+ water + 2000 instanced difference(union(zsphere,cylinder),cylinder) detectors",
                    "n_photon": N, "n_genstep": G, "kernel": args.kernel, "block": args.block,
                    "refill_period": args.refill, "bvh_leaf": args.leaf, "trace_regroup_pct": args.regroup,
+                   "refill_batch": args.batch,
                    "outputs": "photon+seq+hit", "steps_per_photon": S, "hits": int(n_hit)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -237,7 +241,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if not args.no_adjoint:
-        a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)), regroup=args.regroup)
+        a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)), regroup=args.regroup, batch=args.batch)
         line["adjoint"] = a
         line["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
     if not args.no_cpu:
@@ -379,7 +383,7 @@ def _config1_arrays():
 
 def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
     """64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons in
-    the 20,000-detector geometry: autotune (block, refill period, TLAS leaf,
+    the 20,000-detector geometry: autotune (block, refill batch, TLAS leaf,
     trace regroup percent) on
     the first events, capture the whole batch into a hipGraph, time replays.
     History agreement: seq of the first events against the plain kernel run
@@ -434,7 +438,7 @@ def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
             "hits": int(ho[-1]),
             "autotune": {"events": tune_events, "seconds": tune_s, "candidates": len(tune),
                          "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size",
-                                                         "trace_regroup_pct")},
+                                                         "trace_regroup_pct", "refill_batch")},
                          "chosen_ms": best["ms"], "slowest_ms": max(r["ms"] for r in tune)},
             "seqhis_agreement_vs_plain_per_event": same / max(total, 1), "agreement_photons": total}
 

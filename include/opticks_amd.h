@@ -144,6 +144,8 @@ typedef struct mi_config {
     int32_t  trace_regroup_pct; /* trace: a wave's node walk hands over to the leaf phase once
                                    fewer than this percent of its walking lanes still walk
                                    (0 = once none do; 0..100; results do not depend on it) */
+    int32_t  refill_batch;     /* wavefront kernel: a wave refills once this many of its lanes
+                                   are idle, or all are (1..64, default 1; results do not depend on it) */
 } mi_config;
 
 typedef struct mi_sim mi_sim;
@@ -372,14 +374,14 @@ int  mi_batch_destroy(mi_batch* b);
 
 /*
  * Autotuner (no counterpart in the reference): times the batched form on the
- * given events for every block size {64,128,256} x refill (compaction) period
- * {1,4,16} x TLAS leaf size {1,2,4} x trace regroup percent {0,25}, reps timed
- * runs each after one warm run,
+ * given events for every block size {64,128,256} x refill batch {1,16,32} x
+ * TLAS leaf size {1,2,4} x trace regroup percent {0,25} (refill period as
+ * configured), reps timed runs each after one warm run,
  * and leaves the handle configured with the fastest. results (optional,
  * max_results entries) receives each candidate's mean device ms.
  */
 typedef struct mi_tune_result {
-    int32_t block_size, refill_period, bvh_leaf_size, kernel, trace_regroup_pct;
+    int32_t block_size, refill_period, bvh_leaf_size, kernel, trace_regroup_pct, refill_batch;
     double ms;
 } mi_tune_result;
 int  mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids,

@@ -42,7 +42,7 @@ class MiConfig(C.Structure):
                 ("precooked", C.c_void_p), ("precooked_ni", C.c_int64), ("precooked_nk", C.c_int64),
                 ("output_mask", C.c_uint32), ("max_slot", C.c_int64), ("kernel", C.c_int32),
                 ("block_size", C.c_int32), ("refill_period", C.c_int32), ("bvh_leaf_size", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("trace_regroup_pct", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("trace_regroup_pct", C.c_int32), ("refill_batch", C.c_int32)]
 
 
 class MiEventOut(C.Structure):
@@ -58,7 +58,8 @@ class MiAdjoint(C.Structure):
 
 class MiTuneResult(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("refill_period", C.c_int32), ("bvh_leaf_size", C.c_int32),
-                ("kernel", C.c_int32), ("trace_regroup_pct", C.c_int32), ("ms", C.c_double)]
+                ("kernel", C.c_int32), ("trace_regroup_pct", C.c_int32), ("refill_batch", C.c_int32),
+                ("ms", C.c_double)]
 
 
 MI_RECORD_PHOTON, MI_RECORD_PHOTONLITE = 0, 1

@@ -188,6 +188,7 @@ void mi_config_default(mi_config* c)
     c->bvh_leaf_size = 1;
     c->waves_per_cu = 0;
     c->trace_regroup_pct = 0;
+    c->refill_batch = 1;
 }
 
 const char* mi_last_error(void) { return g_err.c_str(); }
@@ -199,6 +200,7 @@ static int check_config(const mi_config* c)
         return fail(MI_ERR_ARG, "block_size must be 64, 128 or 256");
     if (c->bvh_leaf_size < 1 || c->bvh_leaf_size > 64) return fail(MI_ERR_ARG, "bvh_leaf_size must be 1..64");
     if (c->trace_regroup_pct < 0 || c->trace_regroup_pct > 100) return fail(MI_ERR_ARG, "trace_regroup_pct must be 0..100");
+    if (c->refill_batch < 1 || c->refill_batch > 64) return fail(MI_ERR_ARG, "refill_batch must be 1..64");
     if (c->max_bounce < 0) return fail(MI_ERR_ARG, "max_bounce < 0");
     if (c->kernel != MI_KERNEL_PLAIN && c->kernel != MI_KERNEL_WAVEFRONT) return fail(MI_ERR_ARG, "unknown kernel");
     if (c->rng_kind == MI_RNG_PRECOOKED && (!c->precooked || c->precooked_ni <= 0 || c->precooked_nk <= 0))
@@ -537,6 +539,7 @@ static SimParams base_params(const mi_sim* sim)
     P.next = sim->next.p;
     P.refill_period = c.refill_period;
     P.regroup_pct = c.trace_regroup_pct;
+    P.refill_batch = c.refill_batch;
     return P;
 }
 
@@ -1409,7 +1412,7 @@ int mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids, cons
     *n_results = 0;
     if (reps < 1) reps = 1;
     static const int blocks[] = { 64, 128, 256 };
-    static const int refills[] = { 1, 4, 16 };
+    static const int batches[] = { 1, 16, 32 };
     static const int leaves[] = { 1, 2, 4 };
     static const int regroups[] = { 0, 25 };
     const mi_config base = sim->cfg;
@@ -1419,10 +1422,10 @@ int mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids, cons
     int rc = MI_OK;
     for (int leaf : leaves) {
         for (int blk : blocks) {
-            for (int rf : refills) {
+            for (int rb : batches) {
                 for (int rg : regroups) {
                     mi_config c = base;
-                    c.bvh_leaf_size = leaf; c.block_size = blk; c.refill_period = rf; c.trace_regroup_pct = rg;
+                    c.bvh_leaf_size = leaf; c.block_size = blk; c.refill_batch = rb; c.trace_regroup_pct = rg;
                     rc = mi_sim_set_config(sim, &c);
                     if (rc) break;
                     rc = mi_sim_simulate_batch_device(sim, n_event, event_ids, gs_offsets, d_gensteps, po.data(),
@@ -1438,8 +1441,8 @@ int mi_sim_autotune(mi_sim* sim, int32_t n_event, const int32_t* event_ids, cons
                     const double ms = tot / reps;
                     if (results && *n_results < max_results) {
                         mi_tune_result& t = results[*n_results];
-                        t.block_size = blk; t.refill_period = rf; t.bvh_leaf_size = leaf; t.kernel = c.kernel;
-                        t.trace_regroup_pct = rg; t.ms = ms;
+                        t.block_size = blk; t.refill_period = c.refill_period; t.bvh_leaf_size = leaf;
+                        t.kernel = c.kernel; t.trace_regroup_pct = rg; t.refill_batch = rb; t.ms = ms;
                     }
                     ++*n_results;
                     if (ms < best_ms) { best_ms = ms; best = c; }

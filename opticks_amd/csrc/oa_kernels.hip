@@ -242,11 +242,15 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
     bool active = false;
     bool exhausted = false;
     const int period = P.refill_period > 0 ? P.refill_period : 1;
+    /* lanes refill together once `batch` of them are idle: photons started in the
+       same iteration from one genstep stay in step (the same boundary at the same
+       bounce), so the CSG tests of a detector shell run with more lanes at once */
+    const int batch = P.refill_batch > 1 ? P.refill_batch : 1;
     for (int iter = 0;; ++iter) {
         OA_CYC_MARK(c_iter);
         const unsigned long long idle = __ballot(!active);
         const unsigned long long all = __ballot(true);
-        if (!exhausted && idle != 0ull && (idle == all || (iter % period) == 0)) {
+        if (!exhausted && idle != 0ull && (idle == all || ((iter % period) == 0 && __popcll(idle) >= batch))) {
             const int n_idle = __popcll(idle);
             const int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0ull;
@@ -781,9 +785,11 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjP
     Lane L;
     bool active = false;
     bool exhausted = n <= 0;
+    const int batch = P.refill_batch > 1 ? P.refill_batch : 1;
     for (;;) {
         const unsigned long long idle = __ballot(!active);
-        if (!exhausted && idle != 0ull) {
+        const unsigned long long all = __ballot(true);
+        if (!exhausted && idle != 0ull && (idle == all || __popcll(idle) >= batch)) {
             const int n_idle = __popcll(idle);
             const int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0ull;

@@ -114,5 +114,6 @@ struct SimParams {
     /* persistent kernel */
     unsigned long long* next;     /* global photon counter */
     int32_t refill_period;
+    int32_t refill_batch;         /* refill once this many lanes are idle (or all are) */
     int32_t regroup_pct;          /* trace: leaf phase once fewer than this % of the walking lanes still walk */
 };

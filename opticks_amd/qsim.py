@@ -76,6 +76,7 @@ class SEventConfig:
     bvh_leaf_size: int = 1
     waves_per_cu: int = 0
     trace_regroup_pct: int = 0
+    refill_batch: int = 1
 
     ENV = dict(max_bounce=("OPTICKS_MAX_BOUNCE", int), max_time=("OPTICKS_MAX_TIME", float),
                propagate_epsilon=("OPTICKS_PROPAGATE_EPSILON", float),
@@ -121,6 +122,7 @@ class SEventConfig:
         c.bvh_leaf_size = self.bvh_leaf_size
         c.waves_per_cu = self.waves_per_cu
         c.trace_regroup_pct = self.trace_regroup_pct
+        c.refill_batch = self.refill_batch
         return c
 
 
@@ -298,7 +300,7 @@ class QSim:
         return Batch(self, gs_t, gs_offsets, event_ids, photon_t, seq_t, hit_t, adjoint, grad_t)
 
     def autotune(self, gs_t, gs_offsets, event_ids, photon_t, seq_t=None, hit_t=None, reps=2):
-        """Pick block size, refill period, TLAS leaf size and trace regroup percent on these
+        """Pick block size, refill batch, TLAS leaf size and trace regroup percent on these
         events; returns all candidates."""
         ne = len(event_ids)
         ev = np.ascontiguousarray(event_ids, np.int32)
@@ -315,8 +317,10 @@ class QSim:
         self.config.block_size, self.config.refill_period = cfg.block_size, cfg.refill_period
         self.config.bvh_leaf_size = cfg.bvh_leaf_size
         self.config.trace_regroup_pct = cfg.trace_regroup_pct
+        self.config.refill_batch = cfg.refill_batch
         return [dict(block_size=r.block_size, refill_period=r.refill_period, bvh_leaf_size=r.bvh_leaf_size,
-                     trace_regroup_pct=r.trace_regroup_pct, ms=r.ms) for r in res[:min(n.value, 128)]]
+                     trace_regroup_pct=r.trace_regroup_pct, refill_batch=r.refill_batch, ms=r.ms)
+                for r in res[:min(n.value, 128)]]
 
     def merge_hits_device(self, in_t, select_mask=None, time_window=1.0, out_t=None, stream=None):
         """SPM::merge_partial_select on the GPU. in_t: (n,4,4) f32 sphoton records or (n,4)

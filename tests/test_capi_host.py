@@ -43,10 +43,13 @@ def test_defaults_and_errors_without_compute():
     bad = L.MiConfig(); L.lib().mi_config_default(C.byref(bad)); bad.block_size = 96
     assert L.lib().mi_sim_create(None, None, C.byref(bad), C.byref(h)) == -1
     assert "block_size" in L.last_error()
-    assert c.trace_regroup_pct == 0
+    assert c.trace_regroup_pct == 0 and c.refill_batch == 1
     bad.block_size = 64; bad.trace_regroup_pct = 101
     assert L.lib().mi_sim_create(None, None, C.byref(bad), C.byref(h)) == -1
     assert "trace_regroup_pct" in L.last_error()
+    bad.trace_regroup_pct = 0; bad.refill_batch = 65
+    assert L.lib().mi_sim_create(None, None, C.byref(bad), C.byref(h)) == -1
+    assert "refill_batch" in L.last_error()
 
 
 def test_seqhis_labels_and_masks():

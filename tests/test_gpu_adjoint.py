@@ -75,6 +75,10 @@ def test_adjoint_matches_oracle_rows():
     errc = np.abs(Gc.cpu().numpy().astype(np.float64) - G64)
     assert np.all(errc <= 1e-5 * A64 + 1e-30), errc.max()
     assert np.count_nonzero(G) > 1000
+    # launch tuning does not change the replay: batched refills and early node-walk hand-over
+    sim.set_config(SEventConfig(seed=w.seed, refill_batch=32, trace_regroup_pct=25))
+    Gt = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, n_photon=N).cpu().numpy().astype(np.float64)
+    assert np.array_equal(Gt, G)
     # the 256 parameters in order: LS absorption, LS Rayleigh, water absorption, water Rayleigh
     assert np.abs(G).sum(axis=0).reshape(2, 2, 64).sum(axis=2).min() > 0
     sim.close()

@@ -113,11 +113,11 @@ def test_autotune_picks_a_candidate_and_keeps_results(w2):
     photon, seq, hit = buffers(N)
     res = sim.autotune(torch.from_numpy(gs_np).to(DEV), offs, ev_ids, photon, seq, hit, reps=1)
     assert len(res) == 54
-    assert {r["trace_regroup_pct"] for r in res} == {0, 25}
+    assert {r["trace_regroup_pct"] for r in res} == {0, 25} and {r["refill_batch"] for r in res} == {1, 16, 32}
     assert all(r["ms"] > 0 for r in res)
     best = min(res, key=lambda r: r["ms"])
     c = sim.config
-    keys = ("block_size", "refill_period", "bvh_leaf_size", "trace_regroup_pct")
+    keys = ("block_size", "refill_period", "bvh_leaf_size", "trace_regroup_pct", "refill_batch")
     assert tuple(getattr(c, k) for k in keys) == tuple(best[k] for k in keys)
     _, ho, P2, S2, H2 = eager(sim, gs_np, offs, ev_ids, N)
     assert np.array_equal(ho, ho_ref)

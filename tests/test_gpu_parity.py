@@ -114,16 +114,17 @@ def test_config1_full_size_properties_and_samples():
 
 
 def test_plain_wavefront_and_regroup_identical_full_size():
-    """One lane per photon, the persistent wave kernel, and the wave kernel with its
-    node walk handing over to the leaf phase early (25% and 100% regroup) give
-    the same photons and histories bit for bit on the 10M workload."""
+    """One lane per photon, the persistent wave kernel, the wave kernel with its
+    node walk handing over to the leaf phase early (25% and 100% regroup), and
+    with lanes refilled in batches (32, and all 64 at once) give the same photons
+    and histories bit for bit on the 10M workload."""
     w = configs.config1()
     N = w.num_photon
     gs = torch.from_numpy(w.gensteps).to(DEV)
     ref = None
-    for k, rg in ((PLAIN, 0), (WAVE, 0), (WAVE, 25), (WAVE, 100)):
-        print("kernel", k, "trace_regroup_pct", rg, flush=True)
-        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=k, trace_regroup_pct=rg))
+    for k, rg, rb in ((PLAIN, 0, 1), (WAVE, 0, 1), (WAVE, 25, 1), (WAVE, 100, 1), (WAVE, 25, 32), (WAVE, 0, 64)):
+        print("kernel", k, "trace_regroup_pct", rg, "refill_batch", rb, flush=True)
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=k, trace_regroup_pct=rg, refill_batch=rb))
         photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
         seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
         sim.simulate_device(gs, photon, seq, None, n_photon=N)
