@@ -720,9 +720,8 @@ struct ScoreTape {
         float fx = (wl - wl_low) / wl_step;
         float fl = floorf(fx);
         float a = fx - fl;
-        int i0 = (int)fl;
-        i0 = ((i0 % n_wl) + n_wl) % n_wl;
-        int i1 = (i0 + 1) % n_wl;
+        int i0, i1;
+        wrap_pair((int)fl, n_wl, i0, i1);
         const int b0 = A.sample_bin[i0], b1 = A.sample_bin[i1];
         const float sa = ell / (La * La) - (ev == SEG_ABSORB ? 1.f / La : 0.f);
         const float ss = ell / (Ls * Ls) - (ev == SEG_SCATTER ? 1.f / Ls : 0.f);

@@ -19,6 +19,16 @@ namespace oa {
 
 /* ----------------------------------------------------------- tables */
 
+/* Wrap addressing of a table of n texels: i0 = i mod n in [0, n) and its
+   neighbour (i0 + 1) mod n. The integer divisions run only for an index
+   outside the table (wavelengths beyond the domain), the results are the same. */
+OA_D void wrap_pair(int i, int n, int& i0, int& i1)
+{
+    i0 = i;
+    if (i0 < 0 || i0 >= n) i0 = ((i0 % n) + n) % n;
+    i1 = i0 + 1 == n ? 0 : i0 + 1;
+}
+
 /* One lerped float4 of bnd row (2*line+k) at wavelength nm: the FP32 form of
    the linearly filtered, wrap-addressed texture fetch of qbnd::boundary_lookup. */
 OA_D float4 bnd_lookup(const SimParams& P, float nm, int line, int k)
@@ -27,9 +37,8 @@ OA_D float4 bnd_lookup(const SimParams& P, float nm, int line, int k)
     float fx = (nm - P.wl_low) / P.wl_step;
     float fl = floorf(fx);
     float a = fx - fl;
-    int i0 = (int)fl;
-    i0 = ((i0 % nwl) + nwl) % nwl;
-    int i1 = (i0 + 1) % nwl;
+    int i0, i1;
+    wrap_pair((int)fl, nwl, i0, i1);
     const float4* row = P.bnd + (int64_t)(2 * line + k) * nwl;
     float4 v0 = row[i0];
     float4 v1 = row[i1];
@@ -74,9 +83,8 @@ OA_D float icdf_tex(const SimParams& P, float x, int layer)
     float xb = x * (float)n - 0.5f;
     float fl = floorf(xb);
     float a = xb - fl;
-    int i0 = (int)fl;
-    i0 = ((i0 % n) + n) % n;
-    int i1 = (i0 + 1) % n;
+    int i0, i1;
+    wrap_pair((int)fl, n, i0, i1);
     int L = layer < P.n_icdf_layer ? layer : 0;
     const float* row = P.icdf + (int64_t)L * n;
     float v0 = row[i0], v1 = row[i1];
