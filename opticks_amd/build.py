@@ -4,7 +4,11 @@ Build libopticks_amd.so in-tree with hipcc for gfx950.
   python -m opticks_amd.build
 
 Flags: -ffp-contract=off keeps a*b+c unfused so results match the CPU checker
-bit for bit; packed FP32 instructions are disabled (-packed-fp32-ops).
+bit for bit; packed FP32 instructions are disabled (-packed-fp32-ops); the
+AMDGPU machine scheduler runs its iterative ILP strategy, which leaves the
+propagation kernel fewer register spills under its 168-VGPR budget (56 instead
+of 85 spilled VGPRs) and measures 2% faster; instruction order only, the
+arithmetic and so the results are the same.
 """
 from __future__ import annotations
 
@@ -18,7 +22,8 @@ OUT = os.path.join(HERE, "libopticks_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-sched-strategy=iterative-ilp",
+         "-Wall", "-Wno-unused-function"]
 
 
 def build(verbose=False):
