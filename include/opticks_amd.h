@@ -145,7 +145,7 @@ typedef struct mi_config {
                                    fewer than this percent of its walking lanes still walk
                                    (0 = once none do; 0..100; results do not depend on it) */
     int32_t  refill_batch;     /* wavefront kernel: a wave refills once this many of its lanes
-                                   are idle, or all are (1..64, default 1; results do not depend on it) */
+                                   are idle, or all are (1..64, default 16; results do not depend on it) */
 } mi_config;
 
 typedef struct mi_sim mi_sim;

@@ -188,7 +188,7 @@ void mi_config_default(mi_config* c)
     c->bvh_leaf_size = 1;
     c->waves_per_cu = 0;
     c->trace_regroup_pct = 0;
-    c->refill_batch = 1;
+    c->refill_batch = 16;
 }
 
 const char* mi_last_error(void) { return g_err.c_str(); }

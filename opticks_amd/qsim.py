@@ -76,7 +76,7 @@ class SEventConfig:
     bvh_leaf_size: int = 1
     waves_per_cu: int = 0
     trace_regroup_pct: int = 0
-    refill_batch: int = 1
+    refill_batch: int = 16
 
     ENV = dict(max_bounce=("OPTICKS_MAX_BOUNCE", int), max_time=("OPTICKS_MAX_TIME", float),
                propagate_epsilon=("OPTICKS_PROPAGATE_EPSILON", float),

@@ -43,7 +43,7 @@ def test_defaults_and_errors_without_compute():
     bad = L.MiConfig(); L.lib().mi_config_default(C.byref(bad)); bad.block_size = 96
     assert L.lib().mi_sim_create(None, None, C.byref(bad), C.byref(h)) == -1
     assert "block_size" in L.last_error()
-    assert c.trace_regroup_pct == 0 and c.refill_batch == 1
+    assert c.trace_regroup_pct == 0 and c.refill_batch == 16
     bad.block_size = 64; bad.trace_regroup_pct = 101
     assert L.lib().mi_sim_create(None, None, C.byref(bad), C.byref(h)) == -1
     assert "trace_regroup_pct" in L.last_error()
