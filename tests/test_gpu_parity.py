@@ -209,6 +209,22 @@ def test_precooked_uniforms(w0):
     sim.close()
 
 
+def test_wavelengths_outside_the_table_domain_wrap_like_the_checker(w0):
+    """Torch photons at 40, 900 and 1500 nm, outside the 60-820 nm table domain:
+    the lookups take the wrap-addressing path for out-of-range texels, as the
+    reference's wrap-mode textures do, and match the checker bit for bit."""
+    gs = np.stack([configs.torch_genstep(20000, configs.T_SPHERE, wavelength=wl) for wl in (40.0, 900.0, 1500.0)])
+    gs = gs.astype(np.float32)
+    sim = QSim(w0.geometry, w0.physics, SEventConfig(seed=5))
+    hits = sim.simulate(gs, 0)
+    ref_p, ref_s = po.World(w0.geometry, w0.physics).simulate(gs, seed=5)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s == 1.0 and b == 1.0
+    assert np.array_equal(np.unique(sim.photon[:, 2, 3]), np.float32([40.0, 900.0, 1500.0]))
+    assert np.array_equal(hits.view(np.uint32), stable_hits(ref_p).view(np.uint32))
+    sim.close()
+
+
 def test_zero_gensteps_and_empty_counts(w0):
     sim = QSim(w0.geometry, w0.physics, SEventConfig())
     assert sim.simulate(np.zeros((0, 6, 4), np.float32), 0) is None
