@@ -9,14 +9,16 @@ uniform in 200-800 nm, FP32 physics, integer seqhis.
 
 One step = one simulate call with gensteps already in HBM: genstep prefix
 (seeding) -> propagation kernel -> stable hit compaction, outputs photon
-(N,4,4), seq (N,2,2) and hits into preallocated device buffers.
+(N,4,4), seq (N,2,2) and hits into preallocated device buffers. The 10M legs
+run with 64-thread blocks, trace regroup 25% and refill batches of 32 lanes
+(--block, --regroup, --batch); none of these changes a result.
 
 Also reported: the dominant kernel's roofline (HIP events on the kernel's
 stream), the CPU checker's rate on a bounded sample on this host, the
 adjoint/forward device-time ratio on the same geometry with 256 binned
 LS/water absorption and Rayleigh lengths (10M photons, 2,000 detectors), and
 the batched 64-event workload (autotuned, replayed from a captured hipGraph),
-and the rows around the core path (hit merge, sphotonlite output, Cerenkov
+the 256-event forward + adjoint graph, and the rows around the core path (hit merge, sphotonlite output, Cerenkov
 generation, simtrace), each timed on the GPU beside its CPU counterpart.
 
 --copy-calibration-mib M appends one device copy of M MiB after the timed
