@@ -6,7 +6,9 @@
  *      selected" mask), stable;
  *   1. time window 0: return the selection in input order;
  *   2-4. key = (identity << 48) | (unsigned)(time / window) as a 64-bit
- *      integer, then a stable sort of the selected records by key;
+ *      integer, then a stable sort of the selected records by key; here the
+ *      same order is sorted as (identity & 0xffff) << 32 | bucket over its 48
+ *      significant bits (the shift by 48 keeps the same 16 identity bits);
  *   6. reduce each run of equal keys with the record type's reduce_op.
  * Record types and their functors:
  *   sphoton     64 B  key/reduce/select  sysrap/sphoton.h:292-320
@@ -54,9 +56,9 @@ __global__ void k_keys(const uint32_t* in, const uint32_t* sel, const int32_t* d
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *d_nsel) return;
     const uint32_t* r = in + (int64_t)sel[i] * Acc::WORDS;
-    const uint32_t id = Acc::identity(r);
+    const uint32_t id = Acc::identity(r) & 0xffffu;           /* 16 bits of identity in the key */
     const uint32_t bucket = (uint32_t)(Acc::time(r) / tw);
-    keys[i] = ((uint64_t)id << 48) | (uint64_t)bucket;
+    keys[i] = ((uint64_t)id << 32) | (uint64_t)bucket;          /* 48 significant bits */
     vals[i] = sel[i];
 }
 
@@ -171,7 +173,7 @@ hipError_t sort_runs(const Scratch& S, int record, const void* d_in, int64_t n_s
         hipLaunchKernelGGL(k_keys<LiteAcc>, dim3(nblk(n_sel, 256)), dim3(256), 0, s, (const uint32_t*)d_in, S.sel,
                            S.counts, tw, S.keys, S.vals);
     size_t bytes = S.tmp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(S.tmp, bytes, S.keys, S.keys2, S.vals, S.vals2, (int)n_sel, 0, 64, s);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(S.tmp, bytes, S.keys, S.keys2, S.vals, S.vals2, (int)n_sel, 0, 48, s);
     if (e != hipSuccess) return e;
     bytes = S.tmp_bytes;
     e = hipcub::DeviceRunLengthEncode::Encode(S.tmp, bytes, S.keys2, S.keys, S.run_len, S.counts + 1, (int)n_sel, s);
