@@ -83,7 +83,27 @@ __global__ void k_reduce_photon(const uint32_t* in, const uint32_t* order, const
     float bt = __uint_as_float(in[(int64_t)best * 16 + 3]);
     uint32_t fm = in[(int64_t)best * 16 + 15];
     uint32_t hc = in[(int64_t)best * 16 + 7] >> 16;
-    for (int32_t j = 1; j < c; ++j) {
+    int32_t j = 1;
+    /* four records' words in flight per step, folded in the same order as one at a time */
+    for (; j + 4 <= c; j += 4) {
+        uint32_t b[4], tw[4], fw[4], hw[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[u] = order[o + j + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            tw[u] = in[(int64_t)b[u] * 16 + 3];
+            fw[u] = in[(int64_t)b[u] * 16 + 15];
+            hw[u] = in[(int64_t)b[u] * 16 + 7];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float t = __uint_as_float(tw[u]);
+            if (!(fminf(bt, t) == bt)) { best = b[u]; bt = t; }
+            fm |= fw[u];
+            hc = (hc + (hw[u] >> 16)) & 0xffffu;
+        }
+    }
+    for (; j < c; ++j) {
         const uint32_t b = order[o + j];
         const float t = __uint_as_float(in[(int64_t)b * 16 + 3]);
         if (!(fminf(bt, t) == bt)) { best = b; bt = t; }
@@ -109,7 +129,24 @@ __global__ void k_reduce_lite(const uint32_t* in, const uint32_t* order, const i
     uint32_t hci = a[0];
     float t = __uint_as_float(a[1]);
     uint32_t fm = a[3];
-    for (int32_t j = 1; j < c; ++j) {
+    int32_t j = 1;
+    /* four records in flight per step, folded in the same order as one at a time */
+    for (; j + 4 <= c; j += 4) {
+        const uint32_t* b[4];
+        uint32_t hw[4], tw[4], fw[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[u] = in + (int64_t)order[o + j + u] * 4;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { hw[u] = b[u][0]; tw[u] = b[u][1]; fw[u] = b[u][3]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            t = fminf(t, __uint_as_float(tw[u]));
+            fm |= fw[u];
+            const uint32_t hc = ((hci >> 16) + (hw[u] >> 16)) & 0xffffu;
+            hci = (hc << 16) | (hci & 0x0000ffffu);
+        }
+    }
+    for (; j < c; ++j) {
         const uint32_t* b = in + (int64_t)order[o + j] * 4;
         t = fminf(t, __uint_as_float(b[1]));
         fm |= b[3];
