@@ -85,9 +85,15 @@ extern "C" {
 #define MI_OUT_SEQ    0x2u
 #define MI_OUT_HIT    0x4u
 
-/* rng_kind */
+/* rng_kind: the per-photon random stream (qudarap/qrng.h:68-148, sysrap/srng.h:30-36) */
 #define MI_RNG_PHILOX    0    /* Philox4x32-10, counter keyed on absolute photon index */
-#define MI_RNG_PRECOOKED 1    /* uniforms read from a caller array (aligned comparison) */
+#define MI_RNG_PRECOOKED 1    /* uniforms read from a caller array (aligned comparison):
+                                 row = absolute photon index, the cursor wrapping modulo
+                                 precooked_nk (sysrap/s_seq.h:105-129); a launch whose
+                                 photons lack a row fails with MI_ERR_ARG */
+#define MI_RNG_XORWOW    2    /* Marsaglia xorwow (the RNG_XORWOW generator of qrng.h:68-106),
+                                 own seeding per photon and GF(2) jump-ahead for the
+                                 offset + event skipahead (oa_portable.h) */
 
 /* kernel: which propagation kernel runs the bounce loop */
 #define MI_KERNEL_PLAIN      0   /* one thread per photon, CSGOptiX7.cu:432-530 shape */
@@ -131,9 +137,11 @@ typedef struct mi_config {
     uint64_t rng_offset;       /* QRng__SEED_OFFSET offset (0)    */
     uint64_t event_skipahead;  /* OPTICKS_EVENT_SKIPAHEAD 100000  */
     int32_t  rng_kind;         /* MI_RNG_*                        */
-    const float* precooked;    /* (precooked_ni, precooked_nk) uniforms, host */
+    const float* precooked;    /* (precooked_ni, precooked_nk) uniforms, host; copied to the
+                                  device by create/set_config. NULL with the ni/nk the handle
+                                  already holds keeps that copy (mi_sim_get_config returns NULL) */
     int64_t  precooked_ni;
-    int64_t  precooked_nk;
+    int64_t  precooked_nk;     /* 1 .. 2^31-1 */
     uint32_t output_mask;      /* MI_OUT_* bits                   */
     int64_t  max_slot;         /* photons per launch, 0 = from free VRAM */
     int32_t  kernel;           /* MI_KERNEL_*                     */
@@ -168,6 +176,12 @@ const char* mi_version(void);
 int  mi_sim_create(const mi_geometry* geo, const mi_physics* phys,
                    const mi_config* cfg, mi_sim** out);
 void mi_sim_destroy(mi_sim* sim);
+/* set_config validates and builds (TLAS for a new leaf size, precooked upload)
+   before it commits: on failure the handle keeps its previous configuration.
+   A change of any physics or random-stream setting (max_bounce, max_time, tmin,
+   tmin0, eps0_mask, tmax, hit_mask, seed, rng_offset, event_skipahead, rng_kind,
+   precooked values) or of the TLAS invalidates captured batches (mi_batch_launch
+   then fails; recreate them). */
 int  mi_sim_set_config(mi_sim* sim, const mi_config* cfg);
 int  mi_sim_get_config(const mi_sim* sim, mi_config* cfg);
 
@@ -178,8 +192,9 @@ int  mi_sim_reset(mi_sim* sim, int32_t event_id);
 
 /*
  * Device-resident form. d_gensteps: (n_gs,6,4) in device memory.
- * n_photon: total photons of the gensteps when known, or -1 to have the
- * library sum them on the device (one 8-byte read back).
+ * n_photon: photons to propagate, at most the gensteps' total (a smaller
+ * count propagates that prefix), or -1 for the total; the total is read back
+ * (8 bytes) and a larger n_photon fails with MI_ERR_ARG.
  * d_photon (n_photon,4,4), d_seq (n_photon,2,2) u64, d_hit (hit_capacity,4,4):
  * caller-owned device buffers; any may be NULL to skip that output.
  * photon_offset: absolute index of the first photon (multi-launch slicing,
@@ -343,8 +358,9 @@ int  mi_sim_merge_hits_device(mi_sim* sim, int32_t record, const void* d_in, int
  * gensteps' contents may change but not their per-genstep photon counts
  * (checked on the device; mi_batch_wait then fails with MI_ERR_ARG).
  * Launch settings (kernel, block, refill, waves) are those of the handle at
- * create; a later change of TLAS leaf size or precooked randoms invalidates
- * the batch (launch fails; recreate it). bnd values set with mi_sim_set_bnd
+ * create; a later change of TLAS leaf size or of a physics or random-stream
+ * setting (see mi_sim_set_config) invalidates the batch (launch fails;
+ * recreate it). bnd values set with mi_sim_set_bnd
  * are seen by later launches. One launch in flight per batch.
  */
 typedef struct mi_batch mi_batch;

@@ -14,7 +14,7 @@ LIBPATH = os.path.join(HERE, "libopticks_amd.so")
 
 MI_OK, MI_NOTHING_TO_DO = 0, 1
 MI_OUT_PHOTON, MI_OUT_SEQ, MI_OUT_HIT = 0x1, 0x2, 0x4
-MI_RNG_PHILOX, MI_RNG_PRECOOKED = 0, 1
+MI_RNG_PHILOX, MI_RNG_PRECOOKED, MI_RNG_XORWOW = 0, 1, 2
 MI_KERNEL_PLAIN, MI_KERNEL_WAVEFRONT = 0, 1
 
 

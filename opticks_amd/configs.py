@@ -24,7 +24,9 @@ Choices the workload statements leave open (documented, fixed here):
   * world box half-side 6500 mm (config1) / 21000 mm (config2), inner surface absorbs;
   * water absorption 20 m and Rayleigh 30 m (as config0), acrylic absorption and
     Rayleigh 1e6 mm; detector solids are vacuum inside;
-  * detector neck z in [-250,-40] mm, inner bore z in [-260,-60] mm (solid frame);
+  * detector neck z in [-240,-40] mm (h 200 mm as stated; its top is 10 mm above
+    the ellipsoid's z=-50 cut, so the union has no coincident faces), inner bore
+    z in [-260,-60] mm (solid frame);
   * torch subtype T_POINT_ISO=100: T_SPHERE directions about gs.pos, wavelength
     uniform in [gs.wavelength, gs.wavelength+gs.weight] (sysrap/storch.h has no
     isotropic point source at gs.pos);
@@ -36,7 +38,7 @@ import numpy as np
 
 from .foundry import (Foundry, Node, CSG_UNION, CSG_DIFFERENCE, scale_transform,
                       rotation_z_to, fibonacci_sphere)
-from .physics import BoundaryLib, Material, Surface, gaussian_icdf, EMS_SURFACE
+from .physics import BoundaryLib, Material, Surface, gaussian_icdf, EMS_SURFACE, EMS_ZPLUS_SENSOR_A
 
 # sysrap/OpticksGenstep.h, sysrap/storchtype.h
 OpticksGenstep_TORCH = 6
@@ -208,7 +210,7 @@ def detector_tree():
     """difference(union(z-cut ellipsoid, neck cylinder), inner bore cylinder)."""
     ell = Node.zsphere(190.0, -50.0, 190.0)
     ell.transform = scale_transform(254.0 / 190.0, 254.0 / 190.0, 1.0)
-    neck = Node.cylinder(127.0, -250.0, -40.0)
+    neck = Node.cylinder(127.0, -240.0, -40.0)
     bore = Node.cylinder(120.0, -260.0, -60.0)
     return Node.op(CSG_DIFFERENCE, Node.op(CSG_UNION, ell, neck), bore)
 
@@ -395,6 +397,60 @@ def boolean_zoo(num_photon=200_000, seed=42, spacing=400.0):
 def list_zoo(num_photon=200_000, seed=42, spacing=400.0):
     """leaf_zoo world with the list_zoo_nodes solids"""
     return leaf_zoo(num_photon, seed, spacing, nodes=list_zoo_nodes(), name="list_zoo")
+
+
+SURFACE_ZOO_SOURCES = {"centre_A": (0.0, 0.0, 0.0), "outside_A": (-1500.0, 0.0, 0.0),
+                       "centre_B": (1400.0, 0.0, 0.0), "above_B": (1400.0, 0.0, 900.0),
+                       "below_B": (1400.0, 0.0, -900.0)}
+
+
+def surface_zoo(num_photon=200_000, seed=42, mix=(0.2, 0.3, 0.3, 0.2), sens=(0.1, 0.4, 0.3, 0.2)):
+    """Surfaces with every propagate_at_surface outcome and the zplus_sensor_A dispatch
+    (qudarap/qsim.h:1680-1758, 2296-2320). mix / sens = (detect, absorb, specular, diffuse).
+
+      sphere A  r 800 at the origin, boundary Vacuum/mixSurf/mixSurf/Clear: an ordinary
+                surface (ems Surface) on both sides: SA, SD, DR (Lambertian), SR (mirror)
+      sphere B  r 300, its own instance at (1400, 0, 0) (sensor 0), boundary
+                Vacuum/sensA/sensA/Clear with ems Surface_zplus_sensor_A: hits with
+                lposcost >= 0 (upper hemisphere of its frame) detect after one burned
+                uniform, hits with lposcost < 0 take the surface fractions
+      world     box half-side 2000 mm, Vacuum/detSurf/detSurf/Vacuum (detect 1)
+      Clear     n 1.333, absorption and Rayleigh lengths 1e6 mm
+
+    One genstep of num_photon // 5 photons per SURFACE_ZOO_SOURCES entry, 440 nm:
+    T_SPHERE at A's centre, T_POINT_ISO elsewhere (outside A, centre of B, above
+    and below B)."""
+    lib = BoundaryLib()
+    lib.add_material(Material("Vacuum", 1.0, 1e9, 1e9))
+    lib.add_material(Material("Clear", 1.333, 1e6, 1e6))
+    lib.add_surface(Surface("detSurf", detect=1.0, absorb=0.0))
+    d, a, sp, df = mix
+    lib.add_surface(Surface("mixSurf", detect=d, absorb=a, specular=sp, diffuse=df))
+    d, a, sp, df = sens
+    lib.add_surface(Surface("sensA", detect=d, absorb=a, specular=sp, diffuse=df, ems=EMS_ZPLUS_SENSOR_A))
+    b_world = lib.boundary("Vacuum/detSurf/detSurf/Vacuum")
+    b_a = lib.boundary("Vacuum/mixSurf/mixSurf/Clear")
+    b_b = lib.boundary("Vacuum/sensA/sensA/Clear")
+    fd = Foundry()
+    p0 = fd.addPrim(Node.box3(4000.0, 4000.0, 4000.0), b_world)
+    p1 = fd.addPrim(Node.sphere(800.0), b_a)
+    s0 = fd.addSolid("world", [p0, p1])
+    pb = fd.addPrim(Node.sphere(300.0), b_b)
+    s1 = fd.addSolid("sensor", [pb])
+    fd.addInstance(np.eye(4), s0)
+    t = np.eye(4)
+    t[3, :3] = SURFACE_ZOO_SOURCES["centre_B"]
+    fd.addInstance(t, s1, sensor_identifier=0, sensor_index=0)
+    phys = lib.arrays()
+    phys["icdf"], phys["icdf_hd_factor"] = gaussian_icdf()
+    phys["boundary_names"] = lib.boundaries
+    per = num_photon // len(SURFACE_ZOO_SOURCES)
+    gs = [torch_genstep(per, T_SPHERE, wavelength=440.0)]
+    gs += [torch_genstep(per, T_POINT_ISO, pos=p, wavelength=440.0)
+           for k, p in SURFACE_ZOO_SOURCES.items() if k != "centre_A"]
+    return Workload("surface_zoo", fd.arrays(), phys, np.stack(gs).astype(np.float32), seed=seed,
+                    meta=dict(mix=mix, sens=sens, per=per, r_a=800.0, r_b=300.0,
+                              sources=list(SURFACE_ZOO_SOURCES.values())))
 
 
 def uniform_ball(rng, n, r):

@@ -18,9 +18,12 @@
 #include <cmath>
 #include <algorithm>
 #include <array>
+#include <map>
+#include <utility>
 
 #include "../../include/opticks_amd.h"
 #include "oa_layout.h"
+#include "oa_portable.h"
 #include "oa_bvh.h"
 #include "oa_kernels.h"
 #include "oa_merge.h"
@@ -55,6 +58,7 @@ struct DBuf {
         return e;
     }
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); }
 };
 
 template <typename T>
@@ -138,6 +142,10 @@ struct mi_sim {
     int32_t n_bnd = 0, n_wl = 0, n_icdf = 0, n_icdf_layer = 0, icdf_hd = 0;
     float wl_low = 0.f, wl_step = 1.f;
     DBuf<float> precooked;
+    int64_t pre_ni = 0, pre_nk = 0;                 /* shape of the device copy of precooked */
+    /* xorwow jump matrices: per-launch table (n_event x 800 words) and a host cache by skip */
+    DBuf<uint32_t> xw_jump;
+    std::map<uint64_t, std::vector<uint32_t>> xw_cache;
 
     /* work */
     DBuf<float> gs;
@@ -203,8 +211,11 @@ static int check_config(const mi_config* c)
     if (c->refill_batch < 1 || c->refill_batch > 64) return fail(MI_ERR_ARG, "refill_batch must be 1..64");
     if (c->max_bounce < 0) return fail(MI_ERR_ARG, "max_bounce < 0");
     if (c->kernel != MI_KERNEL_PLAIN && c->kernel != MI_KERNEL_WAVEFRONT) return fail(MI_ERR_ARG, "unknown kernel");
-    if (c->rng_kind == MI_RNG_PRECOOKED && (!c->precooked || c->precooked_ni <= 0 || c->precooked_nk <= 0))
-        return fail(MI_ERR_ARG, "precooked rng needs a (ni,nk) array");
+    if (c->rng_kind != MI_RNG_PHILOX && c->rng_kind != MI_RNG_PRECOOKED && c->rng_kind != MI_RNG_XORWOW)
+        return fail(MI_ERR_ARG, "unknown rng_kind");
+    if (c->rng_kind == MI_RNG_PRECOOKED &&
+        (c->precooked_ni <= 0 || c->precooked_nk <= 0 || c->precooked_nk > (int64_t)INT32_MAX))
+        return fail(MI_ERR_ARG, "precooked rng needs a (ni, nk) array with ni >= 1 and 1 <= nk <= 2^31-1");
     return MI_OK;
 }
 
@@ -303,7 +314,8 @@ static void instance_items(const mi_sim* sim, std::vector<oa_bvh::Item>& titems)
     }
 }
 
-/* host SAH build of the TLAS over instance boxes with the given leaf size, then upload */
+/* host SAH build of the TLAS over instance boxes with the given leaf size, uploaded
+   into new buffers that replace the handle's only when every step succeeded */
 static int build_tlas(mi_sim* sim, int leaf)
 {
     std::vector<oa_bvh::Item> titems;
@@ -316,9 +328,18 @@ static int build_tlas(mi_sim* sim, int leaf)
     /* deepest stack a walk can reach: TLAS pushes, then the BLAS walk on top */
     if (tb.stack_need + sim->blas_stack + 1 > OA_STACK_DEPTH)
         return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
-    HIPCHECK(hipStreamSynchronize(sim->stream));
-    HIPCHECK(upload(sim->tlas, tlas.data(), tlas.size(), sim->stream));
-    HIPCHECK(upload(sim->tlas_items, tlas_items.data(), tlas_items.size(), sim->stream));
+    DBuf<BvhNode> nt;
+    DBuf<uint32_t> ni;
+    hipError_t e = hipStreamSynchronize(sim->stream);
+    if (e == hipSuccess) e = upload(nt, tlas.data(), tlas.size(), sim->stream);
+    if (e == hipSuccess) e = upload(ni, tlas_items.data(), tlas_items.size(), sim->stream);
+    if (e != hipSuccess) {
+        nt.release(); ni.release();
+        return fail(MI_ERR_HIP, std::string("TLAS upload: ") + hipGetErrorString(e));
+    }
+    sim->tlas.swap(nt);
+    sim->tlas_items.swap(ni);
+    nt.release(); ni.release();
     sim->tlas_depth = tb.max_depth;
     sim->n_tlas = (int64_t)tlas.size();
     sim->generation++;
@@ -437,16 +458,52 @@ static int upload_physics(mi_sim* sim, const mi_physics* ph)
     return MI_OK;
 }
 
-static int apply_config(mi_sim* sim, const mi_config* cfg)
+/* settings frozen into a captured batch's launch parameters */
+static bool same_physics(const mi_config& a, const mi_config& b)
+{
+    return a.max_bounce == b.max_bounce && a.max_time == b.max_time && a.tmin == b.tmin && a.tmin0 == b.tmin0 &&
+           a.eps0_mask == b.eps0_mask && a.tmax == b.tmax && a.hit_mask == b.hit_mask && a.seed == b.seed &&
+           a.rng_offset == b.rng_offset && a.event_skipahead == b.event_skipahead && a.rng_kind == b.rng_kind;
+}
+
+/*
+ * Validate, build (TLAS for a changed leaf size, precooked upload) and only
+ * then commit. The handle keeps its own device copy of the precooked array;
+ * its config holds a null pointer with that copy's shape, and a config with a
+ * null pointer and the same shape keeps the copy. tlas_built: the TLAS of
+ * cfg's leaf size is already in place (create).
+ */
+static int apply_config(mi_sim* sim, const mi_config* cfg, bool tlas_built)
 {
     int rc = check_config(cfg);
     if (rc) return rc;
-    sim->cfg = *cfg;
-    if (cfg->rng_kind == MI_RNG_PRECOOKED) {
-        HIPCHECK(upload(sim->precooked, cfg->precooked, (size_t)(cfg->precooked_ni * cfg->precooked_nk), sim->stream));
+    const bool pre = cfg->rng_kind == MI_RNG_PRECOOKED;
+    if (pre && !cfg->precooked &&
+        !(sim->precooked.p && sim->pre_ni == cfg->precooked_ni && sim->pre_nk == cfg->precooked_nk))
+        return fail(MI_ERR_ARG, "precooked rng needs the (ni, nk) array (or null with the shape the handle holds)");
+    DBuf<float> npre;
+    if (pre && cfg->precooked) {
+        hipError_t e = upload(npre, cfg->precooked, (size_t)(cfg->precooked_ni * cfg->precooked_nk), sim->stream);
+        if (e != hipSuccess) {
+            npre.release();
+            return fail(MI_ERR_HIP, std::string("precooked upload: ") + hipGetErrorString(e));
+        }
     }
-    sim->cfg.precooked = nullptr;   /* the handle keeps its own device copy */
-    if (cfg->rng_kind == MI_RNG_PRECOOKED) sim->generation++;
+    if (!tlas_built && cfg->bvh_leaf_size != sim->cfg.bvh_leaf_size) {
+        rc = build_tlas(sim, cfg->bvh_leaf_size);   /* bumps generation */
+        if (rc) { npre.release(); return rc; }
+    }
+    if (npre.p) {
+        sim->precooked.swap(npre);
+        npre.release();
+        sim->pre_ni = cfg->precooked_ni;
+        sim->pre_nk = cfg->precooked_nk;
+        sim->generation++;
+    }
+    if (!same_physics(sim->cfg, *cfg)) sim->generation++;
+    sim->cfg = *cfg;
+    sim->cfg.precooked = nullptr;
+    if (!pre) { sim->cfg.precooked_ni = sim->pre_ni; sim->cfg.precooked_nk = sim->pre_nk; }
     return MI_OK;
 }
 
@@ -471,7 +528,7 @@ int mi_sim_create(const mi_geometry* geo, const mi_physics* phys, const mi_confi
     for (auto& e : sim->ev) HIPCHECK(hipEventCreate(&e));
     rc = build_and_upload_geometry(sim, geo);
     if (!rc) rc = upload_physics(sim, phys);
-    if (!rc) rc = apply_config(sim, &c);
+    if (!rc) rc = apply_config(sim, &c, true);
     if (!rc) {
         rc = sim->next.reserve(1) == hipSuccess ? MI_OK : fail(MI_ERR_HIP, "alloc counter");
     }
@@ -506,11 +563,7 @@ void mi_sim_destroy(mi_sim* sim)
 int mi_sim_set_config(mi_sim* sim, const mi_config* cfg)
 {
     if (!sim || !cfg) return fail(MI_ERR_ARG, "null");
-    const int old_leaf = sim->cfg.bvh_leaf_size;
-    int rc = apply_config(sim, cfg);
-    if (rc) return rc;
-    if (cfg->bvh_leaf_size != old_leaf) rc = build_tlas(sim, cfg->bvh_leaf_size);
-    return rc;
+    return apply_config(sim, cfg, false);
 }
 
 int mi_sim_get_config(const mi_sim* sim, mi_config* cfg)
@@ -535,12 +588,48 @@ static SimParams base_params(const mi_sim* sim)
     P.seed = c.seed; P.rng_offset = c.rng_offset; P.skipahead = c.event_skipahead;
     P.rng_kind = c.rng_kind;
     P.precooked = c.rng_kind == MI_RNG_PRECOOKED ? sim->precooked.p : nullptr;
-    P.pre_ni = c.precooked_ni; P.pre_nk = c.precooked_nk;
+    P.pre_ni = sim->pre_ni; P.pre_nk = sim->pre_nk;
+    P.xw_jump = nullptr;
     P.next = sim->next.p;
     P.refill_period = c.refill_period;
     P.regroup_pct = c.trace_regroup_pct;
     P.refill_batch = c.refill_batch;
     return P;
+}
+
+/* every photon index of a launch, [0, end_index), needs a precooked row */
+static int check_precooked_rows(const mi_sim* sim, int64_t end_index)
+{
+    if (sim->cfg.rng_kind == MI_RNG_PRECOOKED && end_index > sim->pre_ni)
+        return fail(MI_ERR_ARG, "precooked uniforms hold " + std::to_string(sim->pre_ni) +
+                                    " rows; the launch needs photon indices up to " + std::to_string(end_index - 1));
+    return MI_OK;
+}
+
+/* xorwow: the jump matrix of each event's skip (offset + event_id * skipahead)
+   uploaded into dst (n_event x 800 words); *table stays null for other generators */
+static int xorwow_table(mi_sim* sim, const int32_t* event_ids, int32_t n_event, DBuf<uint32_t>& dst,
+                        const uint32_t** table)
+{
+    *table = nullptr;
+    if (sim->cfg.rng_kind != MI_RNG_XORWOW) return MI_OK;
+    std::vector<uint32_t> h((size_t)n_event * OA_XORWOW_JUMP_WORDS);
+    std::vector<uint32_t> work;
+    for (int32_t e = 0; e < n_event; ++e) {
+        const uint64_t n = sim->cfg.rng_offset + sim->cfg.event_skipahead * (uint64_t)(uint32_t)event_ids[e];
+        auto it = sim->xw_cache.find(n);
+        if (it == sim->xw_cache.end()) {
+            std::vector<uint32_t> J(OA_XORWOW_JUMP_WORDS);
+            work.resize(OA_XORWOW_WORK_WORDS);
+            oa_xorwow_jump_matrix(n, J.data(), work.data());
+            if (sim->xw_cache.size() >= 4096) sim->xw_cache.clear();
+            it = sim->xw_cache.emplace(n, std::move(J)).first;
+        }
+        std::copy(it->second.begin(), it->second.end(), h.begin() + (size_t)e * OA_XORWOW_JUMP_WORDS);
+    }
+    HIPCHECK(upload(dst, h.data(), h.size(), sim->stream));
+    *table = dst.p;
+    return MI_OK;
 }
 
 /* persistent grid: every resident slot of every CU (occupancy API), or the
@@ -549,7 +638,7 @@ static int wave_grid(const mi_sim* sim)
 {
     const int waves_per_block = sim->cfg.block_size / 64;
     if (sim->cfg.waves_per_cu > 0) return std::max(1, sim->n_cu * sim->cfg.waves_per_cu / waves_per_block);
-    int nb = oa_launch::wave_blocks_per_cu(sim->cfg.block_size);
+    int nb = oa_launch::wave_blocks_per_cu(sim->cfg.block_size, sim->cfg.rng_kind);
     if (nb <= 0) nb = std::max(1, 8 / waves_per_block);
     return std::max(1, sim->n_cu * nb);
 }
@@ -607,15 +696,29 @@ static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gen
     HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, sim->prefix.p, s));
     std::vector<int64_t> hprefix;
     const int64_t max_slot = sim->cfg.max_slot;
-    if (n_photon < 0 || max_slot > 0) {
+    int64_t total = 0;
+    if (max_slot > 0) {
         hprefix.resize((size_t)n_gs + 1);
         HIPCHECK(hipMemcpyAsync(hprefix.data(), sim->prefix.p, sizeof(int64_t) * (n_gs + 1), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
-        if (n_photon < 0) n_photon = hprefix[n_gs];
-        else if (n_photon != hprefix[n_gs]) return fail(MI_ERR_ARG, "n_photon does not match the gensteps");
+        total = hprefix[n_gs];
+        if (n_photon >= 0 && n_photon != total)
+            return fail(MI_ERR_ARG, "with max_slot slicing n_photon must equal the gensteps' photons");
+    } else {
+        HIPCHECK(hipMemcpyAsync(&total, sim->prefix.p + n_gs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (n_photon > total)
+            return fail(MI_ERR_ARG, "n_photon " + std::to_string(n_photon) + " exceeds the gensteps' " +
+                                        std::to_string(total) + " photons");
     }
+    if (n_photon < 0) n_photon = total;
     HIPCHECK(hipEventRecord(sim->ev[1], s));
     if (n_photon == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    int prc = check_precooked_rows(sim, photon_offset + n_photon);
+    if (prc) return prc;
+    const uint32_t* xw_table = nullptr;
+    prc = xorwow_table(sim, &event_id, 1, sim->xw_jump, &xw_table);
+    if (prc) return prc;
     /* slices of whole gensteps with at most max_slot photons (sysrap/SGenstep.h:248) */
     std::vector<std::pair<int64_t, int64_t>> slices;
     if (max_slot > 0 && n_photon > max_slot) {
@@ -647,6 +750,7 @@ static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gen
         HIPCHECK(hipMemcpyAsync(sim->evbuf.p, evh, sizeof(evh), hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(sim->ev_id.p, &evid, sizeof(evid), hipMemcpyHostToDevice, s));
         SimParams P = base_params(sim);
+        P.xw_jump = xw_table;
         P.gensteps = (const float*)d_gensteps + 24 * g0;
         P.gs_prefix = sim->prefix.p;
         P.n_gs = g1 - g0;
@@ -729,12 +833,20 @@ int mi_sim_simulate_batch_device(mi_sim* sim, int32_t n_event, const int32_t* ev
     }
     const int64_t n_photon = hp[n_gs];
     if (n_photon == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    int64_t max_ev = 0;
+    for (int e = 0; e < n_event; ++e) max_ev = std::max(max_ev, photon_offsets[e + 1] - photon_offsets[e]);
+    int prc = check_precooked_rows(sim, max_ev);
+    if (prc) return prc;
+    const uint32_t* xw_table = nullptr;
+    prc = xorwow_table(sim, event_ids, n_event, sim->xw_jump, &xw_table);
+    if (prc) return prc;
     HIPCHECK(sim->evbuf.reserve(evh.size() + n_event + 1));
     HIPCHECK(sim->ev_id.reserve((size_t)n_event));
     HIPCHECK(hipMemcpyAsync(sim->evbuf.p, evh.data(), sizeof(int64_t) * evh.size(), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(sim->ev_id.p, event_ids, sizeof(int32_t) * n_event, hipMemcpyHostToDevice, s));
     HIPCHECK(hipEventRecord(sim->ev[1], s));
     SimParams P = base_params(sim);
+    P.xw_jump = xw_table;
     P.gensteps = (const float*)d_gensteps;
     P.gs_prefix = sim->prefix.p;
     P.n_gs = n_gs;
@@ -977,6 +1089,11 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
         HIPCHECK(hipMemcpyAsync(&n_photon, sim->prefix.p + n_gs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
     }
+    int prc = check_precooked_rows(sim, photon_offset + n_photon);
+    if (prc) return prc;
+    const uint32_t* xw_table = nullptr;
+    prc = xorwow_table(sim, &event_id, 1, sim->xw_jump, &xw_table);
+    if (prc) return prc;
     int64_t evh[4] = { 0, n_gs, 0, n_photon };
     int32_t evid = event_id;
     HIPCHECK(sim->evbuf.reserve(4));
@@ -984,6 +1101,7 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     HIPCHECK(hipMemcpyAsync(sim->evbuf.p, evh, sizeof(evh), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(sim->ev_id.p, &evid, sizeof(evid), hipMemcpyHostToDevice, s));
     SimParams P = base_params(sim);
+    P.xw_jump = xw_table;
     P.gensteps = (const float*)d_gensteps;
     P.gs_prefix = sim->prefix.p;
     P.n_gs = n_gs;
@@ -1010,7 +1128,7 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     HIPCHECK(sim->adj_tmp.reserve(tmp_bytes));
     HIPCHECK(sim->adj_seg.reserve((size_t)n_det + 1));
     /* persistent replay over every resident block slot */
-    const int replay_grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu());
+    const int replay_grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu(sim->cfg.rng_kind));
     double replay_ms = 0.0, reduce_ms = 0.0;
     int64_t h0 = 0;
     do {
@@ -1109,7 +1227,7 @@ struct mi_batch {
     hipEvent_t ev[2] = { nullptr, nullptr };
     DBuf<int64_t> prefix, saved_prefix, evbuf, block_total, ho;
     DBuf<int32_t> ev_id, word_prefix, gs_event;
-    DBuf<uint32_t> seed;
+    DBuf<uint32_t> seed, xw_jump;
     DBuf<unsigned long long> words, next, flag;
     int64_t* h_out = nullptr;   /* pinned: hit offsets (n_event+1), then the prefix-mismatch flag */
     bool launched = false;
@@ -1141,7 +1259,7 @@ int mi_batch_destroy(mi_batch* b)
     if (b->h_out) (void)hipHostFree(b->h_out);
     b->prefix.release(); b->saved_prefix.release(); b->evbuf.release(); b->block_total.release(); b->ho.release();
     b->ev_id.release(); b->word_prefix.release(); b->words.release(); b->next.release(); b->flag.release();
-    b->seed.release(); b->gs_event.release();
+    b->seed.release(); b->gs_event.release(); b->xw_jump.release();
     b->mat_block.release(); b->sample_bin.release(); b->rows.release(); b->comp.release();
     b->key.release(); b->val.release(); b->key2.release(); b->val2.release(); b->tmp.release(); b->seg.release();
     delete b;
@@ -1182,7 +1300,7 @@ static hipError_t batch_work(mi_batch* b, const SimParams& P, const void* d_gens
 static hipError_t batch_adjoint_work(mi_batch* b, SimParams P, const void* d_hit, hipStream_t s)
 {
     const mi_sim* sim = b->sim;
-    const int grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu());
+    const int grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu(sim->cfg.rng_kind));
     hipError_t e;
     for (int64_t c = 0; c < b->n_chunk; ++c) {
         const int64_t begin = c * b->chunk;
@@ -1236,6 +1354,13 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
     }
     b->n_photon = hp[n_gs];
     if (b->n_photon == 0) return bail(fail(MI_NOTHING_TO_DO, "gensteps carry no photons"));
+    int64_t max_ev = 0;
+    for (int e = 0; e < n_event; ++e) max_ev = std::max(max_ev, photon_offsets[e + 1] - photon_offsets[e]);
+    int prc = check_precooked_rows(sim, max_ev);
+    if (prc) return bail(prc);
+    const uint32_t* xw_table = nullptr;
+    prc = xorwow_table(sim, event_ids, n_event, b->xw_jump, &xw_table);
+    if (prc) return bail(prc);
     b->n_words = (b->n_photon + 63) / 64;
     const int64_t nb = (b->n_words + 255) / 256;
     BCHECK(upload(b->evbuf, evh.data(), evh.size(), s));
@@ -1252,6 +1377,7 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
     for (auto& e : b->ev) BCHECK(hipEventCreate(&e));
     BCHECK(hipStreamCreate(&b->cap));
     SimParams P = base_params(sim);
+    P.xw_jump = xw_table;
     P.gensteps = (const float*)d_gensteps;
     P.gs_prefix = b->prefix.p;
     P.n_gs = n_gs;
@@ -1356,7 +1482,8 @@ int mi_batch_launch(mi_batch* b, void* stream)
 {
     if (!b) return fail(MI_ERR_ARG, "null batch");
     if (b->generation != b->sim->generation)
-        return fail(MI_ERR_ARG, "the handle's device buffers moved since capture (leaf size or precooked change): recreate the batch");
+        return fail(MI_ERR_ARG, "the handle's TLAS, precooked uniforms or physics/random-stream settings changed "
+                                "since capture: recreate the batch");
     hipStream_t s = stream ? (hipStream_t)stream : b->sim->stream;
     if (b->launched) HIPCHECK(hipEventSynchronize(b->ev[1]));   /* one replay in flight per batch */
     HIPCHECK(hipEventRecord(b->ev[0], s));

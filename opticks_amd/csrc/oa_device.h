@@ -100,22 +100,37 @@ enum : uint32_t { TC_ZERO = 0, TC_UNION = 1, TC_INTERSECTION = 2, TC_DIFFERENCE 
 
 /* ------------------------------------------------------------ random */
 
-struct Rng {
-    oa_philox_stream ph;
-    const float* pre;     /* precooked row or nullptr */
-    int64_t pre_nk, cursor;
-    int kind;
+/* The per-photon generator is a template parameter of the propagation code:
+   each kernel instantiation carries only the state of its own kind. */
+enum : int { RNG_PHILOX = 0, RNG_PRECOOKED = 1, RNG_XORWOW = 2 };
+
+struct PhiloxRng { oa_philox_stream ph; };
+struct XorwowRng { oa_xorwow_state xw; };
+/* precooked uniforms (the reference's s_seq aligned-randoms mode): a row of nk
+   values per photon; the cursor wraps modulo nk as s_seq::flat recycles
+   (sysrap/s_seq.h:105-129) */
+struct PrecookedRng {
+    const float* row;
+    int32_t cursor, nk;
 };
 
-OA_D float uni(Rng& r)
+OA_D float uni(PhiloxRng& r) { return oa_rng_uniform(&r.ph); }
+OA_D float uni(XorwowRng& r) { return oa_xorwow_uniform(&r.xw); }
+OA_D float uni(PrecookedRng& r)
 {
-    if (r.kind == 1) {
-        float v = (r.cursor < r.pre_nk) ? r.pre[r.cursor] : 1.0f;
-        r.cursor++;
-        return v;
-    }
-    return oa_rng_uniform(&r.ph);
+    const float v = r.row[r.cursor];
+    r.cursor = r.cursor + 1 == r.nk ? 0 : r.cursor + 1;
+    return v;
 }
+
+/* Rejection loops of the reference (marsaglia, Rayleigh, torch sphere,
+   Cerenkov position) are unbounded; here each stops after OA_MAX_TRIALS
+   trials and keeps its last trial, so no input (degenerate gensteps, a
+   precooked row that never accepts) can hold a lane forever. The acceptance
+   probabilities are pi/4 (marsaglia, full-range torch sphere), 2/3 on
+   average (Rayleigh) and >= 1/2 (Cerenkov, positive yields): a random stream
+   reaches the bound with probability below (1/2)^65536. */
+#define OA_MAX_TRIALS 65536
 
 /* ------------------------------------------------------------ photon */
 

@@ -9,7 +9,7 @@ hipError_t seed_map(const int64_t* d_prefix, int64_t n_gs, int64_t n_photon, uin
                     const int64_t* d_ev_gs_begin, int32_t n_event, int32_t* d_gs_event, hipStream_t s);
 hipError_t gs_prefix(const float* d_gs, int64_t n_gs, int64_t* d_prefix, hipStream_t s);
 hipError_t simulate(const SimParams& P, int kernel, int block, int grid, hipStream_t s);
-int wave_blocks_per_cu(int block);   /* resident blocks of the persistent kernel per CU */
+int wave_blocks_per_cu(int block, int rng_kind);   /* resident blocks of the persistent kernel per CU */
 hipError_t prefix_check(const int64_t* a, const int64_t* b, int64_t n, unsigned long long* flag, hipStream_t s);
 hipError_t zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 hipError_t compact_hits(const unsigned long long* d_words, int64_t n_words, int32_t* d_word_prefix,
@@ -32,6 +32,6 @@ hipError_t adjoint_reduce(const float4* d_hit, int64_t begin, int64_t n_max, con
                           int64_t hit_cap, uint32_t n_det, int32_t n_param, const float* d_S, uint32_t* d_key,
                           uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp, size_t tmp_bytes,
                           int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s);
-int adjoint_blocks_per_cu();
+int adjoint_blocks_per_cu(int rng_kind);
 size_t adjoint_sort_bytes(int64_t n_hit, uint32_t n_det);
 }

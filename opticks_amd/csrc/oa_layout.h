@@ -91,9 +91,10 @@ struct SimParams {
     uint32_t eps0_mask;
     uint32_t hit_mask;
     uint64_t seed, rng_offset, skipahead;
-    int32_t  rng_kind;
-    const float* precooked;
+    int32_t  rng_kind;            /* 0 Philox, 1 precooked, 2 xorwow */
+    const float* precooked;       /* (pre_ni, pre_nk) uniforms, a row per photon index */
     int64_t  pre_ni, pre_nk;
+    const uint32_t* xw_jump;      /* xorwow: (n_event, 800) jump matrices of each event's skip */
     /* event(s): photons are numbered globally over all events of the launch */
     const float*   gensteps;      /* (n_gs,6,4) all events concatenated */
     const int64_t* gs_prefix;     /* (n_gs+1) exclusive prefix of numphoton */

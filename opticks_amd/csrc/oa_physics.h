@@ -134,21 +134,25 @@ OA_D v3 uniform_sphere(float u0, float u1)
     return mk(cp * sinTheta, sp * sinTheta, cosTheta);
 }
 
-OA_D void marsaglia(v3& dir, Rng& rng)
+template <class R>
+OA_D void marsaglia(v3& dir, R& rng)
 {
     float u, v, b, a;
+    int trials = 0;
     do {
         float u0 = uni(rng);
         float u1 = uni(rng);
         u = 2.f * u0 - 1.f;
         v = 2.f * u1 - 1.f;
         b = u * u + v * v;
-    } while (b > 1.f);
+    } while (b > 1.f && ++trials < OA_MAX_TRIALS);
+    if (b > 1.f) { u = 0.f; v = 0.f; b = 0.f; }   /* bound reached: (0, 0, -1) */
     a = 2.f * sqrtf(1.f - b);
     dir = mk(a * u, a * v, 2.f * b - 1.f);
 }
 
-OA_D void lambertian(v3& dir, v3 normal, float orient, Rng& rng)
+template <class R>
+OA_D void lambertian(v3& dir, v3 normal, float orient, R& rng)
 {
     float ndotv, u;
     int count = 0;
@@ -161,10 +165,12 @@ OA_D void lambertian(v3& dir, v3 normal, float orient, Rng& rng)
     } while (!(u < ndotv) && (count < 1024));
 }
 
-OA_D void rayleigh(Photon& p, Rng& rng)
+template <class R>
+OA_D void rayleigh(Photon& p, R& rng)
 {
     v3 direction, polarization;
     bool looping = true;
+    int trials = 0;
     do {
         float u0 = uni(rng), u1 = uni(rng), u2 = uni(rng), u3 = uni(rng), u4 = uni(rng);
         float cosTheta = u0;
@@ -187,15 +193,15 @@ OA_D void rayleigh(Photon& p, Rng& rng)
         polarization = normalize(polarization);
         float doCosTheta = dot(polarization, p.pol);
         looping = doCosTheta * doCosTheta < u4;
-    } while (looping);
+    } while (looping && ++trials < OA_MAX_TRIALS);
     p.mom = direction;
     p.pol = polarization;
 }
 
 /* ----------------------------------------------------------- physics */
 
-template <class Tape>
-OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, const State& s, float dist, Tape& tape)
+template <class R, class Tape>
+OA_D int to_boundary(const SimParams& P, uint32_t& flag, R& rng, Photon& p, const State& s, float dist, Tape& tape)
 {
     const float absorption_length = s.material1.y;
     const float scattering_length = s.material1.z;
@@ -236,7 +242,8 @@ OA_D int to_boundary(const SimParams& P, uint32_t& flag, Rng& rng, Photon& p, co
     return CMD_BOUNDARY;
 }
 
-OA_D int at_boundary(uint32_t& flag, Rng& rng, Photon& p, float n1, float n2, v3 normal)
+template <class R>
+OA_D int at_boundary(uint32_t& flag, R& rng, Photon& p, float n1, float n2, v3 normal)
 {
     const float eta = n1 / n2;
     const float _c1 = -dot(p.mom, normal);
@@ -277,7 +284,8 @@ OA_D int at_boundary(uint32_t& flag, Rng& rng, Photon& p, float n1, float n2, v3
     return CMD_CONTINUE;
 }
 
-OA_D int at_surface(uint32_t& flag, Rng& rng, Photon& p, const State& s, v3 normal)
+template <class R>
+OA_D int at_surface(uint32_t& flag, R& rng, Photon& p, const State& s, v3 normal)
 {
     const float detect = s.surface.x, absorb = s.surface.y, diffuse = s.surface.w;
     const float u_surface = uni(rng);
@@ -302,8 +310,8 @@ OA_D int at_surface(uint32_t& flag, Rng& rng, Photon& p, const State& s, v3 norm
     return action;
 }
 
-template <class Tape>
-OA_D int propagate(const SimParams& P, Rng& rng, Photon& p, const Prd& prd, Tape& tape)
+template <class R, class Tape>
+OA_D int propagate(const SimParams& P, R& rng, Photon& p, const Prd& prd, Tape& tape)
 {
     const uint32_t boundary = prd.boundary;
     const uint32_t identity = prd.iindex_identity & 0xffffu;
@@ -340,7 +348,8 @@ OA_D void zero_flags_set(Photon& p, uint32_t flag)
     set_flag(p, flag);
 }
 
-OA_D void gen_torch(Photon& p, Rng& rng, const float4* gq, uint64_t photon_id)
+template <class R>
+OA_D void gen_torch(Photon& p, R& rng, const float4* gq, uint64_t photon_id)
 {
     const float4 q0 = gq[0], q1 = gq[1], q2 = gq[2], q3 = gq[3], q4 = gq[4], q5 = gq[5];
     const uint32_t numphoton = __float_as_uint(q0.w);
@@ -389,11 +398,13 @@ OA_D void gen_torch(Photon& p, Rng& rng, const float4* gq, uint64_t photon_id)
     } else if (type == T_SPHERE_MARSAGLIA) {
         p.wavelength = gwl; p.time = gtime;
         float u, v, b, a;
+        int trials = 0;
         do {
             float u0z = zx + uni(rng) * (zy - zx);
             float u1a = ax + uni(rng) * (ay - ax);
             u = 2.f * u0z - 1.f; v = 2.f * u1a - 1.f; b = u * u + v * v;
-        } while (b > 1.f);
+        } while (b > 1.f && ++trials < OA_MAX_TRIALS);
+        if (b > 1.f) { u = 0.f; v = 0.f; b = 0.f; }
         a = 2.f * sqrtf(1.f - b);
         float rad = fabsf(radius);
         float flip = copysignf(1.f, radius);
@@ -457,7 +468,8 @@ OA_D void gen_torch(Photon& p, Rng& rng, const float4* gq, uint64_t photon_id)
     zero_flags_set(p, F_TORCH);
 }
 
-OA_D void gen_scint(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
+template <class R>
+OA_D void gen_scint(const SimParams& P, Photon& p, R& rng, const float4* gq)
 {
     float u0 = uni(rng), u1 = uni(rng), u2 = uni(rng), u3 = uni(rng);
     float cost = 1.f - 2.f * u0;
@@ -484,7 +496,8 @@ OA_D void gen_scint(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
    flat in energy with rejection on sin^2(theta) against the medium's RINDEX
    from the bnd table (line = genstep matline), cone about the normalized step,
    emission point along the step by rejection on the linear N1 -> N2 yield */
-OA_D void gen_cerenkov(const SimParams& P, Photon& p, Rng& rng, const float4* gq)
+template <class R>
+OA_D void gen_cerenkov(const SimParams& P, Photon& p, R& rng, const float4* gq)
 {
     const float4 q0 = gq[0], q1 = gq[1], q2 = gq[2], q3 = gq[3], q4 = gq[4], q5 = gq[5];
     const int matline = (int)__float_as_uint(q0.z);
@@ -520,13 +533,14 @@ OA_D void gen_cerenkov(const SimParams& P, Photon& p, Rng& rng, const float4* gq
     float fraction, delta, NumberOfPhotons, N, u;
     const float MeanNumberOfPhotonsMax = fmaxf(N1, N2);
     const float DeltaN = (N1 - N2);
+    int trials = 0;
     do {
         fraction = uni(rng);
         delta = fraction * step_length;
         NumberOfPhotons = N1 - fraction * DeltaN;
         u = uni(rng);
         N = u * MeanNumberOfPhotonsMax;
-    } while (N > NumberOfPhotons);
+    } while (N > NumberOfPhotons && ++trials < OA_MAX_TRIALS);
     const float midVelocity = preVelocity + fraction * (postVelocity - preVelocity) * 0.5f;
     p.time = q1.w + delta / midVelocity;
     p.pos = mk(q1.x, q1.y, q1.z) + fraction * dpos;

@@ -1,6 +1,7 @@
 /*
- * oa_portable.h : bit-reproducible FP32 transcendentals and the Philox4x32-10
- * counter-based generator, shared by the HIP kernels and by the CPU checker.
+ * oa_portable.h : bit-reproducible FP32 transcendentals, the Philox4x32-10
+ * counter-based generator and the xorwow generator, shared by the HIP kernels
+ * and by the CPU checker.
  *
  * Everything here is plain C99 arithmetic (no libm calls apart from floorf and
  * sqrtf, which are correctly rounded on both targets) so that gcc and hipcc,
@@ -255,5 +256,165 @@ OA_FN float oa_rng_uniform(oa_philox_stream* s)
 {
     return oa_u32_to_uniform(oa_rng_next_u32(s));
 }
+
+/* --------------------------------------------------------------- XORWOW */
+
+/*
+ * xorwow: G. Marsaglia, "Xorshift RNGs", J. Stat. Softw. 8(14), 2003, section
+ * 3.1: a 160-bit xorshift (x0..x4, shifts 2, 1, 4) plus a Weyl sequence d with
+ * increment 362437; output word = d + x4 after the update. The reference's
+ * qrng<XORWOW> (qudarap/qrng.h:68-106) draws from a generator with this
+ * recurrence. Seeding and subsequences are this project's own convention,
+ * not curand's (whose seeding scramble and skipahead tables are not
+ * reproduced):
+ *   subsequence (photon) start: three outputs w0, w1, w2 of SplitMix64
+ *     (G. Steele, D. Lea, C. Flood, "Fast splittable pseudorandom number
+ *     generators", OOPSLA 2014) whose state starts at mix(seed + gamma) XOR
+ *     photon; x0 = lo(w0), x1 = hi(w0), x2 = lo(w1), x3 = hi(w1),
+ *     x4 = lo(w2), d = hi(w2) (x0 = 1 if x0..x4 would all be zero).
+ *     Distinct photons (< 2^63) start from distinct SplitMix64 inputs.
+ *   skipahead by n words (offset + event * skipahead_event_offset, as
+ *     qrng.h:84-89 skips): the xorshift part is linear over GF(2), so its
+ *     state is multiplied by J = M^n (M the 160x160 one-step matrix), and
+ *     d += 362437 n (mod 2^32). J is formed on the host by square and
+ *     multiply; oa_xorwow_apply multiplies a state by it.
+ * Uniform mapping as for Philox: u = ((x >> 8) + 1) 2^-24 in (0, 1].
+ */
+typedef struct {
+    uint32_t x0, x1, x2, x3, x4, d;
+} oa_xorwow_state;
+
+#define OA_XORWOW_WEYL 362437u
+#define OA_SPLITMIX_GAMMA 0x9E3779B97F4A7C15ull
+#define OA_XORWOW_JUMP_WORDS 800      /* 160 columns x 5 words */
+
+OA_FN uint32_t oa_xorwow_next(oa_xorwow_state* s)
+{
+    const uint32_t t = s->x0 ^ (s->x0 >> 2);
+    s->x0 = s->x1; s->x1 = s->x2; s->x2 = s->x3; s->x3 = s->x4;
+    s->x4 = (s->x4 ^ (s->x4 << 4)) ^ (t ^ (t << 1));
+    s->d += OA_XORWOW_WEYL;
+    return s->d + s->x4;
+}
+
+OA_FN float oa_xorwow_uniform(oa_xorwow_state* s)
+{
+    return oa_u32_to_uniform(oa_xorwow_next(s));
+}
+
+OA_FN uint64_t oa_splitmix64_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+OA_FN void oa_xorwow_seed(oa_xorwow_state* s, uint64_t seed, uint64_t photon)
+{
+    uint64_t sm = oa_splitmix64_mix(seed + OA_SPLITMIX_GAMMA) ^ photon;
+    sm += OA_SPLITMIX_GAMMA; const uint64_t w0 = oa_splitmix64_mix(sm);
+    sm += OA_SPLITMIX_GAMMA; const uint64_t w1 = oa_splitmix64_mix(sm);
+    sm += OA_SPLITMIX_GAMMA; const uint64_t w2 = oa_splitmix64_mix(sm);
+    s->x0 = (uint32_t)w0; s->x1 = (uint32_t)(w0 >> 32);
+    s->x2 = (uint32_t)w1; s->x3 = (uint32_t)(w1 >> 32);
+    s->x4 = (uint32_t)w2; s->d = (uint32_t)(w2 >> 32);
+    if ((s->x0 | s->x1 | s->x2 | s->x3 | s->x4) == 0u) s->x0 = 1u;
+}
+
+/* state x <- J x for a jump matrix J stored as 160 columns of 5 words:
+   column j (words 5j..5j+4) is the image of bit (j % 32) of word j / 32 */
+#define OA_XORWOW_COLS(v, first)                                                          \
+    for (int b = 0; b < 32; ++b) {                                                        \
+        const uint32_t m = 0u - (((v) >> b) & 1u);                                        \
+        const uint32_t* c = J + 5 * ((first) + b);                                        \
+        o0 ^= c[0] & m; o1 ^= c[1] & m; o2 ^= c[2] & m; o3 ^= c[3] & m; o4 ^= c[4] & m;   \
+    }
+
+OA_FN void oa_xorwow_apply(oa_xorwow_state* s, const uint32_t* J)
+{
+    uint32_t o0 = 0u, o1 = 0u, o2 = 0u, o3 = 0u, o4 = 0u;
+    OA_XORWOW_COLS(s->x0, 0)
+    OA_XORWOW_COLS(s->x1, 32)
+    OA_XORWOW_COLS(s->x2, 64)
+    OA_XORWOW_COLS(s->x3, 96)
+    OA_XORWOW_COLS(s->x4, 128)
+    s->x0 = o0; s->x1 = o1; s->x2 = o2; s->x3 = o3; s->x4 = o4;
+}
+
+/* skip n words: J = M^n (precomputed for this n, or NULL when n == 0) */
+OA_FN void oa_xorwow_skip(oa_xorwow_state* s, const uint32_t* J, uint64_t n)
+{
+    if (n != 0u && J) oa_xorwow_apply(s, J);
+    s->d += OA_XORWOW_WEYL * (uint32_t)n;
+}
+
+/*
+ * Host-side jump matrices (plain C, shared by the runtime and the checker).
+ * A matrix is 160 columns x 5 words. Products use a table of the XORs of
+ * every 8-column group of the left factor (the "four Russians" method), so
+ * one product costs 20 x 256 + 160 x 20 five-word XORs.
+ */
+OA_FN void oa_xorwow_step_matrix(uint32_t* M)
+{
+    for (int j = 0; j < 160; ++j) {
+        oa_xorwow_state s;
+        uint32_t in[5] = { 0u, 0u, 0u, 0u, 0u };
+        in[j / 32] = 1u << (j % 32);
+        s.x0 = in[0]; s.x1 = in[1]; s.x2 = in[2]; s.x3 = in[3]; s.x4 = in[4]; s.d = 0u;
+        (void)oa_xorwow_next(&s);
+        uint32_t* c = M + 5 * j;
+        c[0] = s.x0; c[1] = s.x1; c[2] = s.x2; c[3] = s.x3; c[4] = s.x4;
+    }
+}
+
+/* C = A B (C may alias neither A nor B); tab: 20*256*5 words of workspace */
+OA_FN void oa_xorwow_matmul(uint32_t* C, const uint32_t* A, const uint32_t* B, uint32_t* tab)
+{
+    for (int g = 0; g < 20; ++g) {
+        uint32_t* t = tab + 5 * 256 * g;
+        for (int k = 0; k < 5; ++k) t[k] = 0u;
+        for (int v = 1; v < 256; ++v) {
+            const int low = v & -v;                      /* lowest set bit of v */
+            int bit = 0;
+            while ((1 << bit) != low) ++bit;
+            const uint32_t* a = A + 5 * (8 * g + bit);
+            const uint32_t* prev = t + 5 * (v ^ low);
+            for (int k = 0; k < 5; ++k) t[5 * v + k] = prev[k] ^ a[k];
+        }
+    }
+    for (int j = 0; j < 160; ++j) {
+        const uint32_t* b = B + 5 * j;
+        uint32_t o[5] = { 0u, 0u, 0u, 0u, 0u };
+        for (int g = 0; g < 20; ++g) {
+            const uint32_t v = (b[g / 4] >> (8 * (g % 4))) & 0xffu;
+            const uint32_t* t = tab + 5 * (256 * g + v);
+            for (int k = 0; k < 5; ++k) o[k] ^= t[k];
+        }
+        for (int k = 0; k < 5; ++k) C[5 * j + k] = o[k];
+    }
+}
+
+/* J = M^n by square and multiply; work: 3 * 800 + 20*256*5 words */
+OA_FN void oa_xorwow_jump_matrix(uint64_t n, uint32_t* J, uint32_t* work)
+{
+    uint32_t* P = work;            /* M^(2^k)  */
+    uint32_t* T = work + 800;      /* product  */
+    uint32_t* tab = work + 1600;
+    oa_xorwow_step_matrix(P);
+    for (int j = 0; j < 160; ++j)
+        for (int k = 0; k < 5; ++k) J[5 * j + k] = (k == j / 32) ? (1u << (j % 32)) : 0u;
+    while (n) {
+        if (n & 1u) {
+            oa_xorwow_matmul(T, P, J, tab);
+            for (int i = 0; i < 800; ++i) J[i] = T[i];
+        }
+        n >>= 1;
+        if (n) {
+            oa_xorwow_matmul(T, P, P, tab);
+            for (int i = 0; i < 800; ++i) P[i] = T[i];
+        }
+    }
+}
+#define OA_XORWOW_WORK_WORDS (1600 + 20 * 256 * 5)
 
 #endif /* OA_PORTABLE_H */

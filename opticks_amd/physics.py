@@ -25,7 +25,7 @@ C_LIGHT_MM_PER_NS = 299.792458
 WL_LOW, WL_STEP, N_WL = 60.0, 1.0, 761
 
 # smatsur.h
-EMS_MATERIAL, EMS_NOSURFACE, EMS_SURFACE = 0, 1, 2
+EMS_MATERIAL, EMS_NOSURFACE, EMS_SURFACE, EMS_ZPLUS_SENSOR_A = 0, 1, 2, 3
 
 
 def wavelength_domain():

@@ -67,7 +67,8 @@ class SEventConfig:
     seed: int = 0
     rng_offset: int = 0
     event_skipahead: int = 100000
-    precooked: np.ndarray | None = None
+    rng: str = "philox"             # "philox" or "xorwow" (the reference's RNG_PHILOX / RNG_XORWOW, sysrap/srng.h)
+    precooked: np.ndarray | None = None   # (ni, nk) uniforms; overrides rng
     output_mask: int = L.MI_OUT_PHOTON | L.MI_OUT_SEQ | L.MI_OUT_HIT
     max_slot: int = 0
     kernel: int = L.MI_KERNEL_WAVEFRONT
@@ -109,6 +110,10 @@ class SEventConfig:
         c.seed = self.seed
         c.rng_offset = self.rng_offset
         c.event_skipahead = self.event_skipahead
+        rng = {"philox": L.MI_RNG_PHILOX, "xorwow": L.MI_RNG_XORWOW}
+        if self.rng not in rng:
+            raise ValueError(f"rng must be one of {sorted(rng)}")
+        c.rng_kind = rng[self.rng]
         if self.precooked is not None:
             self._pre = np.ascontiguousarray(self.precooked, np.float32)
             c.rng_kind = L.MI_RNG_PRECOOKED
@@ -179,8 +184,17 @@ class QSim:
             pass
 
     def set_config(self, config: SEventConfig):
-        self.config = config
         cfg = config.to_c()
+        L.check(L.lib().mi_sim_set_config(self.handle, C.byref(cfg)), "mi_sim_set_config")
+        self.config = config
+
+    def get_config_c(self) -> L.MiConfig:
+        """The handle's configuration as the C struct (precooked pointer null: the handle keeps its copy)."""
+        cfg = L.MiConfig()
+        L.check(L.lib().mi_sim_get_config(self.handle, C.byref(cfg)), "mi_sim_get_config")
+        return cfg
+
+    def set_config_c(self, cfg: L.MiConfig):
         L.check(L.lib().mi_sim_set_config(self.handle, C.byref(cfg)), "mi_sim_set_config")
 
     def simulate(self, gs: np.ndarray, eventID: int = 0):

@@ -113,20 +113,29 @@ typedef struct {
 
 /* ------------------------------------------------------------ random */
 
+/* kind 0 Philox, 1 precooked uniforms (row of nk values per photon, the
+   cursor wrapping modulo nk as s_seq::flat recycles, sysrap/s_seq.h:105-129),
+   2 xorwow (oa_portable.h convention) */
 typedef struct {
     int kind;
     oa_philox_stream ph;
+    oa_xorwow_state xw;
     const float* pre;       /* precooked row for this photon */
     int64_t nk, cursor;
 } orng;
 
+/* the reference's rejection loops are unbounded; both sides stop after this
+   many trials (unreachable for random streams and valid inputs) */
+#define OA_MAX_TRIALS 65536
+
 static inline float ou(orng* r)
 {
     if (r->kind == 1) {
-        float v = (r->cursor < r->nk) ? r->pre[r->cursor] : 1.0f;
-        r->cursor++;
+        float v = r->pre[r->cursor];
+        r->cursor = r->cursor + 1 == r->nk ? 0 : r->cursor + 1;
         return v;
     }
+    if (r->kind == 2) return oa_xorwow_uniform(&r->xw);
     return oa_rng_uniform(&r->ph);
 }
 
@@ -1306,13 +1315,15 @@ static f3 uniform_sphere(float u0, float u1)
 static void random_direction_marsaglia(f3* dir, orng* rng)
 {
     float u, v, b, a;
+    int trials = 0;
     do {
         float u0 = ou(rng);
         float u1 = ou(rng);
         u = 2.f * u0 - 1.f;
         v = 2.f * u1 - 1.f;
         b = u * u + v * v;
-    } while (b > 1.f);
+    } while (b > 1.f && ++trials < OA_MAX_TRIALS);
+    if (b > 1.f) { u = 0.f; v = 0.f; b = 0.f; }   /* bound reached: (0, 0, -1) */
     a = 2.f * sqrtf(1.f - b);
     dir->x = a * u;
     dir->y = a * v;
@@ -1342,6 +1353,7 @@ static void rayleigh_scatter(oph* p, orng* rng)
 {
     f3 direction, polarization;
     int looping = 1;
+    int trials = 0;
     do {
         float u0 = ou(rng), u1 = ou(rng), u2 = ou(rng), u3 = ou(rng), u4 = ou(rng);
         float cosTheta = u0;
@@ -1370,7 +1382,7 @@ static void rayleigh_scatter(oph* p, orng* rng)
         float doCosTheta = dot3(polarization, p->pol);
         float doCosTheta2 = doCosTheta * doCosTheta;
         looping = doCosTheta2 < u4;
-    } while (looping);
+    } while (looping && ++trials < OA_MAX_TRIALS);
     p->mom = direction;
     p->pol = polarization;
 }
@@ -1605,11 +1617,13 @@ static void torch_generate(oph* p, orng* rng, const float* gs, uint64_t photon_i
     } else if (type == T_SPHERE_MARSAGLIA) {
         p->wavelength = gwl; p->time = gtime;
         float u, v, b, a;
+        int trials = 0;
         do {
             float u0z = zen_x + ou(rng) * (zen_y - zen_x);
             float u1a = azi_x + ou(rng) * (azi_y - azi_x);
             u = 2.f * u0z - 1.f; v = 2.f * u1a - 1.f; b = u * u + v * v;
-        } while (b > 1.f);
+        } while (b > 1.f && ++trials < OA_MAX_TRIALS);
+        if (b > 1.f) { u = 0.f; v = 0.f; b = 0.f; }
         a = 2.f * sqrtf(1.f - b);
         float rad = fabsf(radius);
         float flip = copysignf(1.f, radius);
@@ -1751,13 +1765,14 @@ static void cerenkov_generate(oph* p, orng* rng, const float* gs, const oracle_p
     float fraction, delta, NumberOfPhotons, N, u;
     float MeanNumberOfPhotonsMax = fmaxf(N1, N2);
     float DeltaN = (N1 - N2);
+    int trials = 0;
     do {
         fraction = ou(rng);
         delta = fraction * step_length;
         NumberOfPhotons = N1 - fraction * DeltaN;
         u = ou(rng);
         N = u * MeanNumberOfPhotonsMax;
-    } while (N > NumberOfPhotons);
+    } while (N > NumberOfPhotons && ++trials < OA_MAX_TRIALS);
     float midVelocity = preVelocity + fraction * (postVelocity - preVelocity) * 0.5f;
     p->time = gtime + delta / midVelocity;
     p->pos = add3(gpos, smul3(fraction, dpos));
@@ -1788,18 +1803,41 @@ static int64_t genstep_of(const int64_t* off, int64_t n_gs, int64_t idx)
 }
 
 /* one photon through the CSGOptiX7.cu:432-530 simulate body; returns error bits */
+/* words skipped at the start of every photon stream of an event */
+static uint64_t event_skip(const oracle_cfg* cfg, int32_t event_idx)
+{
+    return cfg->rng_offset + cfg->event_skipahead * (uint64_t)event_idx;
+}
+
+/* xorwow jump matrix of an event (NULL when not xorwow or no skip); free() it */
+static uint32_t* xorwow_jump(const oracle_cfg* cfg, int32_t event_idx)
+{
+    const uint64_t n = event_skip(cfg, event_idx);
+    if (cfg->rng_kind != 2 || n == 0) return NULL;
+    uint32_t* J = (uint32_t*)malloc(sizeof(uint32_t) * OA_XORWOW_JUMP_WORDS);
+    uint32_t* work = (uint32_t*)malloc(sizeof(uint32_t) * OA_XORWOW_WORK_WORDS);
+    oa_xorwow_jump_matrix(n, J, work);
+    free(work);
+    return J;
+}
+
 static int simulate_one(const oworld* w, const oracle_phys* ph, const oracle_cfg* cfg, int32_t event_idx,
                         const float* gs, uint64_t photon_idx, oph* p, uint64_t* seqhis, uint64_t* seqbnd, oscore* sc,
-                        oprd* last)
+                        oprd* last, const uint32_t* xwJ)
 {
     int bad = 0;
     orng rng;
     rng.kind = cfg->rng_kind;
+    if (last) { memset(last, 0, sizeof(*last)); last->ly = NAN; }
     if (rng.kind == 1) {
+        /* every photon needs its row: the runtime rejects shorter arrays */
+        if ((int64_t)photon_idx >= cfg->precooked_ni || cfg->precooked_nk <= 0) { memset(p, 0, sizeof(*p)); return 8; }
         rng.nk = cfg->precooked_nk;
         rng.cursor = 0;
-        rng.pre = ((int64_t)photon_idx < cfg->precooked_ni) ? cfg->precooked + (int64_t)photon_idx * cfg->precooked_nk : NULL;
-        if (!rng.pre) { rng.nk = 0; }
+        rng.pre = cfg->precooked + (int64_t)photon_idx * cfg->precooked_nk;
+    } else if (rng.kind == 2) {
+        oa_xorwow_seed(&rng.xw, cfg->seed, photon_idx);
+        oa_xorwow_skip(&rng.xw, xwJ, event_skip(cfg, event_idx));
     } else {
         oa_rng_init(&rng.ph, cfg->seed, cfg->rng_offset, cfg->event_skipahead, (uint64_t)event_idx, photon_idx);
     }
@@ -1864,6 +1902,7 @@ int64_t oracle_simulate_ex(void* wp, const oracle_phys* ph, const oracle_cfg* cf
     if (p_end > total) p_end = total;
     if (p_begin < 0) p_begin = 0;
     int bad = 0;
+    uint32_t* xwJ = xorwow_jump(cfg, event_idx);
 #ifdef _OPENMP
     if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
 #endif
@@ -1873,7 +1912,8 @@ int64_t oracle_simulate_ex(void* wp, const oracle_phys* ph, const oracle_cfg* cf
         oph p;
         uint64_t seqhis[2], seqbnd[2];
         oprd last;
-        bad |= simulate_one(w, ph, cfg, event_idx, gs, (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, NULL, &last);
+        bad |= simulate_one(w, ph, cfg, event_idx, gs, (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, NULL, &last,
+                            xwJ);
         int64_t o = idx - p_begin;
         if (photon_out) memcpy(photon_out + 16 * o, &p, 64);
         if (lite_out) make_lite(&p, &last, lite_out + 4 * o);
@@ -1882,6 +1922,7 @@ int64_t oracle_simulate_ex(void* wp, const oracle_phys* ph, const oracle_cfg* cf
             seq_out[4 * o + 2] = seqbnd[0]; seq_out[4 * o + 3] = seqbnd[1];
         }
     }
+    free(xwJ);
     free(off);
     return bad ? -(int64_t)bad - 1 : total;
 }
@@ -1911,6 +1952,7 @@ int oracle_adjoint_rows(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
     int64_t* off = genstep_offsets(gensteps, n_gs);
     const int64_t total = off[n_gs];
     int bad = 0;
+    uint32_t* xwJ = xorwow_jump(cfg, event_idx);
 #ifdef _OPENMP
     if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
 #endif
@@ -1925,8 +1967,9 @@ int oracle_adjoint_rows(void* wp, const oracle_phys* ph, const oracle_cfg* cfg, 
         oph p;
         uint64_t seqhis[2], seqbnd[2];
         bad |= simulate_one(w, ph, cfg, event_idx, gensteps + 24 * genstep_of(off, n_gs, idx),
-                            (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, &sc, NULL);
+                            (uint64_t)(photon_offset + idx), &p, seqhis, seqbnd, &sc, NULL, xwJ);
     }
+    free(xwJ);
     free(off);
     return bad ? -bad : 0;
 }
@@ -2014,6 +2057,48 @@ void oracle_uniforms(uint64_t seed, uint64_t offset, uint64_t skip, uint64_t eve
     oa_philox_stream s;
     oa_rng_init(&s, seed, offset, skip, event_idx, photon);
     for (int i = 0; i < n; ++i) out[i] = oa_rng_uniform(&s);
+}
+
+void oracle_xorwow_words(uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event_idx, uint64_t photon,
+                         int32_t n, uint32_t* out, uint32_t* state6)
+{
+    oracle_cfg c;
+    memset(&c, 0, sizeof(c));
+    c.rng_kind = 2; c.rng_offset = offset; c.event_skipahead = skip;
+    uint32_t* J = xorwow_jump(&c, (int32_t)event_idx);
+    oa_xorwow_state s;
+    oa_xorwow_seed(&s, seed, photon);
+    oa_xorwow_skip(&s, J, event_skip(&c, (int32_t)event_idx));
+    if (state6) { state6[0] = s.x0; state6[1] = s.x1; state6[2] = s.x2; state6[3] = s.x3; state6[4] = s.x4; state6[5] = s.d; }
+    for (int i = 0; i < n; ++i) out[i] = oa_xorwow_next(&s);
+    free(J);
+}
+
+void oracle_xorwow_from_state(const uint32_t* state6, uint64_t skip, int32_t n, uint32_t* out, uint32_t* state_out)
+{
+    oa_xorwow_state s;
+    s.x0 = state6[0]; s.x1 = state6[1]; s.x2 = state6[2]; s.x3 = state6[3]; s.x4 = state6[4]; s.d = state6[5];
+    uint32_t* J = NULL;
+    if (skip) {
+        J = (uint32_t*)malloc(sizeof(uint32_t) * OA_XORWOW_JUMP_WORDS);
+        uint32_t* work = (uint32_t*)malloc(sizeof(uint32_t) * OA_XORWOW_WORK_WORDS);
+        oa_xorwow_jump_matrix(skip, J, work);
+        free(work);
+    }
+    oa_xorwow_skip(&s, J, skip);
+    if (state_out) {
+        state_out[0] = s.x0; state_out[1] = s.x1; state_out[2] = s.x2; state_out[3] = s.x3; state_out[4] = s.x4;
+        state_out[5] = s.d;
+    }
+    for (int i = 0; i < n; ++i) out[i] = oa_xorwow_next(&s);
+    free(J);
+}
+
+void oracle_xorwow_jump(uint64_t n, uint32_t* J800)
+{
+    uint32_t* work = (uint32_t*)malloc(sizeof(uint32_t) * OA_XORWOW_WORK_WORDS);
+    oa_xorwow_jump_matrix(n, J800, work);
+    free(work);
 }
 
 int oracle_step_boundary(const float* mom, const float* pol, const float* normal, float n1, float n2,

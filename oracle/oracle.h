@@ -71,7 +71,7 @@ typedef struct oracle_cfg {
     uint64_t seed;
     uint64_t rng_offset;
     uint64_t event_skipahead;
-    int32_t  rng_kind;          /* 0 philox, 1 precooked */
+    int32_t  rng_kind;          /* 0 philox, 1 precooked, 2 xorwow */
     const float* precooked;     /* (precooked_ni, precooked_nk)  */
     int64_t  precooked_ni;
     int64_t  precooked_nk;
@@ -144,6 +144,14 @@ void  oracle_sincosf(float x, float* s, float* c);
 void  oracle_philox_block(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4);
 void  oracle_uniforms(uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event_idx,
                       uint64_t photon, int32_t n, float* out);
+/* xorwow: the first n raw words of a photon stream (and its state after the
+   skip, x0..x4, d, when state6 is not NULL); the jump matrix M^n */
+void  oracle_xorwow_words(uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event_idx,
+                          uint64_t photon, int32_t n, uint32_t* out, uint32_t* state6);
+void  oracle_xorwow_jump(uint64_t n, uint32_t* J800);
+/* from an explicit state (x0..x4, d): skip `skip` words by the jump matrix, then n words */
+void  oracle_xorwow_from_state(const uint32_t* state6, uint64_t skip, int32_t n, uint32_t* out,
+                               uint32_t* state_out);
 
 /* Single physics steps for closed-form checks (qudarap/QSim.cu test kernels). */
 int oracle_step_boundary(const float* mom, const float* pol, const float* normal,

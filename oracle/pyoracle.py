@@ -98,6 +98,10 @@ def lib():
         L.oracle_philox_block.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.oracle_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                       C.c_int32, C.c_void_p]
+        L.oracle_xorwow_words.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                          C.c_int32, C.c_void_p, C.c_void_p]
+        L.oracle_xorwow_jump.argtypes = [C.c_uint64, C.c_void_p]
+        L.oracle_xorwow_from_state.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p]
         L.oracle_step_boundary.restype = C.c_int
         L.oracle_step_boundary.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
                                            C.c_float, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
@@ -153,7 +157,7 @@ class World:
         except Exception:
             pass
 
-    def cfg(self, seed=0, max_bounce=31, traverse=1, nthreads=0, precooked=None, **kw):
+    def cfg(self, seed=0, max_bounce=31, traverse=1, nthreads=0, precooked=None, rng="philox", **kw):
         c = OracleCfg()
         c.max_bounce = max_bounce
         c.max_time = kw.get("max_time", 1e27)
@@ -164,6 +168,7 @@ class World:
         c.seed = seed
         c.rng_offset = kw.get("rng_offset", 0)
         c.event_skipahead = kw.get("event_skipahead", 100000)
+        c.rng_kind = {"philox": 0, "xorwow": 2}[rng]
         if precooked is not None:
             self._pre = np.ascontiguousarray(precooked, np.float32)
             c.rng_kind = 1
