@@ -132,17 +132,25 @@ int main() {
     assert out.startswith("opticks_amd") and "caught: mi_sim_create:" in out
 
 
-def test_cpp_npy_reader_and_folder_load(tmp_path):
-    """mi_npy.hh reads the reference-layout arrays as numpy wrote them, and
-    MI355XSimulator::Load passes them to mi_sim_create: on a host without a GPU
-    the geometry validates and creation stops at the device query."""
+@pytest.mark.parametrize("writer", ["numpy", "reference NP.hh"])
+def test_cpp_npy_reader_and_folder_load(tmp_path, writer):
+    """mi_npy.hh reads the reference-layout arrays as numpy wrote them and as the
+    reference's own NP.hh wrote them (tests/golden/np_foundry, bnd in double
+    precision, tests/golden/make_np_fixture.py), and MI355XSimulator::Load passes
+    them to mi_sim_create: on a host without a GPU the geometry validates and
+    creation stops at the device query."""
     import shutil
     import subprocess
     if shutil.which("g++") is None:
         pytest.skip("no g++")
     from opticks_amd.loader import save_csgfoundry
     w = configs.leaf_zoo(num_photon=10)
-    cf = save_csgfoundry(str(tmp_path / "geom"), w.geometry, w.physics)
+    if writer == "numpy":
+        cf = save_csgfoundry(str(tmp_path / "geom"), w.geometry, w.physics)
+        base = str(tmp_path / "geom")
+    else:
+        base = os.path.join(ROOT, "tests", "golden", "np_foundry")
+        cf = os.path.join(base, "CSGFoundry")
     src = tmp_path / "npy.cpp"
     src.write_text(r'''
 #include <cstdio>
@@ -169,7 +177,7 @@ int main(int argc, char** argv) {
     cmd = ["g++", "-std=c++17", "-I", os.path.join(ROOT, "opticks_amd", "csrc"), str(src), "-o", str(exe),
            "-L", lib, "-lopticks_amd", f"-Wl,-rpath,{lib}", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True)
-    out = subprocess.run([str(exe), cf, str(tmp_path / "geom")], capture_output=True, text=True, check=True).stdout
+    out = subprocess.run([str(exe), cf, base], capture_output=True, text=True, check=True).stdout
     lines = dict(l.split(" ", 1) for l in out.strip().splitlines())
     ref = {"solid": w.geometry["solid"].view(np.int32), "prim": w.geometry["prim"], "node": w.geometry["node"],
            "plan": w.geometry["plan"], "inst": w.geometry["inst"], "SSim/stree/standard/bnd": w.physics["bnd"],

@@ -85,3 +85,36 @@ def test_cpp_adaptor_simulates_like_the_checker(tmp_path):
     assert np.array_equal(photon.view(np.uint32), ref_p.view(np.uint32))
     ref_hit = ref_p[(ref_p[:, 3, 3].view(np.uint32) & 0x40) != 0]
     assert n_hit == len(ref_hit) > 100 and np.array_equal(hit.view(np.uint32), ref_hit.view(np.uint32))
+
+
+def _check_outputs(out_dir, w, seed, n_photon):
+    photon = np.fromfile(out_dir / "photon.bin", np.float32).reshape(-1, 4, 4)
+    seq = np.fromfile(out_dir / "seq.bin", np.uint64).reshape(-1, 2, 2)
+    hit = np.fromfile(out_dir / "hit.bin", np.float32).reshape(-1, 4, 4)
+    assert len(photon) == n_photon
+    ref_p, ref_s = po.World(w.geometry, w.physics).simulate(w.gensteps, seed=seed)
+    assert np.array_equal(seq, ref_s.view(np.uint64).reshape(seq.shape))
+    assert np.array_equal(photon.view(np.uint32), ref_p.view(np.uint32))
+    ref_hit = ref_p[(ref_p[:, 3, 3].view(np.uint32) & 0x40) != 0]
+    assert np.array_equal(hit.view(np.uint32), ref_hit.view(np.uint32))
+
+
+def test_ssimulator_protocol_caller_on_reference_written_folder(tmp_path):
+    """oracle/_ref/ssimulator_check (built here from the reference's sysrap/SSimulator.h,
+    tests/cpp/ssimulator_check.cpp) drives the path through an SSimulator pointer on
+    the folder the reference's NP.hh wrote (tests/golden/np_foundry)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ssimulator_check")
+    if not os.access(exe, os.X_OK):
+        pytest.skip("oracle/_ref/ssimulator_check not built (needs the reference checkout at build time)")
+    w = configs.leaf_zoo(num_photon=4000)
+    np.save(tmp_path / "gs.npy", w.gensteps)
+    print("running the SSimulator caller", flush=True)
+    r = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "np_foundry"), str(tmp_path / "gs.npy"),
+                        str(w.seed), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    n_photon, n_hit, timed = (int(v) for v in lines[-3].split()[:3])
+    assert n_photon == w.num_photon and timed == 1 and "MI355X" in lines[-3]
+    assert lines[-2].split()[2:] == ["0", "0", "0"]
+    assert lines[-1] == "render -1 -1"
+    _check_outputs(tmp_path, w, w.seed, n_photon)

@@ -45,3 +45,26 @@ def test_double_precision_tables_and_missing_arrays(tmp_path):
     os.remove(os.path.join(cf, "inst.npy"))
     with pytest.raises(FileNotFoundError):
         load_csgfoundry(str(tmp_path))
+
+
+def test_reference_np_written_folder():
+    """tests/golden/np_foundry holds the leaf-zoo workload written by the reference's
+    own NP.hh (NP::save with the bnd metadata of sstandard.h:500-506, bnd in double
+    precision): the loader reads every array bit for bit and the simulated photons
+    equal those of the in-memory workload."""
+    base = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "np_foundry")
+    w = configs.leaf_zoo(num_photon=4000)
+    sd = os.path.join(base, "CSGFoundry", "SSim", "stree", "standard")
+    assert np.load(os.path.join(sd, "bnd.npy")).dtype == np.float64
+    assert open(os.path.join(sd, "bnd_meta.txt")).read().split() == [
+        "domain_low:60", "domain_high:820", "domain_step:1", "domain_range:760"]
+    g, p = load_csgfoundry(base)
+    for k in ("solid", "prim", "node", "inst", "itra", "tran", "plan"):
+        assert np.asarray(w.geometry[k]).tobytes() == np.asarray(g[k]).tobytes(), k
+    assert p["bnd"].dtype == np.float32 and p["bnd"].tobytes() == w.physics["bnd"].tobytes()
+    assert p["optical"].tobytes() == w.physics["optical"].tobytes()
+    assert p["icdf"].tobytes() == w.physics["icdf"].tobytes() and p["icdf_hd_factor"] == 20
+    assert (p["wl_low"], p["wl_step"], p["n_wl"]) == (60.0, 1.0, 761)
+    a = po.World(w.geometry, w.physics).simulate(w.gensteps, seed=3)
+    b = po.World(g, p).simulate(w.gensteps, seed=3)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
