@@ -209,6 +209,29 @@ int  mi_sim_simulate_device(mi_sim* sim, int32_t event_id,
                             void* stream);
 
 /*
+ * No counterpart in the reference: the form of mi_sim_simulate_device for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors as
+ * mi_sim_simulate_device, but every device buffer beyond the caller's inputs and
+ * outputs (genstep prefix, event bounds, refill counter, xorwow jump matrix,
+ * hit bitmask, compaction prefixes) lives in one statically sized scratch array
+ * per GPU (about 20 MiB, held until the process ends); the call makes no
+ * device or host allocation on its success path. Limits: at most
+ * MI_NOALLOC_MAX_GENSTEPS gensteps and MI_NOALLOC_MAX_PHOTONS photons per call
+ * (MI_ERR_CAPACITY beyond); no max_slot slicing. All handles and calls on one
+ * GPU share that scratch: calls that overlap on one GPU (other threads or
+ * streams) race and must be serialized by the caller.
+ */
+#define MI_NOALLOC_MAX_GENSTEPS (1ll << 20)
+#define MI_NOALLOC_MAX_PHOTONS  (1ll << 26)
+int  mi_sim_simulate_device_noalloc(mi_sim* sim, int32_t event_id,
+                                    const void* d_gensteps, int64_t n_gs,
+                                    int64_t n_photon, int64_t photon_offset,
+                                    void* d_photon, void* d_seq,
+                                    void* d_hit, int64_t hit_capacity, int64_t* n_hit_out,
+                                    void* stream);
+
+/*
  * Same, also writing d_photonlite (n_photon) sphotonlite records
  * (sysrap/sphotonlite.h:41-60), as CSGOptiX7.cu:520-526 does for
  * evt->photonlite: hitcount 1 | identity, time, 16-bit lposcost / lposfphi of

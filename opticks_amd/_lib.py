@@ -72,7 +72,7 @@ SYMBOLS = ["mi_config_default", "mi_last_error", "mi_version", "mi_sim_create", 
            "mi_sim_last_adjoint_timing", "mi_batch_create", "mi_batch_launch", "mi_batch_wait",
            "mi_batch_num_photon", "mi_batch_destroy", "mi_sim_autotune", "mi_batch_create_adjoint",
            "mi_batch_last_timing", "mi_sim_merge_hits_device", "mi_sim_simulate_lite_device",
-           "mi_sim_simtrace_device", "mi_sim_simtrace"]
+           "mi_sim_simtrace_device", "mi_sim_simtrace", "mi_sim_simulate_device_noalloc"]
 
 _lib = None
 
@@ -97,6 +97,7 @@ def lib():
     L.mi_sim_simulate_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                          C.POINTER(C.c_int64), C.c_void_p]
+    L.mi_sim_simulate_device_noalloc.argtypes = L.mi_sim_simulate_device.argtypes
     L.mi_sim_simulate_batch_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                                C.c_void_p, C.c_void_p]

@@ -146,6 +146,10 @@ struct mi_sim {
     /* xorwow jump matrices: per-launch table (n_event x 800 words) and a host cache by skip */
     DBuf<uint32_t> xw_jump;
     std::map<uint64_t, std::vector<uint32_t>> xw_cache;
+    /* no-allocation entry point: this GPU's scratch array (device address) and
+       host workspace for an xorwow jump matrix, both set up at create */
+    void* scratch = nullptr;
+    std::vector<uint32_t> xw_host_J, xw_host_work;
 
     /* work */
     DBuf<float> gs;
@@ -532,6 +536,12 @@ int mi_sim_create(const mi_geometry* geo, const mi_physics* phys, const mi_confi
     if (!rc) {
         rc = sim->next.reserve(1) == hipSuccess ? MI_OK : fail(MI_ERR_HIP, "alloc counter");
     }
+    if (!rc) {
+        hipError_t e = oa_launch::noalloc_scratch(sim->device, &sim->scratch);
+        if (e != hipSuccess) rc = fail(MI_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+        sim->xw_host_J.resize(OA_XORWOW_JUMP_WORDS);
+        sim->xw_host_work.resize(OA_XORWOW_WORK_WORDS);
+    }
     if (rc) { mi_sim_destroy(sim); return rc; }
     *out = sim;
     return MI_OK;
@@ -798,6 +808,97 @@ int mi_sim_simulate_device(mi_sim* sim, int32_t event_id, const void* d_gensteps
 {
     return simulate_device_impl(sim, event_id, d_gensteps, n_gs, n_photon, photon_offset, d_photon, d_seq, d_hit,
                                 hit_capacity, n_hit_out, nullptr, stream);
+}
+
+/*
+ * No-allocation form of mi_sim_simulate_device: every buffer beyond the
+ * caller's inputs and outputs lives in this GPU's static scratch array (its
+ * address looked up at create), and the call makes no device or host
+ * allocation. Differences from mi_sim_simulate_device: no max_slot slicing,
+ * photons find their genstep by binary search of the prefix (same results),
+ * limits of OA_NOALLOC_MAX_GS gensteps and OA_NOALLOC_MAX_PHOTON photons.
+ */
+int mi_sim_simulate_device_noalloc(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs,
+                                   int64_t n_photon, int64_t photon_offset, void* d_photon, void* d_seq, void* d_hit,
+                                   int64_t hit_capacity, int64_t* n_hit_out, void* stream)
+{
+    if (!sim) return fail(MI_ERR_ARG, "null sim");
+    if (n_hit_out) *n_hit_out = 0;
+    if (n_gs <= 0) return fail(MI_NOTHING_TO_DO, "zero gensteps");
+    if (!d_gensteps) return fail(MI_ERR_ARG, "null gensteps");
+    if (d_hit && !d_photon) return fail(MI_ERR_ARG, "hits are gathered from the photon array: pass d_photon");
+    if (n_gs > OA_NOALLOC_MAX_GS) return fail(MI_ERR_CAPACITY, "no-allocation simulate: more than 2^20 gensteps");
+    if (n_photon > OA_NOALLOC_MAX_PHOTON) return fail(MI_ERR_CAPACITY, "no-allocation simulate: more than 2^26 photons");
+    if (!sim->scratch) return fail(MI_ERR_HIP, "no-allocation simulate: no scratch address");
+    hipStream_t s = stream ? (hipStream_t)stream : sim->stream;
+    char* base = (char*)sim->scratch;
+    int64_t* prefix = (int64_t*)(base + OA_NOALLOC_OFF_PREFIX);
+    int64_t* ev = (int64_t*)(base + OA_NOALLOC_OFF_EV);
+    int32_t* ev_id = (int32_t*)(base + OA_NOALLOC_OFF_EVID);
+    unsigned long long* next = (unsigned long long*)(base + OA_NOALLOC_OFF_NEXT);
+    uint32_t* xw = (uint32_t*)(base + OA_NOALLOC_OFF_XW);
+    unsigned long long* words = (unsigned long long*)(base + OA_NOALLOC_OFF_WORDS);
+    int32_t* word_prefix = (int32_t*)(base + OA_NOALLOC_OFF_WPREFIX);
+    int64_t* block_total = (int64_t*)(base + OA_NOALLOC_OFF_BTOTAL);
+    HIPCHECK(hipEventRecord(sim->ev[0], s));
+    HIPCHECK(oa_launch::gs_prefix((const float*)d_gensteps, n_gs, prefix, s));
+    int64_t total = 0;
+    HIPCHECK(hipMemcpyAsync(&total, prefix + n_gs, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (n_photon > total) return fail(MI_ERR_ARG, "n_photon exceeds the gensteps' photons");
+    if (n_photon < 0) n_photon = total;
+    if (n_photon == 0) return fail(MI_NOTHING_TO_DO, "gensteps carry no photons");
+    if (n_photon > OA_NOALLOC_MAX_PHOTON) return fail(MI_ERR_CAPACITY, "no-allocation simulate: more than 2^26 photons");
+    int rc = check_precooked_rows(sim, photon_offset + n_photon);
+    if (rc) return rc;
+    const uint32_t* xw_table = nullptr;
+    if (sim->cfg.rng_kind == MI_RNG_XORWOW) {
+        const uint64_t skip = sim->cfg.rng_offset + sim->cfg.event_skipahead * (uint64_t)(uint32_t)event_id;
+        oa_xorwow_jump_matrix(skip, sim->xw_host_J.data(), sim->xw_host_work.data());
+        HIPCHECK(hipMemcpyAsync(xw, sim->xw_host_J.data(), sizeof(uint32_t) * OA_XORWOW_JUMP_WORDS,
+                                hipMemcpyHostToDevice, s));
+        xw_table = xw;
+    }
+    HIPCHECK(oa_launch::noalloc_setup(ev, ev_id, next, n_gs, n_photon, event_id, s));
+    HIPCHECK(hipEventRecord(sim->ev[1], s));
+    const int64_t n_words = (n_photon + 63) / 64;
+    SimParams P = base_params(sim);
+    P.xw_jump = xw_table;
+    P.gensteps = (const float*)d_gensteps;
+    P.gs_prefix = prefix;
+    P.seed_map = nullptr;                 /* genstep of each photon by binary search of the prefix */
+    P.gs_event = nullptr;
+    P.n_gs = n_gs;
+    P.n_event = 1;
+    P.ev_gs_begin = ev;
+    P.ev_ph_begin = ev + 2;
+    P.ev_id = ev_id;
+    P.n_photon = n_photon;
+    P.photon_offset = photon_offset;
+    P.photon = (float4*)d_photon;
+    P.seq = (uint64_t*)d_seq;
+    P.photonlite = nullptr;
+    P.hitwords = words;
+    P.next = next;
+    if (sim->cfg.kernel == MI_KERNEL_WAVEFRONT) HIPCHECK(oa_launch::zero_u64(words, n_words, s));
+    HIPCHECK(oa_launch::simulate(P, sim->cfg.kernel, sim->cfg.block_size, wave_grid(sim), s));
+    HIPCHECK(hipEventRecord(sim->ev[2], s));
+    const int64_t nb = (n_words + 255) / 256;
+    HIPCHECK(oa_launch::compact_hits(words, n_words, word_prefix, block_total, (const float4*)d_photon,
+                                     (float4*)d_hit, d_hit ? hit_capacity : 0, s));
+    HIPCHECK(hipEventRecord(sim->ev[3], s));
+    int64_t hits = 0;
+    HIPCHECK(hipMemcpyAsync(&hits, block_total + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float a = 0.f, b = 0.f, c = 0.f, t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&a, sim->ev[0], sim->ev[1]));
+    HIPCHECK(hipEventElapsedTime(&b, sim->ev[1], sim->ev[2]));
+    HIPCHECK(hipEventElapsedTime(&c, sim->ev[2], sim->ev[3]));
+    HIPCHECK(hipEventElapsedTime(&t, sim->ev[0], sim->ev[3]));
+    sim->last_ms[0] = a; sim->last_ms[1] = b; sim->last_ms[2] = c; sim->last_ms[3] = t;
+    if (n_hit_out) *n_hit_out = hits;
+    if (d_hit && hits > hit_capacity) return fail(MI_ERR_CAPACITY, "hit_capacity too small");
+    return MI_OK;
 }
 
 int mi_sim_simulate_lite_device(mi_sim* sim, int32_t event_id, const void* d_gensteps, int64_t n_gs, int64_t n_photon,

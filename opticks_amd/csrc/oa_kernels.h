@@ -33,5 +33,10 @@ hipError_t adjoint_reduce(const float4* d_hit, int64_t begin, int64_t n_max, con
                           uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp, size_t tmp_bytes,
                           int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s);
 int adjoint_blocks_per_cu(int rng_kind);
+/* no-allocation entry point: the device address of the per-GPU scratch array
+   (call with `device` current), and the per-launch event setup kernel */
+hipError_t noalloc_scratch(int device, void** ptr);
+hipError_t noalloc_setup(int64_t* ev, int32_t* ev_id, unsigned long long* next, int64_t n_gs, int64_t n_photon,
+                         int32_t event_id, hipStream_t s);
 size_t adjoint_sort_bytes(int64_t n_hit, uint32_t n_det);
 }

@@ -25,6 +25,24 @@
 #define OA_BVH_WIDTH    4
 #define OA_CSG_MAX_HEIGHT 7
 
+/* Scratch of the no-allocation simulate entry point: one statically sized
+   device array per GPU (oa_kernels.hip), laid out as 256-byte aligned pieces.
+   Limits per call: OA_NOALLOC_MAX_GS gensteps, OA_NOALLOC_MAX_PHOTON photons. */
+#define OA_NOALLOC_MAX_GS      (1ll << 20)
+#define OA_NOALLOC_MAX_PHOTON  (1ll << 26)
+#define OA_ALIGN256(x)         ((((long long)(x)) + 255ll) & ~255ll)
+#define OA_NOALLOC_WORDS       (OA_NOALLOC_MAX_PHOTON / 64 + 1)                 /* hit bitmask words */
+#define OA_NOALLOC_BLOCKS      ((OA_NOALLOC_WORDS + 255) / 256 + 1)             /* compaction block totals */
+#define OA_NOALLOC_OFF_PREFIX  0ll                                              /* (MAX_GS+1) int64 */
+#define OA_NOALLOC_OFF_EV      (OA_NOALLOC_OFF_PREFIX + OA_ALIGN256(8 * (OA_NOALLOC_MAX_GS + 1)))
+#define OA_NOALLOC_OFF_EVID    (OA_NOALLOC_OFF_EV + 256ll)                      /* 4 int64 event bounds */
+#define OA_NOALLOC_OFF_NEXT    (OA_NOALLOC_OFF_EVID + 256ll)                    /* 1 int32 event id */
+#define OA_NOALLOC_OFF_XW      (OA_NOALLOC_OFF_NEXT + 256ll)                    /* 1 u64 refill counter */
+#define OA_NOALLOC_OFF_WORDS   (OA_NOALLOC_OFF_XW + OA_ALIGN256(4 * 800))       /* xorwow jump matrix */
+#define OA_NOALLOC_OFF_WPREFIX (OA_NOALLOC_OFF_WORDS + OA_ALIGN256(8 * OA_NOALLOC_WORDS))
+#define OA_NOALLOC_OFF_BTOTAL  (OA_NOALLOC_OFF_WPREFIX + OA_ALIGN256(4 * OA_NOALLOC_WORDS))
+#define OA_NOALLOC_SCRATCH_BYTES (OA_NOALLOC_OFF_BTOTAL + OA_ALIGN256(8 * OA_NOALLOC_BLOCKS))
+
 /* Optional work counters (build with -DOA_STATS): traces, TLAS node visits,
    instance tests, BLAS node visits, prim tests, tree prim tests, leaf
    evaluations inside trees, propagation steps, wave loop iterations, active
