@@ -1,25 +1,30 @@
 """
 Benchmark: photons propagated per second on one MI355X.
 
-Workload (default): the CSG boolean geometry at 10M photons -- 1,000 torch
-gensteps x 10,000 photons in a 5 m liquid-scintillator sphere, 120 mm acrylic
-shell, water, 2,000 instanced detector solids (difference(union(z-cut
-ellipsoid, cylinder), cylinder)) on a Fibonacci lattice at 5.5 m; wavelengths
-uniform in 200-800 nm, FP32 physics, integer seqhis.
+Headline workload: the batched form, 64 events x Poisson(200) scintillation
+gensteps x Poisson(5000) photons (about 64M photons, uneven) in the
+20,000-detector geometry (LS sphere r 17.7 m, 120 mm acrylic shell, water,
+difference(union(z-cut ellipsoid, cylinder), cylinder) detectors on a Fibonacci
+lattice at 19.4 m), FP32 physics, integer seqhis. Setup (untimed): the
+autotuner picks block size, refill batch, TLAS leaf size and trace regroup on
+the first events, and the whole batch (genstep prefix, seeding, persistent
+propagation, stable hit compaction, per-event hit offsets) is captured into a
+hipGraph. One step = one replay over all events with gensteps resident in HBM,
+outputs photon (N,4,4), seq (N,2,2) and hits into preallocated buffers.
 
-One step = one simulate call with gensteps already in HBM: genstep prefix
-(seeding) -> propagation kernel -> stable hit compaction, outputs photon
-(N,4,4), seq (N,2,2) and hits into preallocated device buffers. The 10M legs
-run with 64-thread blocks, trace regroup 25% and refill batches of 32 lanes
-(--block, --regroup, --batch); none of these changes a result.
+roofline: the propagation kernel (k_sim_wave), timed with HIP events on its
+stream in an eager launch of the same batch; bound "valu": algorithmic FP32
+flops per launch (work counts of profiles/work_*.json x per-unit costs, FLOPS
+below) over the FP32 vector peak, with the HBM view (compulsory bytes, bytes with
+the table/BVH/CSG/transform fetches, PMC traffic) and the measured VALU issue
+rate and lane utilisation (profiles/pmc_*.json) beside it. cpu_baseline: the
+CPU checker on whole events of the same workload for a bounded time.
 
-Also reported: the dominant kernel's roofline (HIP events on the kernel's
-stream), the CPU checker's rate on a bounded sample on this host, the
-adjoint/forward device-time ratio on the same geometry with 256 binned
-LS/water absorption and Rayleigh lengths (10M photons, 2,000 detectors), and
-the batched 64-event workload (autotuned, replayed from a captured hipGraph),
-the 256-event forward + adjoint graph, and the rows around the core path (hit merge, sphotonlite output, Cerenkov
-generation, simtrace), each timed on the GPU beside its CPU counterpart.
+Extra legs: the 10M-photon single event with its own roofline and
+CPU rate, the adjoint/forward ratio with 256 binned LS/water lengths (10M
+photons, 2,000 detectors), the 256-event forward + adjoint captured graph, and
+the rows around the core path (hit merge, sphotonlite output, Cerenkov
+generation, simtrace), each timed beside its CPU counterpart.
 
 --copy-calibration-mib M appends one device copy of M MiB after the timed
 region: a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE counters.
@@ -38,6 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, published
 # VALU issue peak: 256 CUs x 2.4 GHz x one wave64 VALU instruction per CU per
 # clock (4 SIMDs, 16 lanes, 4 cycles per wave64 FP32 instruction)
 VALU_PEAK_GINST = 256 * 2.4
@@ -52,6 +58,51 @@ def algorithmic_bytes(n_photon, n_gs, steps_per_photon, hit_frac):
     per_photon = 64 + 32 + 0.125
     per_step = 4 * 32 + 4
     return n_gs * (96 + 8) + n_photon * (per_photon + steps_per_photon * per_step)
+
+
+# Algorithmic FP32 work of one unit, counted from the kernel source with the rule
+# add/sub/mul = 1, FMA = 2, compare/select/min/max = 0; divisions, square roots,
+# reciprocals, log/sin/cos counted apart as transcendentals (T). Units come from the
+# work counters (opticks_amd.workcount): per photon, traces, TLAS and BLAS node
+# visits, instance tests, prim tests, CSG tree evaluations, leaf evaluations in
+# trees, propagation steps.
+FLOPS = {
+    "node_visit": (48, 0),      # 4-wide node: 4 boxes x 6 slab FMAs (box_enter)
+    "trace": (62, 5),           # ray setup (slab_ray: 3 mul, 3 rcp) + hit record: normal to world (15),
+                                # object ray again (33), local hit point (6), lposcost (5, sqrt, div)
+    "inst": (36, 3),            # world->object ray (33) + its slab setup (3 mul, 3 rcp)
+    "prim_leaf": (30, 3),       # single-leaf prim (sphere/box): quadratic or slabs, roots, normal
+    "tree": (15, 0),            # tree evaluation: classification dot products of 3 leaves
+    "tree_leaf": (71, 4),       # leaf in a tree, averaged over the detector's scaled zsphere
+                                # (frame transform 48 + 45) and two cylinders (60 each)
+    "step": (185, 23),          # per propagation step: normalize + cosTheta (13, 2T), fill_state
+                                # (4 lerped lookups, 56, 4T), to_boundary (9, 3T), at_boundary
+                                # Fresnel (107, 14T) counted for every step (an upper bound)
+    "photon": (55, 10),         # scintillation generation (mom, pol, wavelength, position, time)
+}
+
+
+def algorithmic_flops(work, n_photon):
+    """(flops, transcendentals) per launch from the per-photon work units"""
+    u = work["per_photon"]
+    units = {"node_visit": u["tlas"] + u["blas"], "trace": u["trace"], "inst": u["inst"],
+             "prim_leaf": u["prim"] - u["tree"], "tree": u["tree"], "tree_leaf": u["leaf"], "step": u["step"],
+             "photon": 1.0}
+    f = sum(FLOPS[k][0] * v for k, v in units.items()) * n_photon
+    t = sum(FLOPS[k][1] * v for k, v in units.items()) * n_photon
+    return f, t
+
+
+def algorithmic_bytes_full(work, n_photon, n_gs):
+    """The fetch model of the measurement statement: compulsory bytes plus, per step,
+    fill_state's 4 lookups x 2 texels x 16 B and 3 optical int4; 128 B per 4-wide BVH
+    node visited, 64 B per instance transform, 16 B per prim record and 64 B per CSG
+    node of a tree evaluation (7 per detector tree) or single-leaf prim (mostly
+    L2-resident: the geometry and tables are a few MB)."""
+    u = work["per_photon"]
+    per_photon = (64 + 32 + 0.125 + u["step"] * (4 * 2 * 16 + 3 * 16) + (u["tlas"] + u["blas"]) * 128
+                  + u["inst"] * 64 + u["prim"] * 16 + (u["tree"] * 7 + (u["prim"] - u["tree"])) * 64)
+    return n_gs * (96 + 8) + n_photon * per_photon
 
 
 def mean_steps(seq):
@@ -133,84 +184,128 @@ def bench_adjoint(torch, dev, steps, regroup=0, batch=1):
             "adjoint_forward_ratio": a / f}
 
 
+def batched_workload():
+    """64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons,
+    20,000 detectors at 19.4 m around a 17.7 m LS sphere (opticks_amd.configs.config2)"""
+    from opticks_amd import configs
+    w = configs.config2()
+    gs_all = np.concatenate(w.events).astype(np.float32)
+    offs = np.cumsum([0] + [len(e) for e in w.events])
+    return w, gs_all, offs, list(range(len(w.events)))
+
+
+def pmc_summary(name, n_photon, kname):
+    """HBM bytes and VALU instructions per launch of the dominant kernel on this
+    workload, from rocprofv3 --pmc runs kept in profiles/ (FETCH_SIZE, WRITE_SIZE
+    scaled by a copy kernel of known size; SQ_INSTS_VALU, SQ_THREAD_CYCLES_VALU,
+    SQ_ACTIVE_INST_VALU)"""
+    pj = profile_json(name)
+    if not pj or pj.get("n_photon") != n_photon or str(pj.get("kernel", "")).split("<")[0] != kname:
+        return None
+    return pj
+
+
+def roofline(kernel_ms, n_photon, n_gs, steps, work, pmc):
+    """The dominant kernel's rooflines. It is bound by VALU issue under divergence,
+    so `bound` is "valu": achieved = algorithmic FP32 flops per launch / kernel time
+    against the FP32 vector peak. Beside it the HBM view: the compulsory bytes and the
+    bytes with the table, BVH node, CSG node and instance-transform reads per step
+    (algorithmic fetches, mostly L2-resident), and the PMC-measured traffic."""
+    t = kernel_ms * 1e-3
+    out = {"bound": "valu", "kernel": "k_sim_wave", "kernel_ms": kernel_ms, "peak": VALU_PEAK_TFLOPS,
+           "unit": "TFLOP/s", "traffic": None}
+    if work is not None:
+        F, T = algorithmic_flops(work, n_photon)
+        out.update(achieved=F / t / 1e12, frac=F / t / 1e12 / VALU_PEAK_TFLOPS, alg_flops_per_launch=F,
+                   alg_transcendentals_per_launch=T, work_units=work["per_photon"])
+    else:
+        out.update(achieved=None, frac=None)
+    Bc = algorithmic_bytes(n_photon, n_gs, steps, 0.0)
+    Bf = algorithmic_bytes_full(work, n_photon, n_gs) if work is not None else None
+    out["hbm"] = {"peak_gb_s": HBM_PEAK_GBS, "compulsory_bytes_per_launch": Bc,
+                  "compulsory_gb_s": Bc / t / 1e9, "compulsory_frac": Bc / t / 1e9 / HBM_PEAK_GBS,
+                  "with_fetches_bytes_per_launch": Bf,
+                  "with_fetches_frac": (Bf / t / 1e9 / HBM_PEAK_GBS) if Bf else None}
+    if pmc:
+        out["traffic"] = pmc.get("hbm_bytes_per_launch")
+        if pmc.get("valu_insts_per_launch"):
+            rate = pmc["valu_insts_per_launch"] / t / 1e9
+            out["valu_issue"] = {"achieved_ginst_s": rate, "peak_ginst_s": VALU_PEAK_GINST,
+                                 "frac": rate / VALU_PEAK_GINST, "insts_per_launch": pmc["valu_insts_per_launch"],
+                                 "lane_utilisation": pmc.get("valu_lane_utilisation"),
+                                 "insts_per_photon_step": pmc["valu_insts_per_launch"] / (n_photon * steps)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kernel", default="wavefront", choices=["wavefront", "plain"])
-    ap.add_argument("--block", type=int, default=64)
-    ap.add_argument("--refill", type=int, default=1)
-    ap.add_argument("--leaf", type=int, default=1)
-    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--tune-events", type=int, default=8, help="events the autotuner times (untimed setup)")
     ap.add_argument("--regroup", type=int, default=25,
                     help="trace_regroup_pct of the 10M forward/adjoint legs (best of 0-60 there)")
     ap.add_argument("--batch", type=int, default=32,
                     help="refill_batch of the 10M forward/adjoint legs (best of 1-64 there)")
-    ap.add_argument("--cpu-sample", type=int, default=0, help="photons for the CPU baseline (0 = all)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU baseline: events of the workload until this much wall time on all threads")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="headline only (no 10M / adjoint / 256-event / next rows)")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--copy-calibration-mib", type=int, default=0)
-    ap.add_argument("--no-batched", action="store_true", help="skip the 64-event batched workload")
     ap.add_argument("--no-full", action="store_true", help="skip the 256-event forward+adjoint graph")
     ap.add_argument("--no-next", action="store_true", help="skip the hit merge / lite / Cerenkov / simtrace rows")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "photon ranges would shard per GPU with no exchange until the hit gather", file=sys.stderr)
+              "events would shard per GPU with no exchange until the hit gather", file=sys.stderr)
         sys.exit(2)
 
     import torch
-    from opticks_amd import configs
-    from opticks_amd import _lib as L
     from opticks_amd.qsim import QSim, SEventConfig
 
     dev = torch.device("cuda:0")
-    w = configs.config1()
+    w, gs_all, offs, ev = batched_workload()
     N = w.num_photon
-    G = len(w.gensteps)
-    kernel = L.MI_KERNEL_WAVEFRONT if args.kernel == "wavefront" else L.MI_KERNEL_PLAIN
-    cfg = SEventConfig(seed=w.seed, kernel=kernel, block_size=args.block, refill_period=args.refill,
-                       bvh_leaf_size=args.leaf, waves_per_cu=args.waves_per_cu, trace_regroup_pct=args.regroup,
-                       refill_batch=args.batch)
-    sim = QSim(w.geometry, w.physics, cfg)
-    gs = torch.from_numpy(w.gensteps).to(dev)
+    G = len(gs_all)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs = torch.from_numpy(gs_all).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
-    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+    hit = torch.empty((N // 4, 4, 4), dtype=torch.float32, device=dev)
 
-    n_hit = 0
+    # setup (untimed): autotune the launch on the first events, capture the whole batch
+    t0 = time.perf_counter()
+    tune = sim.autotune(gs, offs[:args.tune_events + 1], ev[:args.tune_events], photon, seq, hit, reps=1)
+    tune_s = time.perf_counter() - t0
+    best = min(tune, key=lambda r: r["ms"])
+    b = sim.batch(gs, offs, ev, photon, seq, hit)
     for _ in range(args.warmup):
-        n_hit = sim.simulate_device(gs, photon, seq, hit, n_photon=N)
+        b.launch(); b.wait()
     torch.cuda.synchronize()
-    prop_ms = []
+    dev_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        n_hit = sim.simulate_device(gs, photon, seq, hit, n_photon=N)
-        prop_ms.append(sim.last_timing()["propagate_ms"])
+        b.launch()
+        ho, ms = b.wait()
+        dev_ms.append(ms)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = 1e3 * dt / args.steps
     value = N * args.steps / dt
+    n_hit = int(ho[-1])
+    b.close()
 
-    S = mean_steps(seq.cpu().numpy().view(np.uint64))
-    kernel_ms = float(np.mean(prop_ms))
-    B = algorithmic_bytes(N, G, S, n_hit / N)
-    achieved = B / (kernel_ms * 1e-3) / 1e9
-    kname = "k_sim_wave" if kernel == L.MI_KERNEL_WAVEFRONT else "k_sim_plain"
-    # HBM bytes and VALU instructions per launch of this kernel on this workload,
-    # from rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE scaled by a copy kernel;
-    # SQ_INSTS_VALU, SQ_THREAD_CYCLES_VALU, SQ_ACTIVE_INST_VALU)
-    traffic, valu = None, None
-    pj = profile_json("pmc_k_sim_wave_csg10M.json")
-    if pj and pj.get("n_photon") == N and str(pj.get("kernel", "")).split("<")[0] == kname:
-        traffic = pj.get("hbm_bytes_per_launch")
-        if pj.get("valu_insts_per_launch"):
-            rate = pj["valu_insts_per_launch"] / (kernel_ms * 1e-3) / 1e9
-            valu = {"achieved_ginst_s": rate, "peak_ginst_s": VALU_PEAK_GINST, "frac": rate / VALU_PEAK_GINST,
-                    "insts_per_launch": pj["valu_insts_per_launch"],
-                    "lane_utilisation": pj.get("valu_lane_utilisation")}
+    # the propagation kernel alone, HIP events on its stream, same batch launched eagerly
+    prop = []
+    for _ in range(3):
+        sim.simulate_batch(gs, offs, ev, photon, seq, hit)
+        prop.append(sim.last_timing()["propagate_ms"])
+    kernel_ms = float(np.mean(prop[1:]))
+    S = mean_steps(seq[:20_000_000].cpu().numpy().view(np.uint64))
+    work = profile_json("work_k_sim_wave_batched64.json")
+    pmc = pmc_summary("pmc_k_sim_wave_batched64.json", N, "k_sim_wave")
 
     line = {
         "metric": "photons propagated/sec on 1 MI355X vs CPU qsim; adjoint/forward time ratio",
@@ -224,16 +319,20 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic gensteps built in code (numpy default_rng(42) placement; Philox photon streams seed 42)",
-        "config": {"workload": "csg-boolean-10M: 1000 torch gensteps x 10000 photons, LS sphere r5m + acrylic "
-                               "+ water + 2000 instanced difference(union(zsphere,cylinder),cylinder) detectors",
-                   "n_photon": N, "n_genstep": G, "kernel": args.kernel, "block": args.block,
-                   "refill_period": args.refill, "bvh_leaf": args.leaf, "trace_regroup_pct": args.regroup,
-                   "refill_batch": args.batch,
-                   "outputs": "photon+seq+hit", "steps_per_photon": S, "hits": int(n_hit)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_ms": kernel_ms, "alg_bytes_per_launch": B, "valu": valu},
+        "data": "synthetic scintillation gensteps built in code (numpy default_rng([42, event]) counts and "
+                "placement; Philox photon streams, seed 42, event skipahead)",
+        "config": {"workload": "batched-64-events: 64 events x Poisson(200) scintillation gensteps x "
+                               "Poisson(5000) photons, LS sphere r17.7m + acrylic + water + 20000 instanced "
+                               "difference(union(zsphere,cylinder),cylinder) detectors at r19.4m; one step = "
+                               "one replay of the captured hipGraph over all events",
+                   "n_photon": int(N), "n_event": len(ev), "n_genstep": int(G), "kernel": "wavefront",
+                   "outputs": "photon+seq+hit", "steps_per_photon": S, "hits": n_hit,
+                   "device_ms_per_step": float(np.mean(dev_ms)),
+                   "autotune": {"events": args.tune_events, "seconds": tune_s, "candidates": len(tune),
+                                "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size",
+                                                                "trace_regroup_pct", "refill_batch")},
+                                "chosen_ms": best["ms"], "slowest_ms": max(r["ms"] for r in tune)}},
+        "roofline": roofline(kernel_ms, N, G, S, work, pmc),
         "cpu_baseline": None,
     }
     if args.copy_calibration_mib > 0:
@@ -242,31 +341,93 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if not args.no_adjoint:
-        a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)), regroup=args.regroup, batch=args.batch)
-        line["adjoint"] = a
-        line["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
-    if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sample = args.cpu_sample if args.cpu_sample > 0 else N
-        rate, cdt = cpu_baseline(w, sample, threads)
-        sample1 = min(sample, 1_000_000)
-        rate1, cdt1 = cpu_baseline(w, sample1, 1)
-        line["cpu_baseline"] = {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
-                                "sample": f"first {sample} photons of the same workload "
-                                          f"({cdt:.1f} s wall on {threads} threads; C restatement of "
-                                          f"qsim/CSG, OpenMP over photons, own median BVH)",
-                                "value_1_thread": rate1,
-                                "sample_1_thread": f"first {sample1} photons, {cdt1:.1f} s on 1 thread",
-                                "host_cpu": cpu_model()}
-    if not args.no_next:
-        line["next_rows"] = bench_next_rows(torch, dev, sim, gs, photon, max(1, min(args.steps, 3)), not args.no_cpu)
-    if not args.no_batched:
-        line["batched"] = bench_batched(torch, dev, max(1, min(args.steps, 3)))
-    if not args.no_full:
-        line["full_batched_adjoint"] = bench_full(torch, dev, max(1, min(args.steps, 2)))
-    print(json.dumps(line), flush=True)
     sim.close()
+    del photon, seq, hit
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_events(w, args.cpu_seconds)
+    if not args.no_legs:
+        c1 = bench_config1(torch, dev, max(1, min(args.steps, 5)), args.regroup, args.batch, not args.no_cpu)
+        line["config1"] = c1
+        if not args.no_adjoint:
+            a = bench_adjoint(torch, dev, max(1, min(args.steps, 3)), regroup=args.regroup, batch=args.batch)
+            line["adjoint"] = a
+            line["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
+        if not args.no_next:
+            line["next_rows"] = c1.pop("_next_rows", None)
+        if not args.no_full:
+            line["full_batched_adjoint"] = bench_full(torch, dev, max(1, min(args.steps, 2)))
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_events(w, seconds):
+    """The CPU checker (C restatement of qsim/CSG, OpenMP over photons, own BVH) on
+    whole events of the batched workload, in order, until `seconds` of wall time on
+    all threads; then a 1-thread rate on part of the first event."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    W = pyoracle.World(w.geometry, w.physics)
+    W.simulate(w.events[0], event_idx=0, seed=w.seed, p_end=2000, nthreads=threads)    # warm
+    done = photons = 0
+    t = 0.0
+    while done < len(w.events) and t < seconds:
+        n = int(w.events[done].view(np.uint32)[:, 0, 3].sum())
+        t0 = time.perf_counter()
+        W.simulate(w.events[done], event_idx=done, seed=w.seed, nthreads=threads)
+        t += time.perf_counter() - t0
+        photons += n
+        done += 1
+    n1 = 200_000
+    t0 = time.perf_counter()
+    W.simulate(w.events[0], event_idx=0, seed=w.seed, p_end=n1, nthreads=1)
+    t1 = time.perf_counter() - t0
+    return {"value": photons / t, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"events 0..{done - 1} of the same 64-event workload ({photons} photons, {t:.1f} s wall on "
+                      f"{threads} threads; C restatement of qsim/CSG, OpenMP over photons, own median BVH)",
+            "value_1_thread": n1 / t1, "sample_1_thread": f"first {n1} photons of event 0, {t1:.1f} s on 1 thread",
+            "host_cpu": cpu_model()}
+
+
+def bench_config1(torch, dev, steps, regroup, batch, cpu):
+    """the 10M-photon single event (1,000 torch gensteps x 10,000 photons,
+    2,000 detectors), one simulate call per step with gensteps resident: seeding,
+    propagation, stable hit compaction into preallocated buffers."""
+    from opticks_amd import configs
+    from opticks_amd.qsim import QSim, SEventConfig
+    w = configs.config1()
+    N = w.num_photon
+    G = len(w.gensteps)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, trace_regroup_pct=regroup, refill_batch=batch))
+    gs = torch.from_numpy(w.gensteps).to(dev)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
+    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+    n_hit = sim.simulate_device(gs, photon, seq, hit, n_photon=N)
+    torch.cuda.synchronize()
+    prop_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n_hit = sim.simulate_device(gs, photon, seq, hit, n_photon=N)
+        prop_ms.append(sim.last_timing()["propagate_ms"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    S = mean_steps(seq.cpu().numpy().view(np.uint64))
+    kernel_ms = float(np.mean(prop_ms))
+    out = {"workload": "csg-boolean-10M: 1000 torch gensteps x 10000 photons, LS sphere r5m + acrylic + water + "
+                       "2000 instanced difference(union(zsphere,cylinder),cylinder) detectors",
+           "n_photon": int(N), "photons_per_s": N * steps / dt, "ms_per_step": 1e3 * dt / steps,
+           "trace_regroup_pct": regroup, "refill_batch": batch, "steps_per_photon": S, "hits": int(n_hit),
+           "roofline": roofline(kernel_ms, N, G, S, profile_json("work_k_sim_wave_csg10M.json"),
+                                pmc_summary("pmc_k_sim_wave_csg10M.json", N, "k_sim_wave"))}
+    if cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        rate, cdt = cpu_baseline(w, N, threads)
+        out["cpu_baseline"] = {"value": rate, "unit": "photons/s", "cores": threads, "kind": "port",
+                               "sample": f"all {N} photons ({cdt:.1f} s wall on {threads} threads)"}
+    out["_next_rows"] = bench_next_rows(torch, dev, sim, gs, photon, max(1, min(steps, 3)), cpu)
+    sim.close()
+    return out
 
 
 def _timed(torch, fn, steps):
@@ -381,68 +542,6 @@ def _config1_arrays():
     from opticks_amd import configs
     w = configs.config1(num_gs=1, per_gs=1)
     return w.geometry, w.physics
-
-
-def bench_batched(torch, dev, steps, tune_events=8, check_events=4):
-    """64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons in
-    the 20,000-detector geometry: autotune (block, refill batch, TLAS leaf,
-    trace regroup percent) on
-    the first events, capture the whole batch into a hipGraph, time replays.
-    History agreement: seq of the first events against the plain kernel run
-    per event (one launch per event, one lane per photon)."""
-    from opticks_amd import configs
-    from opticks_amd import _lib as L
-    from opticks_amd.qsim import QSim, SEventConfig
-    w = configs.config2()
-    N = w.num_photon
-    gs_all = np.concatenate(w.events)
-    offs = np.cumsum([0] + [len(e) for e in w.events])
-    ev = list(range(len(w.events)))
-    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
-    gs = torch.from_numpy(gs_all).to(dev)
-    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
-    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
-    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
-    t0 = time.perf_counter()
-    tune = sim.autotune(gs, offs[:tune_events + 1], ev[:tune_events], photon, seq, hit, reps=1)
-    tune_s = time.perf_counter() - t0
-    best = min(tune, key=lambda r: r["ms"])
-    b = sim.batch(gs, offs, ev, photon, seq, hit)
-    b.launch(); b.wait()
-    torch.cuda.synchronize()
-    dev_ms = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.launch()
-        ho, ms = b.wait()
-        dev_ms.append(ms)
-    dt = (time.perf_counter() - t0) / steps
-    # agreement with the plain per-event form on the first events
-    S = seq[: int(b.photon_offsets[check_events])].cpu().numpy()
-    plain = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, kernel=L.MI_KERNEL_PLAIN))
-    same = total = 0
-    for e in range(check_events):
-        a0, a1 = int(b.photon_offsets[e]), int(b.photon_offsets[e + 1])
-        ge = torch.from_numpy(w.events[e]).to(dev)
-        sq = torch.empty((a1 - a0, 2, 2), dtype=torch.int64, device=dev)
-        ph = torch.empty((a1 - a0, 4, 4), dtype=torch.float32, device=dev)
-        plain.simulate_device(ge, ph, sq, None, eventID=e, n_photon=a1 - a0)
-        same += int(np.all(sq.cpu().numpy() == S[a0:a1], axis=(1, 2)).sum())
-        total += a1 - a0
-    plain.close()
-    b.close()
-    sim.close()
-    mean_ms = float(np.mean(dev_ms))
-    return {"workload": "64 events (Poisson(200) scintillation gensteps x Poisson(5000) photons), 20000 CSG "
-                        "detectors at 19.4 m, LS r 17.7 m; one captured hipGraph replay per step",
-            "n_photon": int(N), "n_event": len(ev), "photons_per_s": N / (mean_ms * 1e-3),
-            "photons_per_s_wall": N / dt, "device_ms_per_step": mean_ms, "wall_ms_per_step": dt * 1e3,
-            "hits": int(ho[-1]),
-            "autotune": {"events": tune_events, "seconds": tune_s, "candidates": len(tune),
-                         "chosen": {k: best[k] for k in ("block_size", "refill_period", "bvh_leaf_size",
-                                                         "trace_regroup_pct", "refill_batch")},
-                         "chosen_ms": best["ms"], "slowest_ms": max(r["ms"] for r in tune)},
-            "seqhis_agreement_vs_plain_per_event": same / max(total, 1), "agreement_photons": total}
 
 
 def bench_full(torch, dev, steps):

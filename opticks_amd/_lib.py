@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libopticks_amd.so")
+# OPTICKS_AMD_LIB selects another build of the same library (the OA_STATS counters
+# build used by opticks_amd.workcount); the default is the in-tree libopticks_amd.so
+LIBPATH = os.environ.get("OPTICKS_AMD_LIB") or os.path.join(HERE, "libopticks_amd.so")
 
 MI_OK, MI_NOTHING_TO_DO = 0, 1
 MI_OUT_PHOTON, MI_OUT_SEQ, MI_OUT_HIT = 0x1, 0x2, 0x4

@@ -28,6 +28,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libopticks_amd.so")
+OUT_STATS = os.path.join(HERE, "libopticks_amd_stats.so")   # OA_STATS counters build (opticks_amd.workcount)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -75,14 +76,15 @@ def build(verbose=False, stats=False, jobs=None):
                 if verbose and errs:
                     sys.stderr.write("\n".join(errs) + "\n")
                 objs.append(os.path.join(tmpdir, u[2]))
-        tmp = OUT + ".tmp"
+        out = OUT_STATS if stats else OUT
+        tmp = out + ".tmp"
         cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write("\n".join(_filter(r.stderr)) + "\n")
             raise RuntimeError("link failed: " + " ".join(cmd))
-        os.replace(tmp, OUT)
-        return OUT
+        os.replace(tmp, out)
+        return out
     finally:
         shutil.rmtree(tmpdir, ignore_errors=True)
 

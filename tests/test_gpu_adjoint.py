@@ -103,7 +103,8 @@ def test_adjoint_closed_form_water_sphere():
     sim.close()
 
 
-FD_EVENTS = 10          # 10 events x 40M photons = 400M photons per FD evaluation
+FD_EVENTS = 40          # 40 events x 40M photons = 1.6G photons per FD evaluation
+N_GROUP = 20            # detector groups of 100 for per-detector-group directions
 
 
 @pytest.fixture(scope="module")
@@ -117,39 +118,52 @@ def fd_setup():
     sim.close()
 
 
+def hit_group(hit, n_hit):
+    """detector group (sensor identifier - 1) // (N_DET / N_GROUP) of each hit"""
+    det = (hit[:n_hit, 3, 1].contiguous().view(torch.int32) & 0xffff).long() - 1
+    return torch.div(det, N_DET // N_GROUP, rounding_mode="floor")
+
+
 def test_adjoint_vs_central_fd(fd_setup):
-    """Central differences of total hits along parameter directions v (theta -> theta(1 + s v), s = +-1%),
-    same seeds and photon streams on both sides; 400M photons as 10 events with their own streams."""
+    """Central differences (theta -> theta(1 + s v), s = +-1%) of the hits along
+    parameter directions v, with common random numbers (same seeds and photon
+    streams on both sides), over 1.6G photons as 40 events with their own streams:
+    the aggregate directions, each property block, the 8 LS-absorption bins of
+    largest yield sensitivity, a random-sign direction, and the hits of each of 20
+    detector groups along the all-parameter direction. FD error from the photons
+    whose outcome changed; adjoint error from 8 disjoint photon subsets. The
+    tolerance is 1% wherever the FD's own error is below 0.3% of its value, and
+    4 combined standard deviations elsewhere (that floor is printed)."""
     w, params, theta, sim, gs_t = fd_setup
     N = w.num_photon
     base_bnd = sim.base_bnd
     h = 0.01
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
     hit = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
-    Gtot = np.zeros(params.shape)
+    Gdet = np.zeros((N_DET, params.n_param))
     parts = np.zeros((8,) + params.shape)     # adjoint error bar: 8 disjoint photon subsets
     for ev in range(FD_EVENTS):
         n_hit = sim.simulate_device(gs_t, photon, None, hit, eventID=ev, n_photon=N)
         G = yield_jacobian(sim, params, gs_t, hit, n_hit, N_DET, event_id=ev, n_photon=N)
-        Gtot += G.sum(dim=0).double().cpu().numpy().reshape(params.shape)
+        Gdet += G.double().cpu().numpy()
         idx = hit_photon_index(hit, n_hit)
         for k in range(8):
             sel = hit[:n_hit][(idx % 8) == k].contiguous()
             Gk = yield_jacobian(sim, params, gs_t, sel, sel.shape[0], N_DET, event_id=ev, n_photon=N)
             parts[k] += Gk.sum(dim=0).double().cpu().numpy().reshape(params.shape)
+    Gtot = Gdet.sum(axis=0).reshape(params.shape)
     assert np.allclose(parts.sum(axis=0), Gtot, rtol=1e-4, atol=1e-6 * np.abs(Gtot).max())
+    Ggrp = Gdet.reshape(N_GROUP, N_DET // N_GROUP, *params.shape).sum(axis=1)
 
     rng = np.random.default_rng(7)
-    directions = {
-        "all": np.ones(params.shape),
-        "LS abs": np.zeros(params.shape), "LS ray": np.zeros(params.shape),
-        "water abs": np.zeros(params.shape), "water ray": np.zeros(params.shape),
-        "random signs": rng.choice([-1.0, 1.0], size=params.shape),
-    }
-    directions["LS abs"][0, 0] = 1
-    directions["LS ray"][0, 1] = 1
-    directions["water abs"][1, 0] = 1
-    directions["water ray"][1, 1] = 1
+    directions = {"all": np.ones(params.shape), "random signs": rng.choice([-1.0, 1.0], size=params.shape)}
+    for name, (blk, prop) in {"LS abs": (0, 0), "LS ray": (0, 1), "water abs": (1, 0), "water ray": (1, 1)}.items():
+        directions[name] = np.zeros(params.shape)
+        directions[name][blk, prop] = 1
+    sens = np.abs(Gtot[0, 0] * theta[0, 0])
+    for b in np.argsort(sens)[::-1][:8]:
+        directions[f"LS abs bin {b}"] = np.zeros(params.shape)
+        directions[f"LS abs bin {b}"][0, 0, b] = 1
     report = {}
     for name, v in directions.items():
         dtheta = theta * v                      # d theta / d s along s -> theta (1 + s v)
@@ -157,29 +171,52 @@ def test_adjoint_vs_central_fd(fd_setup):
         adj_parts = (parts * dtheta).sum(axis=(1, 2, 3))
         sig_adj = float(adj_parts.std(ddof=1) * np.sqrt(len(adj_parts)))
         dH, n_changed = 0, 0
+        dG = np.zeros(N_GROUP)
+        nG = np.zeros(N_GROUP)
         bnds = [params.apply(base_bnd, (theta * (1.0 + sgn * h * v)).astype(np.float32)) for sgn in (+1, -1)]
         for ev in range(FD_EVENTS):
-            masks = []
+            masks, groups = [], []
             for bnd in bnds:
                 sim.set_bnd(bnd)
                 nh = sim.simulate_device(gs_t, photon, None, hit, eventID=ev, n_photon=N)
                 m = torch.zeros(N, dtype=torch.bool, device=DEV)
                 m[hit_photon_index(hit, nh)] = True
                 masks.append(m)
+                if name == "all":
+                    g = torch.full((N,), -1, dtype=torch.int64, device=DEV)
+                    g[hit_photon_index(hit, nh)] = hit_group(hit, nh)
+                    groups.append(g)
             dH += masks[0].sum().item() - masks[1].sum().item()
             n_changed += (masks[0] ^ masks[1]).sum().item()
+            if name == "all":
+                a, b = groups
+
+                def count(x):
+                    return torch.bincount(x[x >= 0], minlength=N_GROUP)[:N_GROUP].double().cpu().numpy()
+                dG += count(a) - count(b)
+                ch = a != b                      # photons whose group (or hit/no-hit) changed
+                nG += count(a[ch]) + count(b[ch])
         fd = dH / (2 * h)
         sig_fd = np.sqrt(n_changed) / (2 * h)
-        sig = np.hypot(sig_fd, sig_adj)
         report[name] = (adj, fd, sig_adj, sig_fd)
-        print(f"FD check {name:13s} adjoint {adj:12.1f} +- {sig_adj:8.1f}  fd {fd:12.1f} +- {sig_fd:8.1f}"
-              f"  rel diff {(adj - fd) / fd:+.4f}  ({FD_EVENTS * N / 1e6:.0f}M photons)", flush=True)
-        assert abs(adj - fd) < 4 * sig + 1e-9, (name, adj, fd, sig)
+        if name == "all":
+            for g in range(N_GROUP):
+                # adjoint error of a group taken as the all-detector error / sqrt(groups) (the FD error dominates)
+                report[f"detector group {g}"] = (float((Ggrp[g] * dtheta).sum()), dG[g] / (2 * h),
+                                                 sig_adj / np.sqrt(N_GROUP), np.sqrt(nG[g]) / (2 * h))
     sim.set_bnd(base_bnd)
-    # the directions that carry the yield's sensitivity: within 1% (FD error there is 0.3-0.6%)
-    for name in ("all", "LS abs", "water abs"):
-        adj, fd, sa, sf = report[name]
-        assert abs(adj - fd) <= 0.01 * abs(fd), (name, adj, fd, sa, sf)
+    strict = []
+    for name, (adj, fd, sa, sf) in report.items():
+        rel = sf / abs(fd) if fd else np.inf
+        tag = "1%" if rel < 0.003 else "4 sigma"
+        print(f"FD check {name:20s} adjoint {adj:12.1f} +- {sa:8.1f}  fd {fd:12.1f} +- {sf:8.1f}  "
+              f"rel diff {(adj - fd) / fd:+.4f}  FD error {rel:.4f}  tolerance {tag}", flush=True)
+        if tag == "1%":
+            strict.append(name)
+            assert abs(adj - fd) <= 0.01 * abs(fd), (name, adj, fd, sa, sf)
+        else:
+            assert abs(adj - fd) < 4 * np.hypot(sa, sf) + 1e-9, (name, adj, fd, sa, sf)
+    assert {"all", "LS abs", "water abs"} <= set(strict), strict
 
 
 def test_autograd_wrapper_matches_jacobian():
