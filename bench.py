@@ -255,12 +255,21 @@ def main():
     ap.add_argument("--no-full", action="store_true", help="skip the 256-event forward+adjoint graph")
     ap.add_argument("--no-next", action="store_true", help="skip the hit merge / lite / Cerenkov / simtrace rows")
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
+    ap.add_argument("--pmc-pass", choices=["batched64", "csg10M", "full256"], default=None,
+                    help="counter-collection pass: two eager launches of one workload's propagation with fixed "
+                         "launch settings (no autotuner, no graph), then the copy calibration; no JSON line")
+    ap.add_argument("--block", type=int, default=128, help="--pmc-pass: block size")
+    ap.add_argument("--pmc-regroup", type=int, default=0, help="--pmc-pass: trace_regroup_pct")
+    ap.add_argument("--pmc-batch", type=int, default=16, help="--pmc-pass: refill_batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "events would shard per GPU with no exchange until the hit gather", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_pass:
+        pmc_pass(args)
+        return
 
     import torch
     from opticks_amd.qsim import QSim, SEventConfig
@@ -358,6 +367,49 @@ def main():
         if not args.no_full:
             line["full_batched_adjoint"] = bench_full(torch, dev, max(1, min(args.steps, 2)))
     print(json.dumps(line), flush=True)
+
+
+def pmc_pass(args):
+    """Two eager launches of one workload's propagation with the given launch settings
+    and, with --copy-calibration-mib, one device copy of known size: the dispatches a
+    rocprofv3 --pmc run reads (opticks_amd.pmc summarizes them)."""
+    import torch
+    from opticks_amd import configs
+    from opticks_amd.qsim import QSim, SEventConfig
+    dev = torch.device("cuda:0")
+    cfg = dict(block_size=args.block, trace_regroup_pct=args.pmc_regroup, refill_batch=args.pmc_batch)
+    if args.pmc_pass == "csg10M":
+        w = configs.config1()
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, **cfg))
+        gs = torch.from_numpy(w.gensteps).to(dev)
+        N = w.num_photon
+        photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+        seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
+        hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+        for _ in range(2):
+            sim.simulate_device(gs, photon, seq, hit, n_photon=N)
+    else:
+        w = configs.config2() if args.pmc_pass == "batched64" else configs.config4()
+        gs_all = np.concatenate(w.events).astype(np.float32)
+        offs = np.cumsum([0] + [len(e) for e in w.events])
+        ev = list(range(len(w.events)))
+        N = w.num_photon
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, **cfg))
+        gs = torch.from_numpy(gs_all).to(dev)
+        photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
+        seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
+        hit = torch.empty((N // 4, 4, 4), dtype=torch.float32, device=dev)
+        for _ in range(2):
+            sim.simulate_batch(gs, offs, ev, photon, seq, hit)
+    torch.cuda.synchronize()
+    print(f"pmc pass {args.pmc_pass}: {N} photons, settings {cfg}, propagate "
+          f"{sim.last_timing()['propagate_ms']:.3f} ms", flush=True)
+    if args.copy_calibration_mib > 0:
+        src = torch.ones(args.copy_calibration_mib * 1024 * 256, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+    sim.close()
 
 
 def cpu_baseline_events(w, seconds):
