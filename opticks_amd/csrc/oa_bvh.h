@@ -213,6 +213,13 @@ struct Builder {
     uint32_t build(std::vector<Item> v)
     {
         const int n = (int)v.size();
+        if (n == 0) {                   /* no items: a root whose four slots are empty */
+            uint32_t idx = (uint32_t)nodes.size();
+            nodes.push_back(BvhNode());
+            fill(nodes[idx], 0, nullptr, nullptr, nullptr);
+            nodes[idx].depth = 0;
+            return idx;
+        }
         if (n <= leaf_size) {
             uint32_t idx = (uint32_t)nodes.size();
             nodes.push_back(BvhNode());

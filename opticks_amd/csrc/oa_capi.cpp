@@ -122,6 +122,8 @@ struct mi_sim {
     /* geometry */
     int64_t n_inst = 0, n_prim = 0, n_tlas = 0;
     uint32_t tlas_depth = 0, blas_depth = 0, blas_stack = 0;
+    uint32_t front_leaf = OA_BVH_EMPTY;             /* items of the front instances (build_tlas) */
+    int32_t tlas_empty = 0;                         /* every instance is a front instance */
     std::vector<uint32_t> solid_blas_root;
     std::vector<uint32_t> h_inst_blas_root;
     std::vector<std::array<double, 6>> solid_box;   /* padded solid-frame prim box per solid */
@@ -318,17 +320,64 @@ static void instance_items(const mi_sim* sim, std::vector<oa_bvh::Item>& titems)
     }
 }
 
+static double box_area(const oa_bvh::Item& it)
+{
+    const double a = it.hi[0] - it.lo[0], b = it.hi[1] - it.lo[1], c = it.hi[2] - it.lo[2];
+    return a * b + b * c + c * a;
+}
+
+/*
+ * Front instances: the (at most OA_MAX_FRONT) instances whose world box has at
+ * least a quarter of the surface area of the box around all instances -- the
+ * world volume and the large shells inside it, which nearly every ray meets.
+ * They stay out of the TLAS; every trace tests them first (their items follow the
+ * TLAS items, reached through SimParams.front_leaf), so the whole wave runs them
+ * together and the TLAS walk starts with the nearest of their hits as its tmax.
+ * The nearest hit is chosen by (t, instance, prim), not by test order, so which
+ * instances are front changes no result.
+ */
+static void split_front(const std::vector<oa_bvh::Item>& all, std::vector<oa_bvh::Item>& rest,
+                        std::vector<uint32_t>& front)
+{
+    oa_bvh::Item u;
+    for (int k = 0; k < 3; ++k) { u.lo[k] = 1e300; u.hi[k] = -1e300; }
+    for (const auto& it : all)
+        for (int k = 0; k < 3; ++k) { u.lo[k] = std::min(u.lo[k], it.lo[k]); u.hi[k] = std::max(u.hi[k], it.hi[k]); }
+    const double ua = all.empty() ? 0.0 : box_area(u);
+    std::vector<std::pair<double, uint32_t>> big;
+    for (size_t i = 0; i < all.size(); ++i) {
+        const double a = box_area(all[i]);
+        if (a >= 0.25 * ua) big.push_back({ -a, (uint32_t)i });
+    }
+    std::sort(big.begin(), big.end());
+    if (big.size() > OA_MAX_FRONT) big.resize(OA_MAX_FRONT);
+    std::vector<char> is_front(all.size(), 0);
+    front.clear();
+    for (const auto& b : big) { is_front[b.second] = 1; front.push_back(all[b.second].id); }
+    std::sort(front.begin(), front.end());
+    rest.clear();
+    for (size_t i = 0; i < all.size(); ++i)
+        if (!is_front[i]) rest.push_back(all[i]);
+}
+
 /* host SAH build of the TLAS over instance boxes with the given leaf size, uploaded
    into new buffers that replace the handle's only when every step succeeded */
 static int build_tlas(mi_sim* sim, int leaf)
 {
-    std::vector<oa_bvh::Item> titems;
-    instance_items(sim, titems);
+    std::vector<oa_bvh::Item> all, titems;
+    std::vector<uint32_t> front;
+    instance_items(sim, all);
+    split_front(all, titems, front);
     std::vector<BvhNode> tlas;
     std::vector<uint32_t> tlas_items;
     oa_bvh::Builder tb(tlas, tlas_items, leaf);
     uint32_t troot = tb.build(titems);
     if (troot != 0) return fail(MI_ERR_GEOMETRY, "TLAS root is not node 0");
+    uint32_t front_leaf = OA_BVH_EMPTY;
+    if (!front.empty()) {
+        front_leaf = OA_BVH_LEAF_BIT | ((uint32_t)front.size() << 24) | (uint32_t)tlas_items.size();
+        tlas_items.insert(tlas_items.end(), front.begin(), front.end());
+    }
     /* deepest stack a walk can reach: TLAS pushes, then the BLAS walk on top */
     if (tb.stack_need + sim->blas_stack + 1 > OA_STACK_DEPTH)
         return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
@@ -346,6 +395,8 @@ static int build_tlas(mi_sim* sim, int leaf)
     nt.release(); ni.release();
     sim->tlas_depth = tb.max_depth;
     sim->n_tlas = (int64_t)tlas.size();
+    sim->front_leaf = front_leaf;
+    sim->tlas_empty = titems.empty() ? 1 : 0;
     sim->generation++;
     return MI_OK;
 }
@@ -589,6 +640,7 @@ static SimParams base_params(const mi_sim* sim)
     std::memset(&P, 0, sizeof(P));
     P.node = sim->node.p; P.itra = sim->itra.p; P.plan = sim->plan.p; P.ext_leaves = sim->ext_leaves; P.inst = sim->inst.p; P.prim = sim->prim.p;
     P.tlas = sim->tlas.p; P.tlas_items = sim->tlas_items.p; P.blas = sim->blas.p; P.blas_items = sim->blas_items.p;
+    P.front_leaf = sim->front_leaf; P.tlas_empty = sim->tlas_empty;
     P.bnd = sim->bnd.p; P.optical = sim->optical.p; P.icdf = sim->icdf.p;
     P.n_wl = sim->n_wl; P.wl_low = sim->wl_low; P.wl_step = sim->wl_step;
     P.n_icdf = sim->n_icdf; P.n_icdf_layer = sim->n_icdf_layer; P.icdf_hd = sim->icdf_hd;

@@ -24,6 +24,7 @@
 #define OA_STACK_DEPTH  48
 #define OA_BVH_WIDTH    4
 #define OA_CSG_MAX_HEIGHT 7
+#define OA_MAX_FRONT    4       /* instances tested before the TLAS walk (world-sized boxes) */
 
 /* Scratch of the no-allocation simulate entry point: one statically sized
    device array per GPU (oa_kernels.hip), laid out as 256-byte aligned pieces.
@@ -96,6 +97,8 @@ struct SimParams {
     const uint32_t* tlas_items;   /* instance indices */
     const BvhNode* blas;
     const uint32_t* blas_items;   /* prim indices (global) */
+    uint32_t front_leaf;          /* leaf link to the front instances' items (tested first), or empty */
+    int32_t tlas_empty;           /* the TLAS holds no instance: every instance is front */
     /* physics */
     const float4*  bnd;
     const int32_t* optical;       /* (nbnd*4, 4) */

@@ -667,8 +667,11 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
     uint32_t best_inst = 0xffffffffu, best_prim = 0u;
     Isect best_is; best_is.x = best_is.y = best_is.z = best_is.w = 0.f;
     int sp = 0;
-    uint32_t cur = 0u;                       /* TLAS root node */
-    uint32_t leaf = OA_BVH_EMPTY;
+    uint32_t cur = P.tlas_empty ? OA_BVH_EMPTY : 0u;    /* TLAS root node */
+    /* the first leaf phase tests the front instances (world-sized boxes kept out
+       of the TLAS) on every lane together; the walk then starts with their
+       nearest hit as tmax */
+    uint32_t leaf = P.front_leaf;
     bool done = false;
 #ifdef OA_STATS
     uint32_t n1 = 0, n2 = 0;
