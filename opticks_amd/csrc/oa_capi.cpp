@@ -25,6 +25,7 @@
 #include "oa_layout.h"
 #include "oa_portable.h"
 #include "oa_bvh.h"
+#include "oa_bounds.h"
 #include "oa_kernels.h"
 #include "oa_merge.h"
 
@@ -128,14 +129,17 @@ struct mi_sim {
     std::vector<uint32_t> h_inst_blas_root;
     std::vector<std::array<double, 6>> solid_box;   /* padded solid-frame prim box per solid */
     std::vector<float> h_o2w;                       /* (n_inst,16) current instance transforms */
+    /* host copies for the TLAS leaf boxes: CSGNode, itra and CSGPrim arrays, prim range of each solid */
+    std::vector<float> h_node, h_itra, h_prim;
+    std::vector<std::pair<int32_t, int32_t>> h_solid_prims;
     uint64_t generation = 0;                        /* bumped when a device buffer may move */
     int32_t ext_leaves = 0;                         /* geometry uses the extended leaf set */
     DBuf<float4> node, itra, plan;
     DBuf<InstRec> inst;
     DBuf<PrimRec> prim;
     DBuf<BvhNode> tlas, blas;
-    DBuf<uint32_t> tlas_items, blas_items, inst_blas_root;
-    DBuf<float> o2w, inst_box;
+    DBuf<uint32_t> tlas_items, blas_items;
+    DBuf<float> inst_box;
 
     /* physics */
     DBuf<float4> bnd;
@@ -293,27 +297,28 @@ static int validate_geometry(const mi_geometry* g, const mi_physics* ph)
     return MI_OK;
 }
 
-/* padded world boxes of the instances from the current transforms */
+/* padded world boxes of the instances from the current transforms: per prim of
+   the instance's solid, the exact world bound of its CSG leaves (oa_bounds.h)
+   intersected with the transformed corners of its CSGPrim AABB, joined over the
+   solid's prims */
 static void instance_items(const mi_sim* sim, std::vector<oa_bvh::Item>& titems)
 {
     const int64_t ni = sim->n_inst;
+    const int64_t n_tran = (int64_t)sim->h_itra.size() / 16;
     titems.resize(ni);
     for (int64_t i = 0; i < ni; ++i) {
         const float* m = sim->h_o2w.data() + 16 * i;
         int32_t gas;
         std::memcpy(&gas, m + 7, 4);
-        const std::array<double, 6>& bx = sim->solid_box[gas];
+        const std::pair<int32_t, int32_t> pr = sim->h_solid_prims[gas];
+        const oa_bounds::Box all = oa_bounds::instance_box(m, sim->h_node.data(),
+                                                           sim->h_itra.empty() ? nullptr : sim->h_itra.data(), n_tran,
+                                                           sim->h_prim.data(), pr.first, pr.second);
         oa_bvh::Item& it = titems[i];
-        for (int k = 0; k < 3; ++k) { it.lo[k] = 1e300; it.hi[k] = -1e300; }
-        for (int c = 0; c < 8; ++c) {
-            double v[3] = { (c & 1) ? bx[3] : bx[0], (c & 2) ? bx[4] : bx[1], (c & 4) ? bx[5] : bx[2] };
-            for (int k = 0; k < 3; ++k) {
-                double p = v[0] * m[k] + v[1] * m[4 + k] + v[2] * m[8 + k] + m[12 + k];
-                it.lo[k] = std::min(it.lo[k], p); it.hi[k] = std::max(it.hi[k], p);
-            }
-        }
         for (int k = 0; k < 3; ++k) {
-            double pad = 1e-3 * (it.hi[k] - it.lo[k]) + 1e-2;
+            it.lo[k] = all.lo[k]; it.hi[k] = all.hi[k];
+            if (!(it.hi[k] >= it.lo[k])) { it.lo[k] = it.hi[k] = 0.5 * (all.lo[k] + all.hi[k]); }
+            const double pad = 1e-3 * (it.hi[k] - it.lo[k]) + 1e-2;
             it.lo[k] -= pad; it.hi[k] += pad;
         }
         it.id = (uint32_t)i;
@@ -440,6 +445,16 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     sim->h_inst_blas_root.resize(ni);
     sim->solid_box = solid_box;
     sim->h_o2w.assign(g->inst, g->inst + 16 * ni);
+    sim->h_node.assign(g->node, g->node + 16 * g->n_node);
+    if (g->itra && g->n_tran > 0) sim->h_itra.assign(g->itra, g->itra + 16 * g->n_tran);
+    sim->h_prim.assign(g->prim, g->prim + 16 * g->n_prim);
+    sim->h_solid_prims.resize(g->n_solid);
+    for (int64_t so = 0; so < g->n_solid; ++so) {
+        int32_t np, po;
+        std::memcpy(&np, sb + 48 * so + 16, 4);
+        std::memcpy(&po, sb + 48 * so + 20, 4);
+        sim->h_solid_prims[so] = { po, np };
+    }
     for (int64_t i = 0; i < ni; ++i) {
         const float* m = g->inst + 16 * i;
         int32_t gas, ident;
@@ -482,8 +497,6 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     HIPCHECK(upload(sim->prim, prim.data(), prim.size(), s));
     HIPCHECK(upload(sim->blas, blas.data(), blas.size(), s));
     HIPCHECK(upload(sim->blas_items, blas_items.data(), blas_items.size(), s));
-    HIPCHECK(upload(sim->o2w, g->inst, (size_t)ni * 16, s));
-    HIPCHECK(upload(sim->inst_blas_root, sim->h_inst_blas_root.data(), sim->h_inst_blas_root.size(), s));
     HIPCHECK(sim->inst_box.reserve((size_t)ni * 6));
     return MI_OK;
 }
@@ -604,7 +617,7 @@ void mi_sim_destroy(mi_sim* sim)
     if (sim->stream) (void)hipStreamSynchronize(sim->stream);
     sim->node.release(); sim->itra.release(); sim->plan.release(); sim->inst.release(); sim->prim.release();
     sim->tlas.release(); sim->blas.release(); sim->tlas_items.release(); sim->blas_items.release();
-    sim->inst_blas_root.release(); sim->o2w.release(); sim->inst_box.release();
+    sim->inst_box.release();
     sim->bnd.release(); sim->optical.release(); sim->icdf.release(); sim->precooked.release();
     sim->gs.release(); sim->prefix.release(); sim->block_total.release(); sim->evbuf.release();
     sim->word_prefix.release(); sim->ev_id.release(); sim->words.release(); sim->next.release();
@@ -1185,10 +1198,16 @@ int mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst)
         rec[i].m[11] = old[i].m[11];
     }
     HIPCHECK(upload(sim->inst, rec.data(), rec.size(), s));
-    HIPCHECK(upload(sim->o2w, inst, (size_t)n_inst * 16, s));
     sim->h_o2w.assign(inst, inst + 16 * n_inst);
-    HIPCHECK(oa_launch::refit(sim->o2w.p, sim->blas.p, sim->inst_blas_root.p, n_inst, sim->inst_box.p,
-                              sim->tlas.p, sim->n_tlas, sim->tlas_depth, sim->tlas_items.p, s));
+    /* the instances' world boxes as a host build makes them (exact leaf bounds), rounded
+       outward to float; the TLAS node boxes are then refitted level by level on the GPU */
+    std::vector<oa_bvh::Item> items;
+    instance_items(sim, items);
+    std::vector<float> boxes((size_t)n_inst * 6);
+    for (int64_t i = 0; i < n_inst; ++i)
+        oa_bvh::Builder::store(boxes.data() + 6 * i, boxes.data() + 6 * i + 3, items[i].lo, items[i].hi);
+    HIPCHECK(upload(sim->inst_box, boxes.data(), boxes.size(), s));
+    HIPCHECK(oa_launch::refit(sim->tlas.p, sim->n_tlas, sim->tlas_depth, sim->tlas_items.p, sim->inst_box.p, s));
     HIPCHECK(hipStreamSynchronize(s));
     return MI_OK;
 }

@@ -20,9 +20,8 @@ hipError_t event_hit_offsets(const unsigned long long* d_words, const int32_t* d
                              int64_t n_words, int64_t* d_hit_offsets, hipStream_t s);
 hipError_t simtrace(const SimParams& P, float4* out, hipStream_t s);
 hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out, hipStream_t s);
-hipError_t refit(const float* d_o2w, const BvhNode* d_blas, const uint32_t* d_inst_blas_root, int64_t n_inst,
-                 float* d_inst_box, BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth,
-                 const uint32_t* d_tlas_items, hipStream_t s);
+hipError_t refit(BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth, const uint32_t* d_tlas_items,
+                 const float* d_inst_box, hipStream_t s);
 /* adjoint over hits [begin, begin+n) of d_hit, n = n_max or, with d_hit_off (per-event
    hit offsets on the device, batched events), the device count clipped to hit_cap */
 hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_t n_mat, const int32_t* d_sample_bin,
