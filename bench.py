@@ -150,10 +150,14 @@ def profile_json(name):
         return json.load(f)
 
 
-def bench_adjoint(torch, dev, steps, regroup=0, batch=1):
-    """Forward then path-replay adjoint on the binned-length workload: config1
-    geometry and photons, LS and water absorption/Rayleigh lengths as 4 x 64
-    bins (x(1+U(-0.2,0.2)), seed 42); gradient (2000 detectors, 256) f32."""
+def bench_adjoint(torch, dev, steps, regroup=0, batch=1, tape=32):
+    """Forward then yield adjoint on the binned-length workload: config1 geometry and
+    photons, LS and water absorption/Rayleigh lengths as 4 x 64 bins (x(1+U(-0.2,0.2)),
+    seed 42); gradient (2000 detectors, 256) f32. Two ways: the forward keeps each
+    photon's segment records (adjoint_tape records per photon; 32 holds every photon at
+    max_bounce 31) and the adjoint scores hits from them, replaying only photons with
+    more segments; and the path replay of every hit from a plain forward. adjoint_forward_ratio = (taped forward - plain
+    forward + adjoint) / plain forward: all the time the gradient adds."""
     from opticks_amd import configs
     from opticks_amd.adjoint import yield_jacobian
     from opticks_amd.qsim import QSim, SEventConfig
@@ -161,27 +165,39 @@ def bench_adjoint(torch, dev, steps, regroup=0, batch=1):
     params = w.meta["params"]
     N = w.num_photon
     n_det = w.meta["num_det"]
-    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, trace_regroup_pct=regroup, refill_batch=batch))
+    base = dict(seed=w.seed, trace_regroup_pct=regroup, refill_batch=batch)
+    sim = QSim(w.geometry, w.physics, SEventConfig(**base))
     gs = torch.from_numpy(w.gensteps).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     hit = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     n_hit = sim.simulate_device(gs, photon, None, hit, n_photon=N)
     G = yield_jacobian(sim, params, gs, hit, n_hit, n_det, n_photon=N)
-    fwd, adj, rep, red = [], [], [], []
+    fwd, rep_adj = [], []
     for _ in range(steps):
         n_hit = sim.simulate_device(gs, photon, None, hit, n_photon=N)
         fwd.append(sim.last_timing()["total_ms"])
         sim.adjoint_device(gs, hit, n_hit, params.c_struct(n_det), G, n_photon=N)
+        rep_adj.append(sim.last_adjoint_timing()["total_ms"])
+    sim.set_config(SEventConfig(adjoint_tape=tape, **base))
+    n_hit = sim.simulate_device(gs, photon, None, hit, n_photon=N)
+    sim.adjoint_device(gs, hit, n_hit, params.c_struct(n_det), G, n_photon=N)
+    fwd_t, adj, sco, red = [], [], [], []
+    for _ in range(steps):
+        n_hit = sim.simulate_device(gs, photon, None, hit, n_photon=N)
+        fwd_t.append(sim.last_timing()["total_ms"])
+        sim.adjoint_device(gs, hit, n_hit, params.c_struct(n_det), G, n_photon=N)
         t = sim.last_adjoint_timing()
-        adj.append(t["total_ms"]); rep.append(t["replay_ms"]); red.append(t["reduce_ms"])
+        adj.append(t["total_ms"]); sco.append(t["replay_ms"]); red.append(t["reduce_ms"])
     torch.cuda.synchronize()
     sim.close()
-    f, a = float(np.mean(fwd)), float(np.mean(adj))
+    f, ft, a, ra = float(np.mean(fwd)), float(np.mean(fwd_t)), float(np.mean(adj)), float(np.mean(rep_adj))
     return {"workload": "csg-boolean-10M with LS/water absorption+Rayleigh lengths as 4x64 bins "
                         "(256 parameters), 2000 detectors",
-            "trace_regroup_pct": regroup, "refill_batch": batch, "n_photon": int(N), "hits": int(n_hit), "n_param": int(params.n_param), "n_det": int(n_det),
-            "forward_ms": f, "adjoint_ms": a, "replay_ms": float(np.mean(rep)), "reduce_ms": float(np.mean(red)),
-            "adjoint_forward_ratio": a / f}
+            "trace_regroup_pct": regroup, "refill_batch": batch, "n_photon": int(N), "hits": int(n_hit),
+            "n_param": int(params.n_param), "n_det": int(n_det), "adjoint_tape": tape,
+            "forward_ms": f, "forward_taped_ms": ft, "adjoint_ms": a, "score_ms": float(np.mean(sco)),
+            "reduce_ms": float(np.mean(red)), "adjoint_forward_ratio": (ft - f + a) / f,
+            "replay_adjoint_ms": ra, "replay_adjoint_forward_ratio": ra / f}
 
 
 def batched_workload():
@@ -596,12 +612,15 @@ def _config1_arrays():
     return w.geometry, w.physics
 
 
-def bench_full(torch, dev, steps):
+def bench_full(torch, dev, steps, tape=32):
     """256 events (Poisson(200) scintillation gensteps x Poisson(5000) photons, about
     256M photons) in the 20,000-detector geometry with the 256 binned lengths:
-    forward hits and the yield adjoint of every hit in one captured graph."""
+    forward hits and the yield adjoint of every hit in one captured graph, the
+    forward keeping `tape` segment records per photon for the adjoint. The plain
+    forward (no records) is timed eagerly beside it for the ratio."""
     from opticks_amd import configs
     from opticks_amd.qsim import QSim, SEventConfig
+    torch.cuda.empty_cache()          # the segment tape of 255M photons is 98 GB at 32 records
     w = configs.config4()
     params = w.meta["params"]
     n_det = w.meta["num_det"]
@@ -609,7 +628,7 @@ def bench_full(torch, dev, steps):
     gs_all = np.concatenate(w.events)
     offs = np.cumsum([0] + [len(e) for e in w.events])
     ev = list(range(len(w.events)))
-    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, block_size=64))
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, block_size=64, adjoint_tape=tape))
     gs = torch.from_numpy(gs_all).to(dev)
     photon = torch.empty((N, 4, 4), dtype=torch.float32, device=dev)
     seq = torch.empty((N, 2, 2), dtype=torch.int64, device=dev)
@@ -626,16 +645,25 @@ def bench_full(torch, dev, steps):
         fwd.append(t["forward_ms"]); adj.append(t["adjoint_ms"]); tot.append(ms)
     steps_per_photon = mean_steps(seq[: min(N, 20_000_000)].cpu().numpy().view(np.uint64))
     b.close()
+    # the plain forward (no segment records): one eager launch of the same batch, HIP
+    # events; its propagation kernel is the roofline's
+    sim.set_config(SEventConfig(seed=w.seed, block_size=64))
+    sim.simulate_batch(gs, offs, ev, photon, seq, hit)
+    kernel_ms = sim.last_timing()["propagate_ms"]
+    plain_ms = sim.last_timing()["total_ms"]
     sim.close()
     f, a, T = float(np.mean(fwd)), float(np.mean(adj)), float(np.mean(tot))
     B = algorithmic_bytes(N, len(gs_all), steps_per_photon, 0.0)
     return {"workload": "256 events (Poisson(200) scintillation gensteps x Poisson(5000) photons), 20000 CSG "
                         "detectors, LS/water lengths as 4x64 bins; forward + adjoint in one captured graph",
             "n_photon": int(N), "n_event": len(ev), "hits": int(ho[-1]), "n_param": int(params.n_param),
-            "n_det": int(n_det), "forward_ms": f, "adjoint_ms": a, "graph_ms": T,
-            "photons_per_s_forward": N / (f * 1e-3), "photons_per_s_forward_plus_adjoint": N / (T * 1e-3),
-            "adjoint_forward_ratio": a / f, "steps_per_photon": steps_per_photon,
-            "forward_alg_hbm_gb_s": B / (f * 1e-3) / 1e9}
+            "n_det": int(n_det), "adjoint_tape": tape, "forward_taped_ms": f, "adjoint_ms": a, "graph_ms": T,
+            "forward_plain_ms": plain_ms, "photons_per_s_forward": N / (plain_ms * 1e-3),
+            "photons_per_s_forward_plus_adjoint": N / (T * 1e-3),
+            "adjoint_forward_ratio": (T - plain_ms) / plain_ms, "steps_per_photon": steps_per_photon,
+            "forward_alg_hbm_gb_s": B / (plain_ms * 1e-3) / 1e9,
+            "roofline": roofline(kernel_ms, N, len(gs_all), steps_per_photon, profile_json("work_k_sim_wave_full256.json"),
+                                 pmc_summary("pmc_k_sim_wave_full256.json", N, "k_sim_wave"))}
 
 
 if __name__ == "__main__":

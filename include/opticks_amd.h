@@ -154,6 +154,10 @@ typedef struct mi_config {
                                    (0 = once none do; 0..100; results do not depend on it) */
     int32_t  refill_batch;     /* wavefront kernel: a wave refills once this many of its lanes
                                    are idle, or all are (1..64, default 16; results do not depend on it) */
+    int32_t  adjoint_tape;     /* wavefront forwards keep up to this many segment records (12 B)
+                                   per photon for a following adjoint, which then scores hits from
+                                   them and replays only photons with more segments (0..64, default
+                                   0 = none: the adjoint replays every hit; gradients are the same) */
 } mi_config;
 
 typedef struct mi_sim mi_sim;

@@ -44,7 +44,8 @@ class MiConfig(C.Structure):
                 ("precooked", C.c_void_p), ("precooked_ni", C.c_int64), ("precooked_nk", C.c_int64),
                 ("output_mask", C.c_uint32), ("max_slot", C.c_int64), ("kernel", C.c_int32),
                 ("block_size", C.c_int32), ("refill_period", C.c_int32), ("bvh_leaf_size", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("trace_regroup_pct", C.c_int32), ("refill_batch", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("trace_regroup_pct", C.c_int32), ("refill_batch", C.c_int32),
+                ("adjoint_tape", C.c_int32)]
 
 
 class MiEventOut(C.Structure):

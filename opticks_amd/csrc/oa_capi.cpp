@@ -168,6 +168,17 @@ struct mi_sim {
     DBuf<uint64_t> seq;
     std::vector<float> h_photon, h_hit, h_simtrace;
     std::vector<uint64_t> h_seq;
+    /* adjoint tape of the last wavefront forward made with cfg.adjoint_tape > 0, and
+       what it was made for: an adjoint of the same event, photons and gensteps uses it */
+    DBuf<uint32_t> tape, tape_flag;
+    DBuf<uint8_t> tape_n;
+    struct TapeKey {
+        bool valid = false;
+        int32_t event_id = 0, k = 0;
+        int64_t photon_offset = 0, n_photon = 0, n_gs = 0;
+        const void* gs = nullptr;
+        uint64_t generation = 0;
+    } tape_key;
     /* adjoint */
     DBuf<int32_t> adj_mat_block, adj_sample_bin;
     DBuf<float> adj_rows, adj_comp;
@@ -207,6 +218,7 @@ void mi_config_default(mi_config* c)
     c->waves_per_cu = 0;
     c->trace_regroup_pct = 0;
     c->refill_batch = 16;
+    c->adjoint_tape = 0;
 }
 
 const char* mi_last_error(void) { return g_err.c_str(); }
@@ -219,6 +231,7 @@ static int check_config(const mi_config* c)
     if (c->bvh_leaf_size < 1 || c->bvh_leaf_size > 64) return fail(MI_ERR_ARG, "bvh_leaf_size must be 1..64");
     if (c->trace_regroup_pct < 0 || c->trace_regroup_pct > 100) return fail(MI_ERR_ARG, "trace_regroup_pct must be 0..100");
     if (c->refill_batch < 1 || c->refill_batch > 64) return fail(MI_ERR_ARG, "refill_batch must be 1..64");
+    if (c->adjoint_tape < 0 || c->adjoint_tape > 64) return fail(MI_ERR_ARG, "adjoint_tape must be 0..64");
     if (c->max_bounce < 0) return fail(MI_ERR_ARG, "max_bounce < 0");
     if (c->kernel != MI_KERNEL_PLAIN && c->kernel != MI_KERNEL_WAVEFRONT) return fail(MI_ERR_ARG, "unknown kernel");
     if (c->rng_kind != MI_RNG_PHILOX && c->rng_kind != MI_RNG_PRECOOKED && c->rng_kind != MI_RNG_XORWOW)
@@ -624,6 +637,7 @@ void mi_sim_destroy(mi_sim* sim)
     sim->seed.release(); sim->gs_event.release();
     sim->photon.release(); sim->hit.release(); sim->seq.release();
     sim->adj_mat_block.release(); sim->adj_sample_bin.release(); sim->adj_rows.release(); sim->adj_comp.release();
+    sim->tape.release(); sim->tape_n.release(); sim->tape_flag.release();
     sim->adj_key.release(); sim->adj_val.release(); sim->adj_key2.release(); sim->adj_val2.release();
     sim->adj_tmp.release(); sim->adj_seg.release();
     sim->m_flags.release(); sim->m_sel.release(); sim->m_vals.release(); sim->m_vals2.release();
@@ -809,6 +823,14 @@ static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gen
     } else {
         slices.emplace_back(0, n_gs);
     }
+    /* adjoint tape: wavefront kernel, full photon records (not the lite form) */
+    const int32_t K = sim->cfg.adjoint_tape;
+    const bool taping = K > 0 && sim->cfg.kernel == MI_KERNEL_WAVEFRONT && !d_lite;
+    sim->tape_key.valid = false;
+    if (taping) {
+        HIPCHECK(sim->tape.reserve((size_t)n_photon * 3 * (size_t)K));
+        HIPCHECK(sim->tape_n.reserve((size_t)n_photon));
+    }
     int64_t hits = 0;
     double prop_ms = 0.0, comp_ms = 0.0;
     for (auto& sl : slices) {
@@ -838,6 +860,11 @@ static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gen
         P.photon = d_photon ? (float4*)d_photon + 4 * p0 : nullptr;
         P.seq = d_seq ? (uint64_t*)d_seq + 4 * p0 : nullptr;
         P.photonlite = d_lite ? (uint4*)d_lite + p0 : nullptr;
+        if (taping) {
+            P.tape = sim->tape.p + (size_t)p0 * 3 * (size_t)K;
+            P.tape_n = sim->tape_n.p + p0;
+            P.tape_k = K;
+        }
         int rc = launch(sim, P, s);
         if (rc) return rc;
         HIPCHECK(hipEventRecord(sim->ev[2], s));
@@ -855,6 +882,11 @@ static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gen
         prop_ms += a; comp_ms += b;
         HIPCHECK(hipEventRecord(sim->ev[1], s));
         hits += nh;
+    }
+    if (taping) {
+        mi_sim::TapeKey& tk = sim->tape_key;
+        tk.valid = true; tk.event_id = event_id; tk.k = K; tk.photon_offset = photon_offset;
+        tk.n_photon = n_photon; tk.n_gs = n_gs; tk.gs = d_gensteps; tk.generation = sim->generation;
     }
     sim->last_ms[1] = prop_ms;
     sim->last_ms[2] = comp_ms;
@@ -1283,6 +1315,12 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     P.ev_id = sim->ev_id.p;
     P.n_photon = n_photon;
     P.photon_offset = photon_offset;
+    const mi_sim::TapeKey& tk = sim->tape_key;
+    if (tk.valid && tk.event_id == event_id && tk.photon_offset == photon_offset && tk.n_photon == n_photon &&
+        tk.n_gs == n_gs && tk.gs == d_gensteps && tk.generation == sim->generation) {
+        HIPCHECK(sim->tape_flag.reserve(1));
+        P.tape = sim->tape.p; P.tape_n = sim->tape_n.p; P.tape_k = tk.k; P.tape_overflow = sim->tape_flag.p;
+    }
     const uint32_t n_det = (uint32_t)adj->n_det;
     /* hits are processed in chunks of consecutive hits (photon order) whose
        score rows fit OA_ADJ_ROWS_MAX floats; the per-detector Kahan sums
@@ -1415,6 +1453,8 @@ struct mi_batch {
     DBuf<uint32_t> key, val, key2, val2;
     DBuf<unsigned char> tmp;
     DBuf<int64_t> seg;
+    DBuf<uint32_t> tape, tape_flag;     /* segment records of the forward (mi_config.adjoint_tape) */
+    DBuf<uint8_t> tape_n;
     hipEvent_t ev_mid = nullptr;
     double last_fwd_ms = 0.0, last_adj_ms = 0.0;
 };
@@ -1434,6 +1474,7 @@ int mi_batch_destroy(mi_batch* b)
     b->seed.release(); b->gs_event.release(); b->xw_jump.release();
     b->mat_block.release(); b->sample_bin.release(); b->rows.release(); b->comp.release();
     b->key.release(); b->val.release(); b->key2.release(); b->val2.release(); b->tmp.release(); b->seg.release();
+    b->tape.release(); b->tape_n.release(); b->tape_flag.release();
     delete b;
     return MI_OK;
 }
@@ -1591,6 +1632,14 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
         BCHECK(b->tmp.reserve(b->tmp_bytes));
         BCHECK(b->seg.reserve((size_t)b->n_det + 1));
         BCHECK(hipEventCreate(&b->ev_mid));
+        const int32_t K = sim->cfg.adjoint_tape;
+        if (K > 0 && sim->cfg.kernel == MI_KERNEL_WAVEFRONT) {
+            if (b->tape.reserve((size_t)b->n_photon * 3 * (size_t)K) != hipSuccess ||
+                b->tape_n.reserve((size_t)b->n_photon) != hipSuccess || b->tape_flag.reserve(1) != hipSuccess)
+                return bail(fail(MI_ERR_CAPACITY, "batch adjoint: cannot allocate the segment tape (" +
+                                                      std::to_string(b->n_photon * 12 * (int64_t)K) + " bytes)"));
+            P.tape = b->tape.p; P.tape_n = b->tape_n.p; P.tape_k = K; P.tape_overflow = b->tape_flag.p;
+        }
     }
     /* one graph: forward child graph, an event record node (splits the
        device time), adjoint child graph */

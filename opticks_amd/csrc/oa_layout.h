@@ -138,4 +138,11 @@ struct SimParams {
     int32_t refill_period;
     int32_t refill_batch;         /* refill once this many lanes are idle (or all are) */
     int32_t regroup_pct;          /* trace: leaf phase once fewer than this % of the walking lanes still walk */
+    /* segment records for a following adjoint (null: none): per launch-local photon
+       tape_k records of 3 words (bnd line | event << 30, wavelength, length), and the
+       number of segments the photon had (tape_k + 1 and more: the records stop at tape_k) */
+    uint32_t* tape;
+    uint8_t*  tape_n;
+    int32_t   tape_k;
+    uint32_t* tape_overflow;      /* set by the tape scoring when a hit needs the replay */
 };

@@ -60,7 +60,40 @@ enum : int { SEG_SAIL = 0, SEG_ABSORB = 1, SEG_SCATTER = 2 };
 
 /* The forward kernels record nothing. */
 struct NoTape {
+    __device__ inline __attribute__((always_inline)) void start(const SimParams&, int64_t) {}
     __device__ inline __attribute__((always_inline)) void seg(int, float, float, float, float, int) {}
+    __device__ inline __attribute__((always_inline)) void finish(const SimParams&, int64_t) {}
+};
+
+/* A forward kernel run for a following adjoint keeps, per photon, the arguments
+   of its segment calls -- bnd line, segment event, wavelength and segment length
+   (the absorption and scattering lengths are the table values at that line and
+   wavelength, looked up again when the record is read) -- up to P.tape_k of them,
+   and the number it had. The adjoint then scores a hit from its records instead
+   of replaying the photon. */
+struct RecordTape {
+    uint32_t* slot;
+    int32_t n, k;
+    __device__ inline __attribute__((always_inline)) void start(const SimParams& P, int64_t g)
+    {
+        slot = P.tape + g * 3 * (int64_t)P.tape_k;
+        n = 0;
+        k = P.tape_k;
+    }
+    __device__ inline __attribute__((always_inline)) void seg(int line, float wl, float, float, float ell, int ev)
+    {
+        if (n < k) {
+            uint32_t* r = slot + 3 * n;
+            r[0] = (uint32_t)line | ((uint32_t)ev << 30);
+            r[1] = __float_as_uint(wl);
+            r[2] = __float_as_uint(ell);
+        }
+        n++;
+    }
+    __device__ inline __attribute__((always_inline)) void finish(const SimParams& P, int64_t g)
+    {
+        P.tape_n[g] = (uint8_t)(n < 255 ? n : 255);
+    }
 };
 
 OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float cosTheta)

@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_plain(SimParams P)
 
 /* ------------------------------------------------ persistent kernel */
 
-template <int BLOCK, int F, class R>
+template <int BLOCK, int F, class R, class T = oa::NoTape>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
 {
     using namespace oa;
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
     uint32_t* stk = s_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
     Lane<R, (F & 1) != 0> L;
-    NoTape tape;
+    T tape;
     bool active = false;
     bool exhausted = false;
     const int period = P.refill_period > 0 ? P.refill_period : 1;
@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
                 const int64_t g = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (g < P.n_photon) {
                     lane_start(P, L, g);
+                    tape.start(P, g);
                     active = true;
                 }
             }
@@ -252,8 +253,9 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
 #endif
         OA_CYC_ADD(OA_ST_C_REFILL, c_iter);
         if (active) {
-            if (lane_step<BLOCK, NoTape, (F & 1) != 0, (F & 2) != 0>(P, L, stk, tape)) {
+            if (lane_step<BLOCK, T, (F & 1) != 0, (F & 2) != 0>(P, L, stk, tape)) {
                 lane_finish(P, L);
+                tape.finish(P, L.g);
                 if ((L.p.flagmask & P.hit_mask) != 0u)
                     atomicOr(P.hitwords + (L.g >> 6), 1ull << (L.g & 63));
                 active = false;
@@ -359,6 +361,9 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjP
     __shared__ uint32_t s_stack[OA_STACK_DEPTH * BLOCK];
     uint32_t* stk = s_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    /* with the segment tape, only photons it could not hold are replayed: none, when
+       the tape scoring raised no flag */
+    if (P.tape && *P.tape_overflow == 0u) return;
     const int64_t n = chunk_hits(begin, n_max, hit_off, n_ev, hit_cap);
     ScoreTape tape;
     tape.optical = P.optical; tape.n_wl = P.n_wl; tape.wl_low = P.wl_low; tape.wl_step = P.wl_step;
@@ -382,7 +387,8 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjP
                 const int64_t i = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (i < n) {
                     const int64_t g = hit_photon(P, hits, begin + i, hit_off, n_ev);
-                    if (g >= 0) {
+                    /* a hit whose segments are all on the tape was scored from it */
+                    if (g >= 0 && !(P.tape_n && (int32_t)P.tape_n[g] <= P.tape_k)) {
                         lane_start(P, L, g);
                         tape.row = S + i * (int64_t)A.n_param;
                         active = true;

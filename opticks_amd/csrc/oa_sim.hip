@@ -34,6 +34,10 @@ hipError_t OA_SIM_NAME(sim_launch, OA_SIM_BLOCK, OA_SIM_RNG)(const SimParams& P,
             case 2: hipLaunchKernelGGL((k_sim_plain<B, 2, OA_SIM_R>), dim3(nb), dim3(B), 0, s, P); break;
             default: hipLaunchKernelGGL((k_sim_plain<B, 3, OA_SIM_R>), dim3(nb), dim3(B), 0, s, P); break;
         }
+    } else if (P.tape) {
+        /* a forward for a following adjoint: segment records on the tape (no lite output) */
+        if (f & 2) hipLaunchKernelGGL((k_sim_wave<B, 2, OA_SIM_R, oa::RecordTape>), dim3((unsigned)grid), dim3(B), 0, s, P);
+        else hipLaunchKernelGGL((k_sim_wave<B, 0, OA_SIM_R, oa::RecordTape>), dim3((unsigned)grid), dim3(B), 0, s, P);
     } else {
         switch (f) {
             case 0: hipLaunchKernelGGL((k_sim_wave<B, 0, OA_SIM_R>), dim3((unsigned)grid), dim3(B), 0, s, P); break;

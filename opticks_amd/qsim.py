@@ -78,6 +78,7 @@ class SEventConfig:
     waves_per_cu: int = 0
     trace_regroup_pct: int = 0
     refill_batch: int = 16
+    adjoint_tape: int = 0           # segment records per photon kept by forwards for a following adjoint (0..64)
 
     ENV = dict(max_bounce=("OPTICKS_MAX_BOUNCE", int), max_time=("OPTICKS_MAX_TIME", float),
                propagate_epsilon=("OPTICKS_PROPAGATE_EPSILON", float),
@@ -128,6 +129,7 @@ class SEventConfig:
         c.waves_per_cu = self.waves_per_cu
         c.trace_regroup_pct = self.trace_regroup_pct
         c.refill_batch = self.refill_batch
+        c.adjoint_tape = self.adjoint_tape
         return c
 
 

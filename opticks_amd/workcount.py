@@ -13,6 +13,8 @@ roofline in bench.py.
       python -m opticks_amd.workcount batched64 profiles/work_k_sim_wave_batched64.json
   OPTICKS_AMD_LIB=opticks_amd/libopticks_amd_stats.so \\
       python -m opticks_amd.workcount csg10M profiles/work_k_sim_wave_csg10M.json
+  OPTICKS_AMD_LIB=opticks_amd/libopticks_amd_stats.so \\
+      python -m opticks_amd.workcount full256 profiles/work_k_sim_wave_full256.json
 """
 from __future__ import annotations
 
@@ -59,8 +61,8 @@ def run(which):
     if not hasattr(lib, "oa_stats_read"):
         raise RuntimeError("not a library with counters")
     dev = torch.device("cuda:0")
-    if which == "batched64":
-        w = configs.config2()
+    if which in ("batched64", "full256"):
+        w = configs.config2() if which == "batched64" else configs.config4()
         gs_np = np.concatenate(w.events).astype(np.float32)
         offs = np.cumsum([0] + [len(e) for e in w.events])
         ev = list(range(len(w.events)))
@@ -72,7 +74,8 @@ def run(which):
         read_counters(lib, reset=True)
         sim.simulate_batch(gs, offs, ev, photon)
         torch.cuda.synchronize()
-        desc = "batched-64-events (configs[2] layout, 20000 detectors), library launch defaults"
+        desc = ("batched-64-events (configs[2] layout, 20000 detectors)" if which == "batched64" else
+                "256 events (configs[4] layout: 20000 detectors, binned LS/water lengths)") + ", library launch defaults"
     elif which == "csg10M":
         w = configs.config1()
         N = w.num_photon

@@ -256,3 +256,59 @@ def test_adjoint_rejects_bad_arguments():
     sim.adjoint_device(gs_t, hit, 0, params.c_struct(N_DET), G, n_photon=w.num_photon)
     assert float(G.abs().sum()) == 0.0
     sim.close()
+
+
+@pytest.mark.parametrize("K", [12, 2])
+def test_tape_adjoint_equals_replay_bitwise(K):
+    """Forwards made with adjoint_tape = K keep each photon's segment records; the adjoint
+    scores hits from them (the same ScoreTape calls with the same arguments) and replays
+    only photons with more than K segments (K = 2 sends most hits down that path), so the
+    gradient is the replay's bit for bit, and the forward's outputs do not change."""
+    w = configs.config3(num_gs=100, per_gs=1000)
+    params = w.meta["params"]
+    N = w.num_photon
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    hit0, n0 = run_forward(sim, gs_t, N)
+    G0 = yield_jacobian(sim, params, gs_t, hit0, n0, N_DET, n_photon=N).cpu().numpy()
+    sim.set_config(SEventConfig(seed=w.seed, adjoint_tape=K))
+    hit1, n1 = run_forward(sim, gs_t, N)
+    assert n1 == n0 and torch.equal(hit1[:n1].view(torch.int32), hit0[:n0].view(torch.int32))
+    G1 = yield_jacobian(sim, params, gs_t, hit1, n1, N_DET, n_photon=N).cpu().numpy()
+    assert np.count_nonzero(G0) > 1000
+    assert np.array_equal(G1.view(np.uint32), G0.view(np.uint32))
+    # chunks of hits on the tape give the replay's chunked sums
+    Gc0 = yield_jacobian(sim, params, gs_t, hit1, n1, N_DET, n_photon=N, max_row_floats=37 * 256).cpu().numpy()
+    sim.set_config(SEventConfig(seed=w.seed))
+    run_forward(sim, gs_t, N)
+    Gc1 = yield_jacobian(sim, params, gs_t, hit0, n0, N_DET, n_photon=N, max_row_floats=37 * 256).cpu().numpy()
+    assert np.array_equal(Gc0.view(np.uint32), Gc1.view(np.uint32))
+    sim.close()
+
+
+def test_tape_batch_adjoint_equals_replay_bitwise():
+    """the captured forward + adjoint graph with the segment tape equals the replaying one"""
+    w = configs.config3(num_gs=30, per_gs=1000)
+    params = w.meta["params"]
+    offs = np.array([0, 10, 20, 30])
+    ev_ids = [4, 5, 6]
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    N = w.num_photon
+    out = []
+    for K in (0, 12):
+        sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, adjoint_tape=K))
+        photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+        seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
+        hit = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+        G = torch.full((N_DET, params.n_param), 7.0, dtype=torch.float32, device=DEV)
+        b = sim.batch(gs_t, offs, ev_ids, photon, seq, hit, adjoint=params.c_struct(N_DET, 53 * params.n_param),
+                      grad_t=G)
+        for rep in range(2):
+            b.launch()
+            ho, ms = b.wait()
+        out.append((G.cpu().numpy(), hit[:ho[-1]].cpu().numpy(), photon.cpu().numpy()))
+        b.close()
+        sim.close()
+    assert np.count_nonzero(out[0][0]) > 1000
+    for a, b_ in zip(out[0], out[1]):
+        assert np.array_equal(a.view(np.uint32), b_.view(np.uint32))
