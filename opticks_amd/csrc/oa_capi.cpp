@@ -861,8 +861,10 @@ static int simulate_device_impl(mi_sim* sim, int32_t event_id, const void* d_gen
         P.seq = d_seq ? (uint64_t*)d_seq + 4 * p0 : nullptr;
         P.photonlite = d_lite ? (uint4*)d_lite + p0 : nullptr;
         if (taping) {
-            P.tape = sim->tape.p + (size_t)p0 * 3 * (size_t)K;
+            /* segment-major records with the whole launch's photon count as the stride */
+            P.tape = sim->tape.p + (size_t)p0 * 3;
             P.tape_n = sim->tape_n.p + p0;
+            P.tape_stride = n_photon;
             P.tape_k = K;
         }
         int rc = launch(sim, P, s);
@@ -1317,9 +1319,11 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     P.photon_offset = photon_offset;
     const mi_sim::TapeKey& tk = sim->tape_key;
     if (tk.valid && tk.event_id == event_id && tk.photon_offset == photon_offset && tk.n_photon == n_photon &&
-        tk.n_gs == n_gs && tk.gs == d_gensteps && tk.generation == sim->generation) {
+        tk.n_gs == n_gs && tk.gs == d_gensteps && tk.generation == sim->generation &&
+        oa_launch::tape_reduce_rows(n_param) > 0) {
         HIPCHECK(sim->tape_flag.reserve(1));
-        P.tape = sim->tape.p; P.tape_n = sim->tape_n.p; P.tape_k = tk.k; P.tape_overflow = sim->tape_flag.p;
+        P.tape = sim->tape.p; P.tape_n = sim->tape_n.p; P.tape_stride = tk.n_photon; P.tape_k = tk.k;
+        P.tape_overflow = sim->tape_flag.p;
     }
     const uint32_t n_det = (uint32_t)adj->n_det;
     /* hits are processed in chunks of consecutive hits (photon order) whose
@@ -1340,18 +1344,19 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     /* persistent replay over every resident block slot */
     const int replay_grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu(sim->cfg.rng_kind));
     double replay_ms = 0.0, reduce_ms = 0.0;
+    oa_launch::AdjBufs ab;
+    ab.mat_block = sim->adj_mat_block.p; ab.n_mat = adj->n_mat;
+    ab.sample_bin = sim->adj_sample_bin.p; ab.n_bin = adj->n_bin; ab.n_param = n_param;
+    ab.hit = (const float4*)d_hit; ab.hit_off = nullptr; ab.n_ev = 0; ab.hit_cap = 0;
+    ab.n_det = n_det; ab.S = sim->adj_rows.p;
+    ab.key = sim->adj_key.p; ab.val = sim->adj_val.p; ab.key2 = sim->adj_key2.p; ab.val2 = sim->adj_val2.p;
+    ab.tmp = sim->adj_tmp.p; ab.tmp_bytes = tmp_bytes; ab.seg = sim->adj_seg.p;
+    ab.G = (float*)d_grad; ab.C = sim->adj_comp.p;
     int64_t h0 = 0;
     do {
         const int64_t nh = std::min<int64_t>(chunk, n_hit - h0);
-        const float4* hits = (const float4*)d_hit;
         HIPCHECK(hipEventRecord(sim->ev[0], s));
-        HIPCHECK(oa_launch::adjoint_replay(P, sim->adj_mat_block.p, adj->n_mat, sim->adj_sample_bin.p, adj->n_bin,
-                                           n_param, hits, h0, nh, nullptr, 0, 0, sim->adj_rows.p, replay_grid, s));
-        HIPCHECK(hipEventRecord(sim->ev[1], s));
-        HIPCHECK(oa_launch::adjoint_reduce(hits, h0, nh, nullptr, 0, 0, n_det, n_param, sim->adj_rows.p,
-                                           sim->adj_key.p, sim->adj_val.p, sim->adj_key2.p, sim->adj_val2.p,
-                                           sim->adj_tmp.p, tmp_bytes, sim->adj_seg.p, (float*)d_grad,
-                                           sim->adj_comp.p, h0 == 0 ? 1 : 0, s));
+        HIPCHECK(oa_launch::adjoint_chunk(P, ab, h0, nh, h0 == 0 ? 1 : 0, replay_grid, sim->ev[1], s));
         HIPCHECK(hipEventRecord(sim->ev[2], s));
         HIPCHECK(hipEventSynchronize(sim->ev[2]));
         float a = 0.f, b = 0.f;
@@ -1515,17 +1520,18 @@ static hipError_t batch_adjoint_work(mi_batch* b, SimParams P, const void* d_hit
     const mi_sim* sim = b->sim;
     const int grid = sim->n_cu * std::max(1, oa_launch::adjoint_blocks_per_cu(sim->cfg.rng_kind));
     hipError_t e;
+    oa_launch::AdjBufs ab;
+    ab.mat_block = b->mat_block.p; ab.n_mat = b->n_mat;
+    ab.sample_bin = b->sample_bin.p; ab.n_bin = b->n_bin; ab.n_param = b->n_param;
+    ab.hit = (const float4*)d_hit; ab.hit_off = b->ho.p; ab.n_ev = b->n_event; ab.hit_cap = b->hit_capacity;
+    ab.n_det = (uint32_t)b->n_det; ab.S = b->rows.p;
+    ab.key = b->key.p; ab.val = b->val.p; ab.key2 = b->key2.p; ab.val2 = b->val2.p;
+    ab.tmp = b->tmp.p; ab.tmp_bytes = b->tmp_bytes; ab.seg = b->seg.p;
+    ab.G = b->d_grad; ab.C = b->comp.p;
     for (int64_t c = 0; c < b->n_chunk; ++c) {
         const int64_t begin = c * b->chunk;
         const int64_t n_max = std::min(b->chunk, b->hit_capacity - begin);
-        if ((e = oa_launch::adjoint_replay(P, b->mat_block.p, b->n_mat, b->sample_bin.p, b->n_bin, b->n_param,
-                                           (const float4*)d_hit, begin, n_max, b->ho.p, b->n_event, b->hit_capacity,
-                                           b->rows.p, grid, s)) != hipSuccess)
-            return e;
-        if ((e = oa_launch::adjoint_reduce((const float4*)d_hit, begin, n_max, b->ho.p, b->n_event, b->hit_capacity,
-                                           (uint32_t)b->n_det, b->n_param, b->rows.p, b->key.p, b->val.p, b->key2.p,
-                                           b->val2.p, b->tmp.p, b->tmp_bytes, b->seg.p, b->d_grad, b->comp.p,
-                                           c == 0 ? 1 : 0, s)) != hipSuccess)
+        if ((e = oa_launch::adjoint_chunk(P, ab, begin, n_max, c == 0 ? 1 : 0, grid, nullptr, s)) != hipSuccess)
             return e;
     }
     return hipSuccess;
@@ -1633,12 +1639,13 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
         BCHECK(b->seg.reserve((size_t)b->n_det + 1));
         BCHECK(hipEventCreate(&b->ev_mid));
         const int32_t K = sim->cfg.adjoint_tape;
-        if (K > 0 && sim->cfg.kernel == MI_KERNEL_WAVEFRONT) {
+        if (K > 0 && sim->cfg.kernel == MI_KERNEL_WAVEFRONT && oa_launch::tape_reduce_rows(b->n_param) > 0) {
             if (b->tape.reserve((size_t)b->n_photon * 3 * (size_t)K) != hipSuccess ||
                 b->tape_n.reserve((size_t)b->n_photon) != hipSuccess || b->tape_flag.reserve(1) != hipSuccess)
                 return bail(fail(MI_ERR_CAPACITY, "batch adjoint: cannot allocate the segment tape (" +
                                                       std::to_string(b->n_photon * 12 * (int64_t)K) + " bytes)"));
-            P.tape = b->tape.p; P.tape_n = b->tape_n.p; P.tape_k = K; P.tape_overflow = b->tape_flag.p;
+            P.tape = b->tape.p; P.tape_n = b->tape_n.p; P.tape_stride = b->n_photon; P.tape_k = K;
+            P.tape_overflow = b->tape_flag.p;
         }
     }
     /* one graph: forward child graph, an event record node (splits the

@@ -22,15 +22,34 @@ hipError_t simtrace(const SimParams& P, float4* out, hipStream_t s);
 hipError_t trace(const SimParams& P, const float4* rays, int64_t n, float4* out, hipStream_t s);
 hipError_t refit(BvhNode* d_tlas, int64_t n_tlas, uint32_t max_depth, const uint32_t* d_tlas_items,
                  const float* d_inst_box, hipStream_t s);
-/* adjoint over hits [begin, begin+n) of d_hit, n = n_max or, with d_hit_off (per-event
-   hit offsets on the device, batched events), the device count clipped to hit_cap */
-hipError_t adjoint_replay(const SimParams& P, const int32_t* d_mat_block, int32_t n_mat, const int32_t* d_sample_bin,
-                          int32_t n_bin, int32_t n_param, const float4* d_hit, int64_t begin, int64_t n_max,
-                          const int64_t* d_hit_off, int32_t n_ev, int64_t hit_cap, float* d_S, int grid, hipStream_t s);
-hipError_t adjoint_reduce(const float4* d_hit, int64_t begin, int64_t n_max, const int64_t* d_hit_off, int32_t n_ev,
-                          int64_t hit_cap, uint32_t n_det, int32_t n_param, const float* d_S, uint32_t* d_key,
-                          uint32_t* d_val, uint32_t* d_key2, uint32_t* d_val2, void* d_tmp, size_t tmp_bytes,
-                          int64_t* d_seg, float* d_G, float* d_C, int first, hipStream_t s);
+/* device buffers of the adjoint: parameter map (mat_block, sample_bin), hits with
+   optional per-event hit offsets (batched events: the hit count is read on the
+   device and clipped to hit_cap), score rows S (n_max rows of n_param), sort
+   keys/values and temporary storage, detector runs seg (n_det + 1), and the
+   per-detector sums G with their Kahan compensation C (n_det x n_param) */
+struct AdjBufs {
+    const int32_t* mat_block; int32_t n_mat;
+    const int32_t* sample_bin; int32_t n_bin; int32_t n_param;
+    const float4* hit; const int64_t* hit_off; int32_t n_ev; int64_t hit_cap;
+    uint32_t n_det;
+    float* S;
+    uint32_t *key, *val, *key2, *val2;
+    void* tmp; size_t tmp_bytes;
+    int64_t* seg;
+    float *G, *C;
+};
+/* adjoint of hits [begin, begin+n) (n = n_max, or the device count): score rows,
+   then per-detector Kahan sums in photon order (first: start them, else continue
+   from the previous chunk). Without P.tape every hit is replayed into dense rows;
+   with it, hits are scored from their tape records inside the per-detector
+   reduction and only photons the tape could not hold are replayed. `mid` (may be
+   null) is recorded before the reduction. */
+hipError_t adjoint_chunk(const SimParams& P, const AdjBufs& a, int64_t begin, int64_t n_max, int first, int grid,
+                         hipEvent_t mid, hipStream_t s);
+/* hits per round of the fused tape reduction for n_param parameters (0: more
+   parameters than it handles, OA_TAPE_NJ_MAX x 256) */
+#define OA_TAPE_NJ_MAX 4
+int tape_reduce_rows(int32_t n_param);
 int adjoint_blocks_per_cu(int rng_kind);
 /* no-allocation entry point: the device address of the per-GPU scratch array
    (call with `device` current), and the per-launch event setup kernel */

@@ -138,11 +138,14 @@ struct SimParams {
     int32_t refill_period;
     int32_t refill_batch;         /* refill once this many lanes are idle (or all are) */
     int32_t regroup_pct;          /* trace: leaf phase once fewer than this % of the walking lanes still walk */
-    /* segment records for a following adjoint (null: none): per launch-local photon
-       tape_k records of 3 words (bnd line | event << 30, wavelength, length), and the
-       number of segments the photon had (tape_k + 1 and more: the records stop at tape_k) */
+    /* segment records for a following adjoint (null: none), segment-major so that
+       lanes at the same step write neighbouring records: record j < tape_k of
+       launch-local photon g is the 3 words (bnd line | event << 30, wavelength,
+       length) at tape + 3 * (j * tape_stride + g); tape_n[g] is the number of
+       segments the photon had (above tape_k: the records stop at tape_k) */
     uint32_t* tape;
     uint8_t*  tape_n;
+    int64_t   tape_stride;
     int32_t   tape_k;
-    uint32_t* tape_overflow;      /* set by the tape scoring when a hit needs the replay */
+    uint32_t* tape_overflow;      /* set when a hit's photon had more segments than the tape holds */
 };

@@ -58,11 +58,13 @@ struct State {
 /* Segment events reported to a Tape during propagate_to_boundary. */
 enum : int { SEG_SAIL = 0, SEG_ABSORB = 1, SEG_SCATTER = 2 };
 
-/* The forward kernels record nothing. */
+/* The forward kernels record nothing. Every propagation step makes exactly one
+   segment call (propagate -> to_boundary), so a photon's step number is its
+   segment number: `at` names it before the step, `finish` gets the count. */
 struct NoTape {
-    __device__ inline __attribute__((always_inline)) void start(const SimParams&, int64_t) {}
-    __device__ inline __attribute__((always_inline)) void seg(int, float, float, float, float, int) {}
-    __device__ inline __attribute__((always_inline)) void finish(const SimParams&, int64_t) {}
+    __device__ inline __attribute__((always_inline)) void at(const SimParams&, int64_t, int) {}
+    __device__ inline __attribute__((always_inline)) void seg(const SimParams&, int, float, float, float, float, int) {}
+    __device__ inline __attribute__((always_inline)) void finish(const SimParams&, int64_t, int) {}
 };
 
 /* A forward kernel run for a following adjoint keeps, per photon, the arguments
@@ -70,31 +72,37 @@ struct NoTape {
    (the absorption and scattering lengths are the table values at that line and
    wavelength, looked up again when the record is read) -- up to P.tape_k of them,
    and the number it had. The adjoint then scores a hit from its records instead
-   of replaying the photon. */
+   of replaying the photon. It adds no per-lane state: it holds the photon index
+   and step number the lane keeps anyway, and forms the record address from them
+   and the launch fields only at the store. */
 struct RecordTape {
-    uint32_t* slot;
-    int32_t n, k;
-    __device__ inline __attribute__((always_inline)) void start(const SimParams& P, int64_t g)
+    int64_t g;
+    int32_t step;
+    __device__ inline __attribute__((always_inline)) void at(const SimParams&, int64_t g_, int step_)
     {
-        slot = P.tape + g * 3 * (int64_t)P.tape_k;
-        n = 0;
-        k = P.tape_k;
+        g = g_; step = step_;
     }
-    __device__ inline __attribute__((always_inline)) void seg(int line, float wl, float, float, float ell, int ev)
+    __device__ inline __attribute__((always_inline)) void seg(const SimParams& P, int line, float wl, float, float, float ell,
+                                                              int ev)
     {
-        if (n < k) {
-            uint32_t* r = slot + 3 * n;
+        if (step < P.tape_k) {
+            uint32_t* r = P.tape + 3 * ((int64_t)step * P.tape_stride + g);
             r[0] = (uint32_t)line | ((uint32_t)ev << 30);
             r[1] = __float_as_uint(wl);
             r[2] = __float_as_uint(ell);
         }
-        n++;
     }
-    __device__ inline __attribute__((always_inline)) void finish(const SimParams& P, int64_t g)
+    __device__ inline __attribute__((always_inline)) void finish(const SimParams& P, int64_t g, int steps)
     {
-        P.tape_n[g] = (uint8_t)(n < 255 ? n : 255);
+        P.tape_n[g] = (uint8_t)(steps < 255 ? steps : 255);
     }
 };
+
+/* record j of launch-local photon g */
+OA_D const uint32_t* tape_record(const SimParams& P, int64_t g, int j)
+{
+    return P.tape + 3 * ((int64_t)j * P.tape_stride + g);
+}
 
 OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float cosTheta)
 {
@@ -244,9 +252,17 @@ OA_D int to_boundary(const SimParams& P, uint32_t& flag, R& rng, Photon& p, cons
     const float u_absorption = uni(rng);
     const float scattering_distance = -scattering_length * oa_logf(u_scattering);
     const float absorption_distance = -absorption_length * oa_logf(u_absorption);
+    /* the step's one segment: its length and the process that ends it (the
+       branches below, in the same order) */
+    {
+        const bool absorbs = absorption_distance <= scattering_distance && absorption_distance <= dist;
+        const bool scatters = !(absorption_distance <= scattering_distance) && scattering_distance <= dist;
+        tape.seg(P, s.m1_line, p.wavelength, absorption_length, scattering_length,
+                 absorbs ? absorption_distance : scatters ? scattering_distance : dist,
+                 absorbs ? SEG_ABSORB : scatters ? SEG_SCATTER : SEG_SAIL);
+    }
     if (absorption_distance <= scattering_distance) {
         if (absorption_distance <= dist) {
-            tape.seg(s.m1_line, p.wavelength, absorption_length, scattering_length, absorption_distance, SEG_ABSORB);
             p.time += absorption_distance / gv;
             p.pos = p.pos + absorption_distance * p.mom;
             float u_reemit = reemission_prob == 0.f ? 2.f : uni(rng);
@@ -262,14 +278,12 @@ OA_D int to_boundary(const SimParams& P, uint32_t& flag, R& rng, Photon& p, cons
             return CMD_BREAK;
         }
     } else if (scattering_distance <= dist) {
-        tape.seg(s.m1_line, p.wavelength, absorption_length, scattering_length, scattering_distance, SEG_SCATTER);
         p.time += scattering_distance / gv;
         p.pos = p.pos + scattering_distance * p.mom;
         rayleigh(p, rng);
         flag = F_BULK_SCATTER;
         return CMD_CONTINUE;
     }
-    tape.seg(s.m1_line, p.wavelength, absorption_length, scattering_length, dist, SEG_SAIL);
     p.pos = p.pos + dist * p.mom;
     p.time += dist / gv;
     return CMD_BOUNDARY;

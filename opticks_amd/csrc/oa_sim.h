@@ -141,6 +141,7 @@ OA_D bool lane_step(const SimParams& P, Lane<R, LITE>& L, uint32_t* stk, Tape& t
     prd.normal = normalize(prd.normal);
     OA_COUNT(OA_ST_STEP);
     OA_WAVE_COUNT(OA_ST_W_PHYS, OA_ST_L_PHYS);
+    tape.at(P, L.g, L.bounce);
     const int command = propagate(P, L.rng, L.p, prd, tape);
     L.bounce++;
     seq_add(L, (unsigned)L.bounce, ph_flag(L.p), ph_boundary(L.p));
@@ -240,7 +241,6 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
                 const int64_t g = (int64_t)(base + (unsigned long long)__popcll(below));
                 if (g < P.n_photon) {
                     lane_start(P, L, g);
-                    tape.start(P, g);
                     active = true;
                 }
             }
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_sim_wave(SimParams P)
         if (active) {
             if (lane_step<BLOCK, T, (F & 1) != 0, (F & 2) != 0>(P, L, stk, tape)) {
                 lane_finish(P, L);
-                tape.finish(P, L.g);
+                tape.finish(P, L.g, L.bounce);
                 if ((L.p.flagmask & P.hit_mask) != 0u)
                     atomicOr(P.hitwords + (L.g >> 6), 1ull << (L.g & 63));
                 active = false;
@@ -288,19 +288,37 @@ struct AdjParams {
     int32_t n_param;            /* blocks * 2 * n_bin */
 };
 
+/* the four score-row updates of one segment: row[i0..i3] += x0..x3 in that order
+   (i0..i3 need not differ: two table samples can fall in one bin) */
+struct ScoreTerm {
+    int32_t i0, i1, i2, i3;
+    float x0, x1, x2, x3;
+    bool on;
+};
+
 struct ScoreTape {
     const int32_t* optical;
     int32_t n_wl;
     float wl_low, wl_step;
     AdjParams A;
     float* row;
-    __device__ inline __attribute__((always_inline)) void seg(int m1_line, float wl, float La, float Ls, float ell, int ev)
+    __device__ inline __attribute__((always_inline)) void at(const SimParams&, int64_t, int) {}
+    /* score terms of a segment in material line m1_line at wavelength wl, length ell,
+       ending with event ev, where the table gives absorption length La and Rayleigh
+       length Ls: d log p / dL = ell / L^2 - [event of that process] / L, split over
+       the parameter bins of the two lerped table samples with weights 1 - a, a */
+    __device__ inline __attribute__((always_inline)) ScoreTerm term(int m1_line, float wl, float La, float Ls, float ell,
+                                                                    int ev) const
     {
         using namespace oa;
+        ScoreTerm t;
+        t.on = false;
+        t.i0 = t.i1 = t.i2 = t.i3 = 0;
+        t.x0 = t.x1 = t.x2 = t.x3 = 0.f;
         const int mat = optical[4 * m1_line];
-        if (mat < 0 || mat > A.n_mat) return;
+        if (mat < 0 || mat > A.n_mat) return t;
         const int blk = A.mat_block[mat];
-        if (blk < 0) return;
+        if (blk < 0) return t;
         float fx = (wl - wl_low) / wl_step;
         float fl = floorf(fx);
         float a = fx - fl;
@@ -309,11 +327,26 @@ struct ScoreTape {
         const int b0 = A.sample_bin[i0], b1 = A.sample_bin[i1];
         const float sa = ell / (La * La) - (ev == SEG_ABSORB ? 1.f / La : 0.f);
         const float ss = ell / (Ls * Ls) - (ev == SEG_SCATTER ? 1.f / Ls : 0.f);
-        float* r = row + blk * 2 * A.n_bin;
-        r[b0] += sa * (1.f - a);
-        r[b1] += sa * a;
-        r[A.n_bin + b0] += ss * (1.f - a);
-        r[A.n_bin + b1] += ss * a;
+        const int o = blk * 2 * A.n_bin;
+        t.i0 = o + b0; t.x0 = sa * (1.f - a);
+        t.i1 = o + b1; t.x1 = sa * a;
+        t.i2 = o + A.n_bin + b0; t.x2 = ss * (1.f - a);
+        t.i3 = o + A.n_bin + b1; t.x3 = ss * a;
+        t.on = true;
+        return t;
+    }
+    static __device__ inline __attribute__((always_inline)) void apply(float* r, const ScoreTerm& t)
+    {
+        if (!t.on) return;
+        r[t.i0] += t.x0;
+        r[t.i1] += t.x1;
+        r[t.i2] += t.x2;
+        r[t.i3] += t.x3;
+    }
+    __device__ inline __attribute__((always_inline)) void seg(const SimParams&, int m1_line, float wl, float La, float Ls,
+                                                              float ell, int ev)
+    {
+        apply(row, term(m1_line, wl, La, Ls, ell, ev));
     }
 };
 
@@ -351,7 +384,9 @@ OA_D int64_t hit_photon(const SimParams& P, const float4* hits, int64_t h, const
 
 /* persistent replay: lanes take the next hit of the chunk from a global
    counter (zeroed before the launch) as soon as their photon is done, as
-   k_sim_wave does; the score row of hit h is S + (h - begin) * n_param */
+   k_sim_wave does; the score row of hit h is S + (h - begin) * n_param. With
+   the segment tape only hits whose photon had more segments than it holds are
+   replayed, and their rows are read by the fused tape reduction. */
 template <int BLOCK, bool EXT, class R>
 __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjParams A, const float4* hits,
                                                                int64_t begin, int64_t n_max, const int64_t* hit_off,
@@ -362,7 +397,7 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjP
     uint32_t* stk = s_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
     /* with the segment tape, only photons it could not hold are replayed: none, when
-       the tape scoring raised no flag */
+       the ordering pass (k_hit_det) raised no flag */
     if (P.tape && *P.tape_overflow == 0u) return;
     const int64_t n = chunk_hits(begin, n_max, hit_off, n_ev, hit_cap);
     ScoreTape tape;
@@ -391,6 +426,9 @@ __global__ void __launch_bounds__(BLOCK) OA_OCC k_adjoint_wave(SimParams P, AdjP
                     if (g >= 0 && !(P.tape_n && (int32_t)P.tape_n[g] <= P.tape_k)) {
                         lane_start(P, L, g);
                         tape.row = S + i * (int64_t)A.n_param;
+                        /* with the tape the rows are not cleared beforehand: only these hits use them */
+                        if (P.tape)
+                            for (int32_t k = 0; k < A.n_param; ++k) tape.row[k] = 0.f;
                         active = true;
                     }
                 }

@@ -258,30 +258,36 @@ def test_adjoint_rejects_bad_arguments():
     sim.close()
 
 
-@pytest.mark.parametrize("K", [12, 2])
-def test_tape_adjoint_equals_replay_bitwise(K):
+@pytest.mark.parametrize("K,n_bin,max_slot", [(12, 64, 0), (2, 64, 0), (12, 100, 0), (12, 200, 0), (12, 300, 0),
+                                             (12, 64, 7000)])
+def test_tape_adjoint_equals_replay_bitwise(K, n_bin, max_slot):
     """Forwards made with adjoint_tape = K keep each photon's segment records; the adjoint
-    scores hits from them (the same ScoreTape calls with the same arguments) and replays
-    only photons with more than K segments (K = 2 sends most hits down that path), so the
-    gradient is the replay's bit for bit, and the forward's outputs do not change."""
-    w = configs.config3(num_gs=100, per_gs=1000)
+    scores hits from them inside the per-detector reduction (the same ScoreTape calls with
+    the same arguments, rows summed in the same order) and replays only photons with more
+    than K segments (K = 2 sends most hits down that path), so the gradient is the replay's
+    bit for bit, and the forward's outputs do not change. 100 and 200 bins per property
+    (400 and 800 parameters) run the reduction with two and four parameters per thread;
+    300 bins (1200 parameters) are beyond it, so that adjoint replays every hit; max_slot
+    splits the forward into launches that write their own part of the tape."""
+    w = configs.config3(num_gs=100, per_gs=1000, n_bin=n_bin)
     params = w.meta["params"]
     N = w.num_photon
-    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, max_slot=max_slot))
     gs_t = torch.from_numpy(w.gensteps).to(DEV)
     hit0, n0 = run_forward(sim, gs_t, N)
     G0 = yield_jacobian(sim, params, gs_t, hit0, n0, N_DET, n_photon=N).cpu().numpy()
-    sim.set_config(SEventConfig(seed=w.seed, adjoint_tape=K))
+    sim.set_config(SEventConfig(seed=w.seed, adjoint_tape=K, max_slot=max_slot))
     hit1, n1 = run_forward(sim, gs_t, N)
     assert n1 == n0 and torch.equal(hit1[:n1].view(torch.int32), hit0[:n0].view(torch.int32))
     G1 = yield_jacobian(sim, params, gs_t, hit1, n1, N_DET, n_photon=N).cpu().numpy()
     assert np.count_nonzero(G0) > 1000
     assert np.array_equal(G1.view(np.uint32), G0.view(np.uint32))
     # chunks of hits on the tape give the replay's chunked sums
-    Gc0 = yield_jacobian(sim, params, gs_t, hit1, n1, N_DET, n_photon=N, max_row_floats=37 * 256).cpu().numpy()
-    sim.set_config(SEventConfig(seed=w.seed))
+    rows = 37 * params.n_param
+    Gc0 = yield_jacobian(sim, params, gs_t, hit1, n1, N_DET, n_photon=N, max_row_floats=rows).cpu().numpy()
+    sim.set_config(SEventConfig(seed=w.seed, max_slot=max_slot))
     run_forward(sim, gs_t, N)
-    Gc1 = yield_jacobian(sim, params, gs_t, hit0, n0, N_DET, n_photon=N, max_row_floats=37 * 256).cpu().numpy()
+    Gc1 = yield_jacobian(sim, params, gs_t, hit0, n0, N_DET, n_photon=N, max_row_floats=rows).cpu().numpy()
     assert np.array_equal(Gc0.view(np.uint32), Gc1.view(np.uint32))
     sim.close()
 
