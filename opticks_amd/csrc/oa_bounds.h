@@ -18,6 +18,8 @@
  * result with the transformed prim AABB, so they fall back to that box. Boxes
  * only prune traversal: any conservative box gives the same hits.
  *
+ * Also the front-instance split and the free ball of the TLAS (below).
+ *
  * Conventions: CSGNode q0..q3 as the device reads them (sphere q0 = centre,
  * radius; zsphere q1.x/q1.y = z cuts relative to the centre; cylinder q0.w =
  * radius, q1.x/q1.y = z range; box3 q0.xyz = full sides; q3.z typecode, q3.w
@@ -30,6 +32,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 namespace oa_bounds {
 
@@ -229,6 +233,105 @@ inline Box instance_box(const float* o2w, const float* node, const float* itra, 
         all = join(all, meet(box_of_box(M, lo, hi), prim_box(node + 16 * (int64_t)off, itra, n_tran, M)));
     }
     return all;
+}
+
+inline double area(const Box& b)
+{
+    const double x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+    return x * y + y * z + z * x;
+}
+
+/* Front instances: the (at most max_front) instances whose world box has at least
+   a quarter of the surface area of the box around all instances -- the world
+   volume and the large shells inside it, which nearly every ray meets. */
+inline std::vector<char> front_mask(const std::vector<Box>& boxes, size_t max_front)
+{
+    Box u = empty();
+    for (const Box& b : boxes) u = join(u, b);
+    const double ua = boxes.empty() ? 0.0 : area(u);
+    std::vector<std::pair<double, size_t>> big;
+    for (size_t i = 0; i < boxes.size(); ++i)
+        if (area(boxes[i]) >= 0.25 * ua) big.push_back({ -area(boxes[i]), i });
+    std::sort(big.begin(), big.end());
+    if (big.size() > max_front) big.resize(max_front);
+    std::vector<char> front(boxes.size(), 0);
+    for (const auto& b : big) front[b.second] = 1;
+    return front;
+}
+
+/* distance from point c to box b (0 inside) */
+inline double box_distance(const double c[3], const Box& b)
+{
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double e = std::max(std::max(b.lo[k] - c[k], c[k] - b.hi[k]), 0.0);
+        d2 += e * e;
+    }
+    return std::sqrt(d2);
+}
+
+/* distance from point c to the box `local` (instance frame) placed by o2w, when o2w is
+   rigid (orthonormal linear part: distances are kept); 0 otherwise */
+inline double placed_box_distance(const double c[3], const Box& local, const float* o2w)
+{
+    const Affine A = from_rows(o2w);
+    double err = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += A.L[k][i] * A.L[k][j];
+            err = std::max(err, std::fabs(s - (i == j ? 1.0 : 0.0)));
+        }
+    Affine inv;
+    if (err > 1e-6 || !invert(A, inv)) return 0.0;
+    double cl[3];
+    for (int i = 0; i < 3; ++i) cl[i] = inv.t[i] + inv.L[i][0] * c[0] + inv.L[i][1] * c[1] + inv.L[i][2] * c[2];
+    return box_distance(cl, local);
+}
+
+/* A ball that meets no instance not marked front, less a margin for float arithmetic
+   on points up to |c| + r from the origin; r2 <= 0: none. The distance to an instance
+   is the larger of the distances to its world box and (with o2w, 16 floats per
+   instance, and locals, its box in its own frame) to its placed local box: both are
+   lower bounds of the distance to its shape. The centre is the centre of the front
+   boxes' union or of all boxes, whichever gives the larger radius. */
+struct Ball {
+    double c[3], r2;
+};
+
+inline Ball free_ball(const std::vector<Box>& boxes, const std::vector<char>& front,
+                      const std::vector<Box>* locals = nullptr, const float* o2w = nullptr)
+{
+    Box fu = empty(), au = empty();
+    bool any_front = false, any_rest = false;
+    for (size_t i = 0; i < boxes.size(); ++i) {
+        au = join(au, boxes[i]);
+        if (front[i]) { fu = join(fu, boxes[i]); any_front = true; }
+        else any_rest = true;
+    }
+    Ball best = { { 0.0, 0.0, 0.0 }, -1.0 };
+    if (!any_rest) return best;
+    double best_r = 0.0;
+    for (int which = 0; which < 2; ++which) {
+        if (which == 0 && !any_front) continue;
+        const Box& u = which == 0 ? fu : au;
+        double c[3];
+        for (int k = 0; k < 3; ++k) c[k] = 0.5 * (u.lo[k] + u.hi[k]);
+        double r = 1e300;
+        for (size_t i = 0; i < boxes.size(); ++i) {
+            if (front[i]) continue;
+            double d = box_distance(c, boxes[i]);
+            if (locals && o2w) d = std::max(d, placed_box_distance(c, (*locals)[i], o2w + 16 * i));
+            r = std::min(r, d);
+        }
+        const double cn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        const double rm = r - 1e-4 * (r + cn) - 1e-2;
+        if (rm > best_r) {
+            best_r = rm;
+            best = { { c[0], c[1], c[2] }, rm * rm };
+        }
+    }
+    return best;
 }
 
 } // namespace oa_bounds

@@ -125,6 +125,7 @@ struct mi_sim {
     uint32_t tlas_depth = 0, blas_depth = 0, blas_stack = 0;
     uint32_t front_leaf = OA_BVH_EMPTY;             /* items of the front instances (build_tlas) */
     int32_t tlas_empty = 0;                         /* every instance is a front instance */
+    std::vector<char> h_is_front;                   /* per instance: kept out of the TLAS */
     std::vector<uint32_t> solid_blas_root;
     std::vector<uint32_t> h_inst_blas_root;
     std::vector<std::array<double, 6>> solid_box;   /* padded solid-frame prim box per solid */
@@ -140,6 +141,7 @@ struct mi_sim {
     DBuf<BvhNode> tlas, blas;
     DBuf<uint32_t> tlas_items, blas_items;
     DBuf<float> inst_box;
+    DBuf<float4> free_ball;                         /* see free_ball() */
 
     /* physics */
     DBuf<float4> bnd;
@@ -338,44 +340,63 @@ static void instance_items(const mi_sim* sim, std::vector<oa_bvh::Item>& titems)
     }
 }
 
-static double box_area(const oa_bvh::Item& it)
+static oa_bounds::Box item_box(const oa_bvh::Item& it)
 {
-    const double a = it.hi[0] - it.lo[0], b = it.hi[1] - it.lo[1], c = it.hi[2] - it.lo[2];
-    return a * b + b * c + c * a;
+    oa_bounds::Box b;
+    for (int k = 0; k < 3; ++k) { b.lo[k] = it.lo[k]; b.hi[k] = it.hi[k]; }
+    return b;
 }
 
 /*
- * Front instances: the (at most OA_MAX_FRONT) instances whose world box has at
- * least a quarter of the surface area of the box around all instances -- the
- * world volume and the large shells inside it, which nearly every ray meets.
- * They stay out of the TLAS; every trace tests them first (their items follow the
- * TLAS items, reached through SimParams.front_leaf), so the whole wave runs them
+ * Front instances (oa_bounds::front_mask: world-sized boxes, at most OA_MAX_FRONT)
+ * stay out of the TLAS; every trace tests them first (their items follow the TLAS
+ * items, reached through SimParams.front_leaf), so the whole wave runs them
  * together and the TLAS walk starts with the nearest of their hits as its tmax.
  * The nearest hit is chosen by (t, instance, prim), not by test order, so which
  * instances are front changes no result.
  */
 static void split_front(const std::vector<oa_bvh::Item>& all, std::vector<oa_bvh::Item>& rest,
-                        std::vector<uint32_t>& front)
+                        std::vector<uint32_t>& front, std::vector<char>& is_front)
 {
-    oa_bvh::Item u;
-    for (int k = 0; k < 3; ++k) { u.lo[k] = 1e300; u.hi[k] = -1e300; }
-    for (const auto& it : all)
-        for (int k = 0; k < 3; ++k) { u.lo[k] = std::min(u.lo[k], it.lo[k]); u.hi[k] = std::max(u.hi[k], it.hi[k]); }
-    const double ua = all.empty() ? 0.0 : box_area(u);
-    std::vector<std::pair<double, uint32_t>> big;
-    for (size_t i = 0; i < all.size(); ++i) {
-        const double a = box_area(all[i]);
-        if (a >= 0.25 * ua) big.push_back({ -a, (uint32_t)i });
-    }
-    std::sort(big.begin(), big.end());
-    if (big.size() > OA_MAX_FRONT) big.resize(OA_MAX_FRONT);
-    std::vector<char> is_front(all.size(), 0);
+    std::vector<oa_bounds::Box> boxes(all.size());
+    for (size_t i = 0; i < all.size(); ++i) boxes[i] = item_box(all[i]);
+    is_front = oa_bounds::front_mask(boxes, OA_MAX_FRONT);
     front.clear();
-    for (const auto& b : big) { is_front[b.second] = 1; front.push_back(all[b.second].id); }
-    std::sort(front.begin(), front.end());
     rest.clear();
-    for (size_t i = 0; i < all.size(); ++i)
-        if (!is_front[i]) rest.push_back(all[i]);
+    for (size_t i = 0; i < all.size(); ++i) {
+        if (is_front[i]) front.push_back(all[i].id);
+        else rest.push_back(all[i]);
+    }
+}
+
+/*
+ * The free ball (oa_bounds::free_ball over the padded boxes the TLAS is built
+ * over): a trace whose segment from its origin to the nearest front-instance hit
+ * has both ends inside it cannot hit a TLAS instance (the ball is convex), so that
+ * lane skips the TLAS walk -- on a detector shell around a scintillator volume,
+ * every trace that starts inside the shell and ends on an inner surface. It culls
+ * only, so no result depends on it.
+ */
+static float4 free_ball(const mi_sim* sim, const std::vector<oa_bvh::Item>& all, const std::vector<char>& is_front)
+{
+    std::vector<oa_bounds::Box> boxes(all.size()), locals(all.size());
+    for (size_t i = 0; i < all.size(); ++i) boxes[i] = item_box(all[i]);
+    /* each solid's box in its own frame (the exact leaf bounds under the identity) */
+    static const float eye[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+    const int64_t n_tran = (int64_t)sim->h_itra.size() / 16;
+    std::vector<oa_bounds::Box> solid_local(sim->h_solid_prims.size());
+    for (size_t sidx = 0; sidx < solid_local.size(); ++sidx)
+        solid_local[sidx] = oa_bounds::instance_box(eye, sim->h_node.data(),
+                                                    sim->h_itra.empty() ? nullptr : sim->h_itra.data(), n_tran,
+                                                    sim->h_prim.data(), sim->h_solid_prims[sidx].first,
+                                                    sim->h_solid_prims[sidx].second);
+    for (size_t i = 0; i < all.size(); ++i) {
+        int32_t gas;
+        std::memcpy(&gas, sim->h_o2w.data() + 16 * i + 7, 4);
+        locals[i] = solid_local[gas];
+    }
+    const oa_bounds::Ball b = oa_bounds::free_ball(boxes, is_front, &locals, sim->h_o2w.data());
+    return make_float4((float)b.c[0], (float)b.c[1], (float)b.c[2], (float)b.r2);
 }
 
 /* host SAH build of the TLAS over instance boxes with the given leaf size, uploaded
@@ -385,7 +406,9 @@ static int build_tlas(mi_sim* sim, int leaf)
     std::vector<oa_bvh::Item> all, titems;
     std::vector<uint32_t> front;
     instance_items(sim, all);
-    split_front(all, titems, front);
+    std::vector<char> is_front;
+    split_front(all, titems, front, is_front);
+    const float4 ball = free_ball(sim, all, is_front);
     std::vector<BvhNode> tlas;
     std::vector<uint32_t> tlas_items;
     oa_bvh::Builder tb(tlas, tlas_items, leaf);
@@ -401,16 +424,20 @@ static int build_tlas(mi_sim* sim, int leaf)
         return fail(MI_ERR_UNSUPPORTED, "BVH deeper than the LDS traversal stack");
     DBuf<BvhNode> nt;
     DBuf<uint32_t> ni;
+    DBuf<float4> nb;
     hipError_t e = hipStreamSynchronize(sim->stream);
     if (e == hipSuccess) e = upload(nt, tlas.data(), tlas.size(), sim->stream);
     if (e == hipSuccess) e = upload(ni, tlas_items.data(), tlas_items.size(), sim->stream);
+    if (e == hipSuccess) e = upload(nb, &ball, 1, sim->stream);
     if (e != hipSuccess) {
-        nt.release(); ni.release();
+        nt.release(); ni.release(); nb.release();
         return fail(MI_ERR_HIP, std::string("TLAS upload: ") + hipGetErrorString(e));
     }
     sim->tlas.swap(nt);
     sim->tlas_items.swap(ni);
-    nt.release(); ni.release();
+    sim->free_ball.swap(nb);
+    nt.release(); ni.release(); nb.release();
+    sim->h_is_front = is_front;
     sim->tlas_depth = tb.max_depth;
     sim->n_tlas = (int64_t)tlas.size();
     sim->front_leaf = front_leaf;
@@ -630,7 +657,7 @@ void mi_sim_destroy(mi_sim* sim)
     if (sim->stream) (void)hipStreamSynchronize(sim->stream);
     sim->node.release(); sim->itra.release(); sim->plan.release(); sim->inst.release(); sim->prim.release();
     sim->tlas.release(); sim->blas.release(); sim->tlas_items.release(); sim->blas_items.release();
-    sim->inst_box.release();
+    sim->inst_box.release(); sim->free_ball.release();
     sim->bnd.release(); sim->optical.release(); sim->icdf.release(); sim->precooked.release();
     sim->gs.release(); sim->prefix.release(); sim->block_total.release(); sim->evbuf.release();
     sim->word_prefix.release(); sim->ev_id.release(); sim->words.release(); sim->next.release();
@@ -667,7 +694,7 @@ static SimParams base_params(const mi_sim* sim)
     std::memset(&P, 0, sizeof(P));
     P.node = sim->node.p; P.itra = sim->itra.p; P.plan = sim->plan.p; P.ext_leaves = sim->ext_leaves; P.inst = sim->inst.p; P.prim = sim->prim.p;
     P.tlas = sim->tlas.p; P.tlas_items = sim->tlas_items.p; P.blas = sim->blas.p; P.blas_items = sim->blas_items.p;
-    P.front_leaf = sim->front_leaf; P.tlas_empty = sim->tlas_empty;
+    P.front_leaf = sim->front_leaf; P.tlas_empty = sim->tlas_empty; P.free_ball = sim->free_ball.p;
     P.bnd = sim->bnd.p; P.optical = sim->optical.p; P.icdf = sim->icdf.p;
     P.n_wl = sim->n_wl; P.wl_low = sim->wl_low; P.wl_step = sim->wl_step;
     P.n_icdf = sim->n_icdf; P.n_icdf_layer = sim->n_icdf_layer; P.icdf_hd = sim->icdf_hd;
@@ -1242,6 +1269,9 @@ int mi_sim_refit_instances(mi_sim* sim, const float* inst, int64_t n_inst)
         oa_bvh::Builder::store(boxes.data() + 6 * i, boxes.data() + 6 * i + 3, items[i].lo, items[i].hi);
     HIPCHECK(upload(sim->inst_box, boxes.data(), boxes.size(), s));
     HIPCHECK(oa_launch::refit(sim->tlas.p, sim->n_tlas, sim->tlas_depth, sim->tlas_items.p, sim->inst_box.p, s));
+    /* the free ball of the moved boxes, written in place (captured batches read it there) */
+    const float4 ball = free_ball(sim, items, sim->h_is_front);
+    HIPCHECK(hipMemcpyAsync(sim->free_ball.p, &ball, sizeof(ball), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
     return MI_OK;
 }

@@ -99,6 +99,7 @@ struct SimParams {
     const uint32_t* blas_items;   /* prim indices (global) */
     uint32_t front_leaf;          /* leaf link to the front instances' items (tested first), or empty */
     int32_t tlas_empty;           /* the TLAS holds no instance: every instance is front */
+    const float4* free_ball;      /* (cx, cy, cz, r^2): a ball no TLAS instance box meets (r^2 <= 0: none) */
     /* physics */
     const float4*  bnd;
     const int32_t* optical;       /* (nbnd*4, 4) */

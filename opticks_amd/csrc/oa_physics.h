@@ -811,6 +811,17 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             }
         }
         OA_CYC_ADD(OA_ST_C_P2, c_p2);
+        /* after the front phase (the TLAS walk not begun: cur is still the root): a lane
+           whose segment from the origin to the nearest front hit lies in the free ball,
+           which no TLAS instance box meets, has nothing to find in the TLAS */
+        if (cur == 0u) {
+            const float4 fb = *P.free_ball;
+            if (fb.w > 0.f) {
+                const v3 c = mk(fb.x, fb.y, fb.z);
+                const v3 a = o - c, b = o + best_t * d - c;
+                if (dot(a, a) < fb.w && dot(b, b) < fb.w) cur = OA_BVH_EMPTY;
+            }
+        }
     }
     OA_CYC_ADD(OA_ST_C_TRACE, c_trace);
 #ifdef OA_STATS

@@ -7,7 +7,10 @@ box3 leaves under node and instance transforms, combined over CSG trees.
   the leaf, Boolean and list zoos with random rotations;
 * never looser than the transformed corners of the prim AABB (the box used before);
 * the closed forms: a rotated sphere's box is its centre +- r on every axis, a cylinder tilted
-  about x by 90 degrees swaps its height into y.
+  about x by 90 degrees swaps its height into y;
+* the free ball (a trace whose segment lies inside it skips the TLAS walk) holds no hit the
+  checker finds on a non-front instance, and on the detector-shell layouts holds the
+  scintillator sphere (with upright detectors also the acrylic), where traces inside it end.
 """
 import os
 import shutil
@@ -44,7 +47,17 @@ def boxes_of(exe, g, tag):
             a.tofile(fh)
     o = d / f"{tag}.out"
     subprocess.run([str(prog), str(f), str(o)], check=True)
-    return np.fromfile(o, np.float64).reshape(-1, 6)
+    v = np.fromfile(o, np.float64)
+    n = len(g["inst"])
+    return v[:6 * n].reshape(-1, 6)
+
+
+def front_and_ball(exe, g, tag):
+    """(front mask, ball centre, ball r^2) as the TLAS build computes them from these boxes"""
+    boxes_of(exe, g, tag)
+    v = np.fromfile(exe[0] / f"{tag}.out", np.float64)
+    n = len(g["inst"])
+    return v[6 * n:7 * n].astype(bool), v[7 * n:7 * n + 3], v[7 * n + 3]
 
 
 def corner_boxes(g):
@@ -148,3 +161,60 @@ def test_boxes_hold_every_hit_and_are_no_looser(exe, which):
         # the rotated detectors' boxes are much tighter than their corner boxes
         v = np.prod(b[1:, 3:] - b[1:, :3], axis=1) / np.prod(cb[1:, 3:] - cb[1:, :3], axis=1)
         assert np.median(v) < 0.8, np.median(v)
+
+
+def hit_points(g, physics, n_per_inst, seed, traverse):
+    """checker hits of rays aimed at every instance but the world: (points, instance)"""
+    rng = np.random.default_rng(seed)
+    cb = corner_boxes(g)
+    rays = []
+    for i in range(1, len(g["inst"])):
+        lo, hi = cb[i, :3], cb[i, 3:]
+        c, ext = 0.5 * (lo + hi), np.linalg.norm(hi - lo)
+        o = c + 3.0 * ext * rng.normal(size=(n_per_inst, 3)) / np.linalg.norm(rng.normal(size=(n_per_inst, 3)), axis=1)[:, None]
+        p = lo + (hi - lo) * rng.random((n_per_inst, 3))
+        d = p - o
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        r = np.zeros((n_per_inst, 2, 4), np.float32)
+        r[:, 0, :3], r[:, 0, 3] = o, 0.0
+        r[:, 1, :3], r[:, 1, 3] = d, 1e6
+        rays.append(r)
+    rays = np.concatenate(rays)
+    out = po.World(g, physics).trace(rays, traverse=traverse)
+    t = out[:, 0, 3].astype(np.float64)
+    inst = out[:, 1, 3].view(np.uint32).astype(np.int64)
+    hit = (out[:, 1, 1].view(np.uint32) != 0xffff) & (inst > 0)
+    p = rays[hit, 0, :3].astype(np.float64) + t[hit, None] * rays[hit, 1, :3].astype(np.float64)
+    return p, inst[hit]
+
+
+@pytest.mark.parametrize("which", ["config1", "config2", "rotated_config1", "boolean_zoo"])
+def test_free_ball_holds_no_hit(exe, which):
+    """No hit the checker finds on a TLAS (non-front) instance lies inside the free ball,
+    on the detector shells (upright and randomly rotated detectors) and the rotated Boolean
+    zoo; on the shells the ball holds the LS sphere (and with upright detectors the acrylic
+    sphere), where a photon's traces inside the shell end; with upright detectors a 2% larger
+    ball would hold hits."""
+    if which == "config2":
+        w = configs.config2(num_events=1, mean_gs=1, mean_photons=1)
+        g, n, traverse = dict(w.geometry), 6, 1
+    elif which == "boolean_zoo":
+        w = configs.boolean_zoo(num_photon=4)
+        g, n, traverse = rotated(w, 7), 20000, 0
+    else:
+        w = configs.config1(num_gs=1, per_gs=1)
+        g, n, traverse = (dict(w.geometry) if which == "config1" else rotated(w, 5)), 60, 1
+    front, c, r2 = front_and_ball(exe, g, which)
+    assert front[0] and front.sum() == 1             # the global solid (world box, acrylic, LS)
+    p, inst = hit_points(g, w.physics, n, 13, traverse)
+    assert len(p) > 5000
+    dist = np.linalg.norm(p - c, axis=1)
+    if r2 > 0:
+        assert dist.min() > np.sqrt(r2), (dist.min(), np.sqrt(r2))
+    if which != "boolean_zoo":
+        # upright detectors: the ball holds the acrylic sphere; randomly rotated ones
+        # can point a side inward, and it holds the LS sphere
+        inner = w.meta["ls_r"] + (120.0 if which != "rotated_config1" else 0.0)
+        assert r2 > 0 and np.sqrt(r2) > inner, (np.sqrt(r2), inner)
+        if which != "rotated_config1":
+            assert dist.min() < 1.02 * np.sqrt(r2)     # the dome apex: a 2% larger ball holds hits
