@@ -288,11 +288,12 @@ struct AdjParams {
     int32_t n_param;            /* blocks * 2 * n_bin */
 };
 
-/* the four score-row updates of one segment: row[i0..i3] += x0..x3 in that order
-   (i0..i3 need not differ: two table samples can fall in one bin) */
+/* the four score-row updates of one segment, in this order:
+   row[i0] += sa (1 - a), row[i1] += sa a, row[i2] += ss (1 - a), row[i3] += ss a
+   (i0, i1 need not differ: two table samples can fall in one bin) */
 struct ScoreTerm {
     int32_t i0, i1, i2, i3;
-    float x0, x1, x2, x3;
+    float sa, ss, a;
     bool on;
 };
 
@@ -314,7 +315,7 @@ struct ScoreTape {
         ScoreTerm t;
         t.on = false;
         t.i0 = t.i1 = t.i2 = t.i3 = 0;
-        t.x0 = t.x1 = t.x2 = t.x3 = 0.f;
+        t.sa = t.ss = t.a = 0.f;
         const int mat = optical[4 * m1_line];
         if (mat < 0 || mat > A.n_mat) return t;
         const int blk = A.mat_block[mat];
@@ -328,20 +329,21 @@ struct ScoreTape {
         const float sa = ell / (La * La) - (ev == SEG_ABSORB ? 1.f / La : 0.f);
         const float ss = ell / (Ls * Ls) - (ev == SEG_SCATTER ? 1.f / Ls : 0.f);
         const int o = blk * 2 * A.n_bin;
-        t.i0 = o + b0; t.x0 = sa * (1.f - a);
-        t.i1 = o + b1; t.x1 = sa * a;
-        t.i2 = o + A.n_bin + b0; t.x2 = ss * (1.f - a);
-        t.i3 = o + A.n_bin + b1; t.x3 = ss * a;
+        t.i0 = o + b0;
+        t.i1 = o + b1;
+        t.i2 = o + A.n_bin + b0;
+        t.i3 = o + A.n_bin + b1;
+        t.sa = sa; t.ss = ss; t.a = a;
         t.on = true;
         return t;
     }
     static __device__ inline __attribute__((always_inline)) void apply(float* r, const ScoreTerm& t)
     {
         if (!t.on) return;
-        r[t.i0] += t.x0;
-        r[t.i1] += t.x1;
-        r[t.i2] += t.x2;
-        r[t.i3] += t.x3;
+        r[t.i0] += t.sa * (1.f - t.a);
+        r[t.i1] += t.sa * t.a;
+        r[t.i2] += t.ss * (1.f - t.a);
+        r[t.i3] += t.ss * t.a;
     }
     __device__ inline __attribute__((always_inline)) void seg(const SimParams&, int m1_line, float wl, float La, float Ls,
                                                               float ell, int ev)
