@@ -1359,10 +1359,15 @@ int mi_sim_adjoint_device(mi_sim* sim, int32_t event_id, const void* d_gensteps,
     /* hits are processed in chunks of consecutive hits (photon order) whose
        score rows fit OA_ADJ_ROWS_MAX floats; the per-detector Kahan sums
        carry their compensation from chunk to chunk */
+    /* with a tape of at least max_bounce records no photon overflows it (one segment per
+       step): no score rows are needed, so the hits are not chunked unless the caller asks */
+    const bool tape_only = P.tape && P.tape_k >= sim->cfg.max_bounce;
     const int64_t rows_max = adj->max_row_floats > 0 ? adj->max_row_floats : OA_ADJ_ROWS_MAX;
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(n_hit, 1), rows_max / n_param));
+    const int64_t chunk = (tape_only && adj->max_row_floats <= 0)
+                              ? std::max<int64_t>(n_hit, 1)
+                              : std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(n_hit, 1), rows_max / n_param));
     const size_t tmp_bytes = std::max<size_t>(1, oa_launch::adjoint_sort_bytes(chunk, n_det));
-    if (sim->adj_rows.reserve((size_t)(chunk * n_param)) != hipSuccess)
+    if (sim->adj_rows.reserve((size_t)((tape_only ? 1 : chunk) * n_param)) != hipSuccess)
         return fail(MI_ERR_CAPACITY, "adjoint: cannot allocate " + std::to_string(chunk * n_param * 4) + " bytes of score rows");
     HIPCHECK(sim->adj_comp.reserve((size_t)n_det * n_param));
     HIPCHECK(sim->adj_key.reserve((size_t)chunk));
@@ -1651,13 +1656,20 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
         b->n_mat = adj->n_mat; b->n_bin = adj->n_bin; b->n_det = adj->n_det;
         b->n_param = adj->n_block * 2 * adj->n_bin;
         b->d_grad = (float*)d_grad;
+        const int32_t K = sim->cfg.adjoint_tape;
+        const bool taping = K > 0 && sim->cfg.kernel == MI_KERNEL_WAVEFRONT && oa_launch::tape_reduce_rows(b->n_param) > 0;
+        /* a tape of at least max_bounce records holds every photon (one segment per step):
+           no score rows, and one chunk of hits unless the caller limits it */
+        const bool tape_only = taping && K >= sim->cfg.max_bounce;
         const int64_t rows_max = adj->max_row_floats > 0 ? adj->max_row_floats : OA_ADJ_ROWS_MAX;
-        b->chunk = std::max<int64_t>(1, std::min<int64_t>(hit_capacity, rows_max / b->n_param));
+        b->chunk = (tape_only && adj->max_row_floats <= 0)
+                       ? std::max<int64_t>(1, std::min<int64_t>(hit_capacity, (int64_t)1 << 30))
+                       : std::max<int64_t>(1, std::min<int64_t>(hit_capacity, rows_max / b->n_param));
         b->n_chunk = (hit_capacity + b->chunk - 1) / b->chunk;
         b->tmp_bytes = std::max<size_t>(1, oa_launch::adjoint_sort_bytes(b->chunk, (uint32_t)b->n_det));
         BCHECK(upload(b->mat_block, adj->mat_block, (size_t)adj->n_mat + 1, s));
         BCHECK(upload(b->sample_bin, adj->sample_bin, (size_t)sim->n_wl, s));
-        if (b->rows.reserve((size_t)(b->chunk * b->n_param)) != hipSuccess)
+        if (b->rows.reserve((size_t)((tape_only ? 1 : b->chunk) * b->n_param)) != hipSuccess)
             return bail(fail(MI_ERR_CAPACITY, "batch adjoint: cannot allocate " +
                                                   std::to_string(b->chunk * b->n_param * 4) + " bytes of score rows"));
         BCHECK(b->comp.reserve((size_t)b->n_det * b->n_param));
@@ -1668,8 +1680,7 @@ static int batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids, 
         BCHECK(b->tmp.reserve(b->tmp_bytes));
         BCHECK(b->seg.reserve((size_t)b->n_det + 1));
         BCHECK(hipEventCreate(&b->ev_mid));
-        const int32_t K = sim->cfg.adjoint_tape;
-        if (K > 0 && sim->cfg.kernel == MI_KERNEL_WAVEFRONT && oa_launch::tape_reduce_rows(b->n_param) > 0) {
+        if (taping) {
             if (b->tape.reserve((size_t)b->n_photon * 3 * (size_t)K) != hipSuccess ||
                 b->tape_n.reserve((size_t)b->n_photon) != hipSuccess || b->tape_flag.reserve(1) != hipSuccess)
                 return bail(fail(MI_ERR_CAPACITY, "batch adjoint: cannot allocate the segment tape (" +

@@ -258,8 +258,8 @@ def test_adjoint_rejects_bad_arguments():
     sim.close()
 
 
-@pytest.mark.parametrize("K,n_bin,max_slot", [(12, 64, 0), (2, 64, 0), (12, 100, 0), (12, 200, 0), (12, 300, 0),
-                                             (12, 64, 7000)])
+@pytest.mark.parametrize("K,n_bin,max_slot", [(12, 64, 0), (2, 64, 0), (32, 64, 0), (12, 100, 0), (12, 200, 0),
+                                             (12, 300, 0), (12, 64, 7000)])
 def test_tape_adjoint_equals_replay_bitwise(K, n_bin, max_slot):
     """Forwards made with adjoint_tape = K keep each photon's segment records; the adjoint
     scores hits from them inside the per-detector reduction (the same ScoreTape calls with
@@ -268,7 +268,9 @@ def test_tape_adjoint_equals_replay_bitwise(K, n_bin, max_slot):
     bit for bit, and the forward's outputs do not change. 100 and 200 bins per property
     (400 and 800 parameters) run the reduction with two and four parameters per thread;
     300 bins (1200 parameters) are beyond it, so that adjoint replays every hit; max_slot
-    splits the forward into launches that write their own part of the tape."""
+    splits the forward into launches that write their own part of the tape; K = 32, at least
+    max_bounce, holds every photon, so the adjoint needs no score rows and takes all hits in
+    one chunk unless the caller limits it."""
     w = configs.config3(num_gs=100, per_gs=1000, n_bin=n_bin)
     params = w.meta["params"]
     N = w.num_photon
@@ -293,7 +295,9 @@ def test_tape_adjoint_equals_replay_bitwise(K, n_bin, max_slot):
 
 
 def test_tape_batch_adjoint_equals_replay_bitwise():
-    """the captured forward + adjoint graph with the segment tape equals the replaying one"""
+    """the captured forward + adjoint graph with the segment tape equals the replaying one,
+    with a tape of 12 records (53-row chunks, replays of longer photons) and of 32 (at least
+    max_bounce: no score rows; the explicit row limit still chunks the hits)"""
     w = configs.config3(num_gs=30, per_gs=1000)
     params = w.meta["params"]
     offs = np.array([0, 10, 20, 30])
@@ -301,7 +305,7 @@ def test_tape_batch_adjoint_equals_replay_bitwise():
     gs_t = torch.from_numpy(w.gensteps).to(DEV)
     N = w.num_photon
     out = []
-    for K in (0, 12):
+    for K in (0, 12, 32):
         sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed, adjoint_tape=K))
         photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
         seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
@@ -316,5 +320,6 @@ def test_tape_batch_adjoint_equals_replay_bitwise():
         b.close()
         sim.close()
     assert np.count_nonzero(out[0][0]) > 1000
-    for a, b_ in zip(out[0], out[1]):
-        assert np.array_equal(a.view(np.uint32), b_.view(np.uint32))
+    for k in (1, 2):
+        for a, b_ in zip(out[0], out[k]):
+            assert np.array_equal(a.view(np.uint32), b_.view(np.uint32))

@@ -245,16 +245,30 @@ def test_truncation_matches_checker(w1, kw):
     sim.close()
 
 
-def test_refit_after_moving_instances():
+@pytest.mark.parametrize("scale", [1.03, 0.97])
+def test_refit_after_moving_instances(scale):
+    """Detectors moved out to 5665 mm, or in to 5335 mm: inward, the detector domes (5145 mm)
+    lie inside the free ball of the first layout (about 5309 mm), so the refit must shrink
+    it; a batch captured before the refit reads the refitted TLAS and ball in place."""
     w = configs.config1(num_gs=10, per_gs=1000)
     sim = QSim(w.geometry, w.physics, SEventConfig(seed=4))
+    N = w.num_photon
+    gs_t = torch.from_numpy(w.gensteps).to(DEV)
+    photon = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    seq = torch.empty((N, 2, 2), dtype=torch.int64, device=DEV)
+    hit = torch.empty((N, 4, 4), dtype=torch.float32, device=DEV)
+    b = sim.batch(gs_t, np.array([0, len(w.gensteps)]), [0], photon, seq, hit)
     moved = w.geometry["inst"].copy()
-    moved[1:, 3, :3] *= np.float32(1.03)                   # detectors out to 5665 mm
+    moved[1:, 3, :3] *= np.float32(scale)
     sim.refit_instances(moved)
     g2 = dict(w.geometry); g2["inst"] = moved
     sim.simulate(w.gensteps, 0)
     rp, rs = po.World(g2, w.physics).simulate(w.gensteps, seed=4)
     compare(sim.photon, sim.seq, rp, rs)
+    b.launch()
+    b.wait()
+    compare(photon.cpu().numpy(), seq.cpu().numpy().view(np.uint64), rp, rs)
+    b.close()
     sim.close()
 
 

@@ -86,6 +86,15 @@ def test_config4_graph_forward_adjoint_equals_per_event_and_checker():
         np.add.at(abs64, det, np.abs(rows))
     assert np.all(np.abs(Gg - rows64) <= 2e-5 * abs64 + 1e-30)
     b.close()
+    # the same graph with a 32-record tape (at least max_bounce: no score rows, all hits in
+    # one chunk, scored from the tape): the same gradient bit for bit
+    sim.set_config(SEventConfig(seed=w.seed, adjoint_tape=32))
+    G2 = torch.zeros_like(G)
+    b2 = sim.batch(gs_t, offs, ev_ids, photon, seq, hit, adjoint=params.c_struct(N_DET, 0), grad_t=G2)
+    b2.launch()
+    b2.wait()
+    assert np.array_equal(G2.cpu().numpy().view(np.uint32), G.cpu().numpy().view(np.uint32))
+    b2.close()
     sim.close()
 
 
