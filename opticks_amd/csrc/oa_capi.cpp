@@ -374,7 +374,8 @@ static void split_front(const std::vector<oa_bvh::Item>& all, std::vector<oa_bvh
  * over): a trace whose segment from its origin to the nearest front-instance hit
  * has both ends inside it cannot hit a TLAS instance (the ball is convex), so that
  * lane skips the TLAS walk -- on a detector shell around a scintillator volume,
- * every trace that starts inside the shell and ends on an inner surface. It culls
+ * every trace that starts inside the shell and ends on an inner surface; a lane
+ * whose origin is inside it starts its box tests where the ray leaves it. It culls
  * only, so no result depends on it.
  */
 static float4 free_ball(const mi_sim* sim, const std::vector<oa_bvh::Item>& all, const std::vector<char>& is_front)

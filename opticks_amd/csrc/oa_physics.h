@@ -719,6 +719,8 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
        of the TLAS) on every lane together; the walk then starts with their
        nearest hit as tmax */
     uint32_t leaf = P.front_leaf;
+    /* lower bound of the node and BLAS box tests of TLAS instances (set after the front phase) */
+    float tlo = tmin;
     bool done = false;
 #ifdef OA_STATS
     uint32_t n1 = 0, n2 = 0;
@@ -750,7 +752,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
                 const int sp0 = sp;
 #endif
-                cur = visit<STRIDE>(P.tlas, cur, inv, oi, tmin, best_t, stk, sp);
+                cur = visit<STRIDE>(P.tlas, cur, inv, oi, tlo, best_t, stk, sp);
 #ifdef OA_STATS
                 if (cur == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_TLAS_DEAD);
 #endif
@@ -792,7 +794,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
                     const int sp0 = sp;
 #endif
-                    bc = visit<STRIDE>(P.blas, bc, inv2, oi2, tmin, best_t, stk, sp);
+                    bc = visit<STRIDE>(P.blas, bc, inv2, oi2, tlo, best_t, stk, sp);
 #ifdef OA_STATS
                     if (bc == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_BLAS_DEAD);
 #endif
@@ -813,13 +815,24 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
         OA_CYC_ADD(OA_ST_C_P2, c_p2);
         /* after the front phase (the TLAS walk not begun: cur is still the root): a lane
            whose segment from the origin to the nearest front hit lies in the free ball,
-           which no TLAS instance box meets, has nothing to find in the TLAS */
+           which no TLAS instance meets, has nothing to find in the TLAS; one whose origin
+           lies in it can meet an instance only after leaving it, so its box tests start
+           there (boxes only prune: the prim tests keep tmin) */
         if (cur == 0u) {
             const float4 fb = *P.free_ball;
             if (fb.w > 0.f) {
                 const v3 c = mk(fb.x, fb.y, fb.z);
                 const v3 a = o - c, b = o + best_t * d - c;
-                if (dot(a, a) < fb.w && dot(b, b) < fb.w) cur = OA_BVH_EMPTY;
+                const float qa = dot(a, a) - fb.w;
+                if (qa < 0.f) {
+                    if (dot(b, b) < fb.w) {
+                        cur = OA_BVH_EMPTY;
+                    } else {
+                        const float dd = dot(d, d), bb = dot(a, d);
+                        const float t_exit = (sqrtf(bb * bb - dd * qa) - bb) / dd;
+                        tlo = fmaxf(tmin, t_exit);
+                    }
+                }
             }
         }
     }
