@@ -324,7 +324,9 @@ int  mi_sim_set_bnd(mi_sim* sim, const float* bnd);
  * sample_bin (host, n_wl): bin of each wavelength sample, in [0, n_bin).
  * n_det: detectors, hits carry sensor identifiers 1..n_det (others ignored).
  * max_row_floats: device scratch for score rows per chunk of hits, in floats
- * (0 = 2^30, i.e. 4 GiB); hits are processed in chunks that fit it.
+ * (0 = 2^30, i.e. 4 GiB); hits are processed in chunks that fit it. With a
+ * segment tape of at least max_bounce records (mi_config.adjoint_tape) no rows
+ * are used, and the hits are chunked only when this is set.
  */
 typedef struct mi_adjoint {
     const int32_t* mat_block;
@@ -345,9 +347,15 @@ typedef struct mi_adjoint {
  * is spread over the bins of the two table samples each lookup interpolates.
  * d_grad (n_det, n_block*2*n_bin) f32 receives d E[hits of detector d]/d theta,
  * summed per detector in photon order with Kahan compensation.
- * Device scratch: min(n_hit*n_param, max_row_floats) floats of score rows,
- * n_det*n_param floats of Kahan compensation, sort buffers.
- * Synchronous on return; mi_sim_last_adjoint_timing gives [replay, reduce, total] ms.
+ * When the handle's last forward was this event made with mi_config.adjoint_tape
+ * = K > 0 (wavefront kernel), hits are scored from its segment records instead,
+ * inside the per-detector reduction, and only photons with more than K segments
+ * are replayed; the gradient is the same bit for bit.
+ * Device scratch: min(n_hit*n_param, max_row_floats) floats of score rows (none
+ * with a tape of K >= max_bounce records), n_det*n_param floats of Kahan
+ * compensation, sort buffers.
+ * Synchronous on return; mi_sim_last_adjoint_timing gives [replay, reduce, total] ms
+ * (with the tape: [sort and overflow replay, fused scoring and reduction, total]).
  */
 int  mi_sim_adjoint_device(mi_sim* sim, int32_t event_id,
                            const void* d_gensteps, int64_t n_gs,
@@ -402,7 +410,10 @@ int  mi_batch_create(mi_sim* sim, int32_t n_event, const int32_t* event_ids,
  * summed over all events of the batch. hit_capacity also bounds the adjoint's
  * hit slots; score rows use min(hit_capacity, max_row_floats/n_param) hits per
  * chunk, and the chunk loop is fixed at capture, reading the actual hit count on
- * the device. mi_batch_last_timing gives [forward, adjoint] ms of the last replay.
+ * the device. With mi_config.adjoint_tape = K > 0 the batch keeps its own segment
+ * tape (12 K bytes per photon) and scores hits from it (K >= max_bounce: no score
+ * rows, one chunk unless max_row_floats is set). mi_batch_last_timing gives
+ * [forward, adjoint] ms of the last replay.
  */
 int  mi_batch_create_adjoint(mi_sim* sim, int32_t n_event, const int32_t* event_ids,
                              const int64_t* gs_offsets, const void* d_gensteps,
