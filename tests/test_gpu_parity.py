@@ -137,7 +137,11 @@ def test_plain_wavefront_and_regroup_identical_full_size():
         del photon, seq
 
 
-def test_trace_matches_checker_brute_force():
+@pytest.mark.parametrize("dscale", [1.0, 2.5, 0.4])
+def test_trace_matches_checker_brute_force(dscale):
+    """60k rays from anywhere in the world box (a third start inside the free ball, which
+    skips or clips their instance walk), against the checker's brute force; directions of
+    length 2.5 and 0.4 as well as unit ones (t is in units of the direction's length)"""
     w = configs.config1(num_gs=1, per_gs=1)
     sim = QSim(w.geometry, w.physics, SEventConfig())
     rng = np.random.default_rng(7)
@@ -145,7 +149,7 @@ def test_trace_matches_checker_brute_force():
     rays = np.zeros((n, 2, 4), np.float32)
     rays[:, 0, :3] = rng.uniform(-6400, 6400, (n, 3))
     d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
-    rays[:, 1, :3] = d
+    rays[:, 1, :3] = d * dscale
     rays[:, 0, 3] = rng.choice([0.0, 0.05, 10.0], n)
     rays[:, 1, 3] = 1e6
     rt = torch.from_numpy(rays).to(DEV)
