@@ -218,3 +218,28 @@ def test_free_ball_holds_no_hit(exe, which):
         assert r2 > 0 and np.sqrt(r2) > inner, (np.sqrt(r2), inner)
         if which != "rotated_config1":
             assert dist.min() < 1.02 * np.sqrt(r2)     # the dome apex: a 2% larger ball holds hits
+
+
+def test_free_ball_absent_without_tlas_instances(exe):
+    """a world whose only instance is front (the water-sphere workload) has no ball"""
+    w = configs.config0(num_photon=4)
+    front, c, r2 = front_and_ball(exe, dict(w.geometry), "config0")
+    assert front.all() and r2 <= 0
+
+
+def test_free_ball_with_scaled_detectors(exe):
+    """detectors scaled by 1.2 about their own positions: the placed local box no longer
+    bounds their distance (the transform is not rigid), so the ball falls back to the world
+    boxes and still holds no hit the checker finds"""
+    w = configs.config1(num_gs=1, per_gs=1)
+    g = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in w.geometry.items()}
+    g["inst"][1:, :3, :3] *= np.float32(1.2)
+    front, c, r2 = front_and_ball(exe, g, "scaled")
+    assert r2 > 0
+    p, inst = hit_points(g, w.physics, 60, 17, 1)
+    assert len(p) > 5000
+    dist = np.linalg.norm(p - c, axis=1)
+    assert dist.min() > np.sqrt(r2), (dist.min(), np.sqrt(r2))
+    # the world-box distance is looser than the rigid case's placed-box distance
+    _, _, r2_rigid = front_and_ball(exe, dict(w.geometry), "rigid")
+    assert r2 < r2_rigid
