@@ -458,6 +458,7 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     sim->solid_blas_root.resize(g->n_solid);
     uint32_t bdepth = 0, bneed = 0;
     std::vector<std::array<double, 6>> solid_box(g->n_solid);
+    std::vector<uint32_t> solid_flat(g->n_solid);      /* one leaf link over all the solid's prims */
     for (int64_t so = 0; so < g->n_solid; ++so) {
         int32_t np, po;
         std::memcpy(&np, sb + 48 * so + 16, 4);
@@ -476,7 +477,11 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
             it.id = (uint32_t)(po + p);
         }
         oa_bvh::Builder b(blas, blas_items, 1);
+        const uint32_t start = (uint32_t)blas_items.size();
         sim->solid_blas_root[so] = b.build(items);
+        /* the leaves of a solid's BLAS hold each of its prims once, appended together */
+        solid_flat[so] = (np > 0 && np <= 0x7f && (size_t)start + np <= 0x00ffffffu)
+                             ? OA_BVH_LEAF_BIT | ((uint32_t)np << 24) | start : sim->solid_blas_root[so];
         bdepth = std::max(bdepth, b.max_depth);
         bneed = std::max(bneed, b.stack_need);
         solid_box[so] = bx;
@@ -514,6 +519,18 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     sim->n_prim = g->n_prim;
     int trc = build_tlas(sim, sim->cfg.bvh_leaf_size);
     if (trc) return trc;
+    /* Front instances (tested by every trace, on all lanes together) walk no BLAS:
+       their solid's prims are one leaf, tested in turn without box tests. Nearly
+       every trace meets all of a world-sized solid's prim boxes anyway, and the
+       nearest hit does not depend on the order or on box pruning (ties go to the
+       smaller key). The link is valid for any instance, so a later TLAS rebuild
+       that changes which instances are front changes no result. */
+    for (int64_t i = 0; i < ni; ++i) {
+        if (!sim->h_is_front[i]) continue;
+        int32_t gas;
+        std::memcpy(&gas, g->inst + 16 * i + 7, 4);
+        inst[i].m[11] = f32_of(solid_flat[gas]);
+    }
 
     std::vector<PrimRec> prim(g->n_prim);
     for (int64_t p = 0; p < g->n_prim; ++p) {

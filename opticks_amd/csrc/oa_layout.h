@@ -9,7 +9,8 @@
  *                   0xffffffff = empty slot; otherwise the child node index.
  *   InstRec   64 B  world->object 3x4 (rows of qat4::right_multiply: m[0..2],
  *                   m[4..6], m[8..10] linear, m[12..14] translation), then
- *                   gas (solid) index, identity (sensor_identifier+1), blas root.
+ *                   gas (solid) index, identity (sensor_identifier+1), blas root
+ *                   (front instances: one leaf link over all the solid's prims).
  *   PrimRec   16 B  node offset of the prim's root CSGNode, boundary, globalPrimIdx.
  *   nodes/itra      the reference CSGNode / qat4 arrays unchanged (64 B each).
  *   bnd             (nbnd*4*2 rows, nwl, float4): row 2*line+k is one property

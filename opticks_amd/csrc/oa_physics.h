@@ -641,13 +641,22 @@ OA_D void cswap(float& ta, uint32_t& ca, float& tb, uint32_t& cb)
    Leaf links go on the stack like node links, so both levels are traversed
    nearest-first and a shrinking tmax culls what is still pending. Equal entry
    distances keep the node's child order (leaves first). */
-template <int STRIDE>
+template <int STRIDE, bool SINGLE = false>
 OA_D uint32_t visit(const BvhNode* nodes, uint32_t idx, v3 inv, v3 oi, float tmin, float tmax,
                     uint32_t* stk, int& sp)
 {
     const float4* q = reinterpret_cast<const float4*>(nodes + idx);
     const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
     const uint4 ch = *reinterpret_cast<const uint4*>(q + 6);
+    if constexpr (SINGLE) {
+        /* a node with one child (the BLAS root of a one-prim solid): one box test.
+           Empty slots hold boxes at +inf, which every test misses, so the full
+           visit below would return the same link and push nothing. */
+        if (ch.y == OA_BVH_EMPTY) {
+            const float t = box_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, inv, oi, tmin, tmax);
+            return t < INFINITY ? ch.x : OA_BVH_EMPTY;
+        }
+    }
     float t0 = box_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, inv, oi, tmin, tmax);
     float t1 = box_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, inv, oi, tmin, tmax);
     float t2 = box_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, inv, oi, tmin, tmax);
@@ -779,7 +788,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             v3 inv2, oi2;
             slab_ray(oo, od, inv2, oi2);
             const int sp_base = sp;
-            uint32_t bc = __float_as_uint(m[2].w);     /* BLAS root node of this solid */
+            uint32_t bc = __float_as_uint(m[2].w);     /* BLAS root of this solid (front instances: one leaf of all its prims) */
             for (;;) {
                 OA_WAVE_COUNT(OA_ST_W_P2, OA_ST_L_P2);
 #ifdef OA_STATS
@@ -796,7 +805,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
 #ifdef OA_STATS
                     const int sp0 = sp;
 #endif
-                    bc = visit<STRIDE>(P.blas, bc, inv2, oi2, tlo, best_t, stk, sp);
+                    bc = visit<STRIDE, true>(P.blas, bc, inv2, oi2, tlo, best_t, stk, sp);
 #ifdef OA_STATS
                     if (bc == OA_BVH_EMPTY && sp == sp0) OA_COUNT(OA_ST_BLAS_DEAD);
 #endif
