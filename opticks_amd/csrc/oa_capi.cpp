@@ -137,9 +137,9 @@ struct mi_sim {
     int32_t ext_leaves = 0;                         /* geometry uses the extended leaf set */
     DBuf<float4> node, itra, plan;
     DBuf<InstRec> inst;
-    DBuf<PrimRec> prim;
+    DBuf<PrimRec> prim, blas_prims;                 /* by prim index; in BLAS leaf order */
     DBuf<BvhNode> tlas, blas;
-    DBuf<uint32_t> tlas_items, blas_items;
+    DBuf<uint32_t> tlas_items;
     DBuf<float> inst_box;
     DBuf<float4> free_ball;                         /* see free_ball() */
 
@@ -539,8 +539,11 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
         prim[p].node_offset = (uint32_t)pr[1];
         prim[p].boundary = root[6];
         prim[p].global_prim = (uint32_t)pr[15];
-        prim[p].pad = 0;
+        prim[p].index = (uint32_t)p;
     }
+    /* the leaves' items replaced by the records they name, so the BLAS leaf links index them */
+    std::vector<PrimRec> blas_prims(blas_items.size());
+    for (size_t k = 0; k < blas_items.size(); ++k) blas_prims[k] = prim[blas_items[k]];
     sim->ext_leaves = 0;
     for (int64_t k = 0; k < g->n_node; ++k) {
         const uint32_t t = ((const uint32_t*)(g->node + 16 * k))[14];
@@ -554,7 +557,7 @@ static int build_and_upload_geometry(mi_sim* sim, const mi_geometry* g)
     HIPCHECK(upload(sim->inst, inst.data(), inst.size(), s));
     HIPCHECK(upload(sim->prim, prim.data(), prim.size(), s));
     HIPCHECK(upload(sim->blas, blas.data(), blas.size(), s));
-    HIPCHECK(upload(sim->blas_items, blas_items.data(), blas_items.size(), s));
+    HIPCHECK(upload(sim->blas_prims, blas_prims.data(), blas_prims.size(), s));
     HIPCHECK(sim->inst_box.reserve((size_t)ni * 6));
     return MI_OK;
 }
@@ -674,7 +677,7 @@ void mi_sim_destroy(mi_sim* sim)
     if (!sim) return;
     if (sim->stream) (void)hipStreamSynchronize(sim->stream);
     sim->node.release(); sim->itra.release(); sim->plan.release(); sim->inst.release(); sim->prim.release();
-    sim->tlas.release(); sim->blas.release(); sim->tlas_items.release(); sim->blas_items.release();
+    sim->tlas.release(); sim->blas.release(); sim->tlas_items.release(); sim->blas_prims.release();
     sim->inst_box.release(); sim->free_ball.release();
     sim->bnd.release(); sim->optical.release(); sim->icdf.release(); sim->precooked.release();
     sim->gs.release(); sim->prefix.release(); sim->block_total.release(); sim->evbuf.release();
@@ -711,7 +714,7 @@ static SimParams base_params(const mi_sim* sim)
     SimParams P;
     std::memset(&P, 0, sizeof(P));
     P.node = sim->node.p; P.itra = sim->itra.p; P.plan = sim->plan.p; P.ext_leaves = sim->ext_leaves; P.inst = sim->inst.p; P.prim = sim->prim.p;
-    P.tlas = sim->tlas.p; P.tlas_items = sim->tlas_items.p; P.blas = sim->blas.p; P.blas_items = sim->blas_items.p;
+    P.tlas = sim->tlas.p; P.tlas_items = sim->tlas_items.p; P.blas = sim->blas.p; P.blas_prims = sim->blas_prims.p;
     P.front_leaf = sim->front_leaf; P.tlas_empty = sim->tlas_empty; P.free_ball = sim->free_ball.p;
     P.bnd = sim->bnd.p; P.optical = sim->optical.p; P.icdf = sim->icdf.p;
     P.n_wl = sim->n_wl; P.wl_low = sim->wl_low; P.wl_step = sim->wl_step;

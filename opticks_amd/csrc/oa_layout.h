@@ -11,7 +11,9 @@
  *                   m[4..6], m[8..10] linear, m[12..14] translation), then
  *                   gas (solid) index, identity (sensor_identifier+1), blas root
  *                   (front instances: one leaf link over all the solid's prims).
- *   PrimRec   16 B  node offset of the prim's root CSGNode, boundary, globalPrimIdx.
+ *   PrimRec   16 B  node offset of the prim's root CSGNode, boundary, globalPrimIdx,
+ *                   prim index. The BLAS leaves index a copy of the records in leaf
+ *                   order, so a prim test reads its record without an item lookup.
  *   nodes/itra      the reference CSGNode / qat4 arrays unchanged (64 B each).
  *   bnd             (nbnd*4*2 rows, nwl, float4): row 2*line+k is one property
  *                   group of one line, contiguous in wavelength so a lerp reads
@@ -83,7 +85,7 @@ struct PrimRec {
     uint32_t node_offset;
     uint32_t boundary;
     uint32_t global_prim;
-    uint32_t pad;
+    uint32_t index;               /* the prim's index (in the BLAS-ordered copy) */
 };
 
 struct SimParams {
@@ -97,7 +99,7 @@ struct SimParams {
     const BvhNode* tlas;
     const uint32_t* tlas_items;   /* instance indices */
     const BvhNode* blas;
-    const uint32_t* blas_items;   /* prim indices (global) */
+    const PrimRec* blas_prims;    /* prim records in BLAS leaf order (PrimRec.index: the prim) */
     uint32_t front_leaf;          /* leaf link to the front instances' items (tested first), or empty */
     int32_t tlas_empty;           /* the TLAS holds no instance: every instance is front */
     const float4* free_ball;      /* (cx, cy, cz, r^2): a ball no TLAS instance box meets (r^2 <= 0: none) */

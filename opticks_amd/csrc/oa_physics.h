@@ -684,12 +684,14 @@ OA_D void inst_ray(const float4* m, v3 o, v3 d, v3& oo, v3& od)
             r0.z * d.x + r1.z * d.y + r2.z * d.z);
 }
 
-/* one prim of one instance against the nearest hit so far (t, then the (instance, prim) key) */
+/* one prim of one instance (entry k of the BLAS-ordered prim records) against the
+   nearest hit so far (t, then the (instance, prim) key) */
 template <bool EXT>
-OA_D void consider_prim(const SimParams& P, uint32_t ii, uint32_t pi, v3 oo, v3 od, float tmin, float& best_t,
+OA_D void consider_prim(const SimParams& P, uint32_t ii, uint32_t k, v3 oo, v3 od, float tmin, float& best_t,
                         uint64_t& best_key, uint32_t& best_inst, uint32_t& best_prim, Isect& best_is)
 {
-    const PrimRec pr = P.prim[pi];
+    const PrimRec pr = P.blas_prims[k];
+    const uint32_t pi = pr.index;
     Isect is;
     if (!intersect_prim<EXT>(is, P.node + 4u * pr.node_offset, P.itra, P.plan, tmin, oo, od)) return;
     /* a reported hit outside [tmin, tmax] is ignored, as optixReportIntersection does
@@ -814,10 +816,9 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
                 const uint32_t bf = bc & 0x00ffffffu, bn = (bc >> 24) & 0x7fu;
                 OA_CYC_MARK(c_prim);
                 for (uint32_t j = 0; j < bn; ++j) {
-                    const uint32_t pi = P.blas_items[bf + j];
                     OA_COUNT(OA_ST_PRIM);
                     OA_WAVE_COUNT(OA_ST_W_PRIM, OA_ST_L_PRIM);
-                    consider_prim<EXT>(P, ii, pi, oo, od, tmin, best_t, best_key, best_inst, best_prim, best_is);
+                    consider_prim<EXT>(P, ii, bf + j, oo, od, tmin, best_t, best_key, best_inst, best_prim, best_is);
                 }
                 OA_CYC_ADD(OA_ST_C_PRIM, c_prim);
                 bc = OA_BVH_EMPTY;
