@@ -453,6 +453,53 @@ def surface_zoo(num_photon=200_000, seed=42, mix=(0.2, 0.3, 0.3, 0.2), sens=(0.1
                               sources=list(SURFACE_ZOO_SOURCES.values())))
 
 
+def prim_clusters(num_photon=200_000, seed=42, rows=(10, 7)):
+    """Solids of several prims, on both sides of the front/TLAS split: a water box
+    whose world solid also holds two planes of rows[0] x rows[1] small acrylic beads
+    (z = +-700 mm; more prims than one BLAS leaf link holds, so the front solid is
+    walked through its BLAS), and eight rotated instances of a five-prim acrylic
+    cluster (sphere, translated box, cylinder and z-sphere, offset sphere) at
+    (+-400, +-400, +-250) mm; a torch point source at the centre."""
+    lib = BoundaryLib()
+    lib.add_material(Material("Water", 1.33, 20000.0, 30000.0))
+    lib.add_material(Material("Acrylic", 1.50, 1e6, 1e6))
+    lib.add_surface(Surface("wallSurf", detect=0.5, absorb=0.5))
+    b_world = lib.boundary("Water/wallSurf/wallSurf/Water")
+    b_obj = lib.boundary("Water///Acrylic")
+    fd = Foundry()
+    world = [fd.addPrim(Node.box3(2000.0, 2000.0, 2000.0), b_world)]
+    nx, ny = rows
+    for z in (-700.0, 700.0):
+        for i in range(nx):
+            for j in range(ny):
+                c = (-810.0 + 1620.0 * i / (nx - 1), -810.0 + 1620.0 * j / (ny - 1), z)
+                world.append(fd.addPrim(Node.sphere(15.0, c), b_obj))
+    s0 = fd.addSolid("world", world)
+
+    def moved(node, xyz):
+        node.transform = np.eye(4)
+        node.transform[3, :3] = xyz
+        return node
+    cluster = [fd.addPrim(Node.sphere(60.0), b_obj),
+               fd.addPrim(moved(Node.box3(80.0, 80.0, 80.0), (150.0, 0.0, 0.0)), b_obj),
+               fd.addPrim(moved(Node.cylinder(40.0, -50.0, 50.0), (-150.0, 0.0, 0.0)), b_obj),
+               fd.addPrim(moved(Node.zsphere(50.0, -20.0, 50.0), (0.0, 150.0, 0.0)), b_obj),
+               fd.addPrim(Node.sphere(30.0, (0.0, -150.0, 0.0)), b_obj)]
+    s1 = fd.addSolid("cluster", cluster)
+    fd.addInstance(np.eye(4), s0)
+    for k in range(8):
+        c = np.array([400.0 if k & 1 else -400.0, 400.0 if k & 2 else -400.0, 250.0 if k & 4 else -250.0])
+        t = rotation_z_to(c + np.array([30.0 * k, -20.0, 45.0]))
+        t[3, :3] = c
+        fd.addInstance(t, s1)
+    phys = lib.arrays()
+    phys["icdf"], phys["icdf_hd_factor"] = gaussian_icdf()
+    phys["boundary_names"] = lib.boundaries
+    gs = torch_genstep(num_photon, T_POINT_ISO, pos=(0.0, 0.0, 0.0), wavelength=440.0)[None]
+    return Workload("prim_clusters", fd.arrays(), phys, gs, seed=seed,
+                    meta=dict(n_world_prims=len(world), n_cluster_prims=len(cluster), b_obj=b_obj, b_world=b_world))
+
+
 def uniform_ball(rng, n, r):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)

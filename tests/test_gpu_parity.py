@@ -431,6 +431,42 @@ def test_list_zoo_trace_and_simulate_match_checker():
     sim.close()
 
 
+@pytest.mark.parametrize("rows", [(10, 7), (5, 4)])
+def test_multi_prim_solids_trace_and_simulate_match_checker(rows):
+    """Eight instances of a five-prim solid (their BLAS roots have several children)
+    and a world solid of 141 prims (above the 127 one leaf link holds: the front solid
+    is walked through its BLAS) or of 41 (tested as one leaf): nearest hits against
+    the checker's brute force, then full propagation"""
+    w = configs.prim_clusters(num_photon=200_000, rows=rows)
+    assert w.meta["n_world_prims"] == 1 + 2 * rows[0] * rows[1]
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=w.seed))
+    rng = np.random.default_rng(12)
+    n = 60000
+    rays = np.zeros((n, 2, 4), np.float32)
+    rays[:, 0, :3] = rng.uniform(-900, 900, (n, 3))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 1, :3] = d
+    rays[:, 0, 3] = rng.choice([0.0, 0.05, 1.0], n)
+    rays[:, 1, 3] = 1e6
+    rt = torch.from_numpy(rays).to(DEV)
+    it = torch.zeros((n, 2, 4), dtype=torch.float32, device=DEV)
+    sim.trace_device(rt, it)
+    g = it.cpu().numpy()
+    W = po.World(w.geometry, w.physics)
+    r = W.trace(rays, traverse=0)
+    assert np.array_equal(g[:, 0].view(np.uint32), r[:, 0].view(np.uint32))
+    assert np.array_equal(g[:, 1, :3].view(np.uint32), r[:, 1, :3].view(np.uint32))
+    inst = g[:, 1, 0].view(np.uint32) >> 16
+    bnd = g[:, 1, 1].view(np.uint32)
+    assert len(np.unique(inst[bnd != 0xFFFF])) == 9                        # the world and every cluster
+    assert ((inst == 0) & (bnd == w.meta["b_obj"])).sum() > 100           # beads of the world solid
+    hits = sim.simulate(w.gensteps, 0)
+    ref_p, ref_s = W.simulate(w.gensteps, seed=w.seed)
+    s, b = compare(sim.photon, sim.seq, ref_p, ref_s)
+    assert s >= 0.999 and len(hits) > 10000
+    sim.close()
+
+
 def test_torch_types_match_checker():
     """every storch type (point, line, disc, circle, rectangle, sphere, Marsaglia sphere)
     generated and propagated in the config0 world, bitwise with the checker"""
