@@ -144,7 +144,7 @@ def test_sphere_and_cylinder_closed_forms(exe):
     assert np.allclose(b[1], [-20.0, -100.0, -20.0, 20.0, 100.0, 20.0], atol=1e-4)
 
 
-@pytest.mark.parametrize("which", ["config1", "leaf_zoo", "boolean_zoo", "list_zoo"])
+@pytest.mark.parametrize("which", ["config1", "leaf_zoo", "boolean_zoo", "list_zoo", "prim_clusters"])
 def test_boxes_hold_every_hit_and_are_no_looser(exe, which):
     if which == "config1":
         w = configs.config1(num_gs=1, per_gs=1)
@@ -188,18 +188,18 @@ def hit_points(g, physics, n_per_inst, seed, traverse):
     return p, inst[hit]
 
 
-@pytest.mark.parametrize("which", ["config1", "config2", "rotated_config1", "boolean_zoo"])
+@pytest.mark.parametrize("which", ["config1", "config2", "rotated_config1", "boolean_zoo", "prim_clusters"])
 def test_free_ball_holds_no_hit(exe, which):
     """No hit the checker finds on a TLAS (non-front) instance lies inside the free ball,
-    on the detector shells (upright and randomly rotated detectors) and the rotated Boolean
-    zoo; on the shells the ball holds the LS sphere (and with upright detectors the acrylic
+    on the detector shells (upright and randomly rotated detectors), the rotated Boolean
+    zoo and the rotated five-prim clusters; on the shells the ball holds the LS sphere (and with upright detectors the acrylic
     sphere), where a photon's traces inside the shell end; with upright detectors a 2% larger
     ball would hold hits."""
     if which == "config2":
         w = configs.config2(num_events=1, mean_gs=1, mean_photons=1)
         g, n, traverse = dict(w.geometry), 6, 1
-    elif which == "boolean_zoo":
-        w = configs.boolean_zoo(num_photon=4)
+    elif which in ("boolean_zoo", "prim_clusters"):
+        w = getattr(configs, which)(num_photon=4)
         g, n, traverse = rotated(w, 7), 20000, 0
     else:
         w = configs.config1(num_gs=1, per_gs=1)
@@ -211,7 +211,7 @@ def test_free_ball_holds_no_hit(exe, which):
     dist = np.linalg.norm(p - c, axis=1)
     if r2 > 0:
         assert dist.min() > np.sqrt(r2), (dist.min(), np.sqrt(r2))
-    if which != "boolean_zoo":
+    if which not in ("boolean_zoo", "prim_clusters"):
         # upright detectors: the ball holds the acrylic sphere; randomly rotated ones
         # can point a side inward, and it holds the LS sphere
         inner = w.meta["ls_r"] + (120.0 if which != "rotated_config1" else 0.0)
