@@ -779,6 +779,10 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
         if (leaf == OA_BVH_EMPTY) continue;
         /* phase 2: the pending instance leaf */
         OA_CYC_MARK(c_p2);
+        /* the leaf phase is chains of dependent loads (instance record, BLAS node, prim
+           record, CSG nodes): waves in it issue first, so their next load starts sooner
+           (0.6% on the 64 events against no priority; raising the walk instead lost 1%) */
+        __builtin_amdgcn_s_setprio(2);
         const uint32_t first = leaf & 0x00ffffffu, count = (leaf >> 24) & 0x7fu;
         leaf = OA_BVH_EMPTY;
         for (uint32_t k = 0; k < count; ++k) {
@@ -825,6 +829,7 @@ OA_D void trace(const SimParams& P, uint32_t* stk, v3 o, v3 d, float tmin, float
             }
         }
         OA_CYC_ADD(OA_ST_C_P2, c_p2);
+        __builtin_amdgcn_s_setprio(0);
         /* after the front phase (the TLAS walk not begun: cur is still the root): a lane
            whose segment from the origin to the nearest front hit lies in the free ball,
            which no TLAS instance meets, has nothing to find in the TLAS; one whose origin
