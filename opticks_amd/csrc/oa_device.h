@@ -524,14 +524,12 @@ OA_D bool leaf_halfspace(Isect& is, float4 q0, float t_min, v3 o, v3 d)
 /* intersect_leaf (CSG/csg_intersect_leaf.h:174-324). Returns false for a miss.
    EXT: also the cone, convexpolyhedron, hyperboloid, phicut and halfspace leaves
    (a separate kernel instantiation, chosen when the geometry holds them). */
+/* a leaf node given by its parameter quads q0, q1 and its q3.z (typecode), q3.w (transform | complement) */
 template <bool EXT>
-OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, const float4* plan, float t_min, v3 ro, v3 rd)
+OA_D bool intersect_leaf_q(Isect& is, float4 q0, float4 q1, uint32_t typecode, uint32_t gtw, const float4* itra,
+                           const float4* plan, float t_min, v3 ro, v3 rd)
 {
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
-    const float4 q0 = nd[0];
-    const float4 q1 = nd[1];
-    const uint32_t typecode = __float_as_uint(nd[3].z);
-    const uint32_t gtw = __float_as_uint(nd[3].w);
     const uint32_t gt = gtw & 0x7fffffffu;
     const bool complement = (gtw & 0x80000000u) != 0u;
     const float4* q = gt > 0u ? itra + 4u * (gt - 1u) : nullptr;
@@ -563,6 +561,14 @@ OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, const 
         is.z = valid ? -is.z : is.z;
     }
     return valid;
+}
+
+template <bool EXT>
+OA_D bool intersect_leaf(Isect& is, const float4* nd, const float4* itra, const float4* plan, float t_min, v3 ro, v3 rd)
+{
+    const float4 q3 = nd[3];
+    return intersect_leaf_q<EXT>(is, nd[0], nd[1], __float_as_uint(q3.z), __float_as_uint(q3.w), itra, plan, t_min,
+                                 ro, rd);
 }
 
 /* ------------------------------------------------------------- CSG tree */
@@ -937,8 +943,11 @@ template <bool EXT>
 OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, const float4* plan, float t_min, v3 ro,
                          v3 rd)
 {
-    const uint32_t typecode = __float_as_uint(root[3].z);
-    if (typecode >= TC_LEAF) return intersect_leaf<EXT>(is, root, itra, plan, t_min, ro, rd);
+    /* the root's quads loaded together: a leaf root needs no second round trip */
+    const float4 q0 = root[0], q1 = root[1], q3 = root[3];
+    const uint32_t typecode = __float_as_uint(q3.z);
+    if (typecode >= TC_LEAF)
+        return intersect_leaf_q<EXT>(is, q0, q1, typecode, __float_as_uint(q3.w), itra, plan, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     if (typecode < TC_NODE) {
         OA_COUNT(OA_ST_TREE);
