@@ -20,7 +20,7 @@ the table/BVH/CSG/transform fetches, PMC traffic) and the measured VALU issue
 rate and lane utilisation (profiles/pmc_*.json) beside it. cpu_baseline: the
 CPU checker on whole events of the same workload for a bounded time.
 
-Extra legs: the 10M-photon single event with its own roofline and
+Extra legs: the 10M-photon single event (2,000 detectors) with its own roofline and
 CPU rate, the adjoint/forward ratio with 256 binned LS/water lengths (10M
 photons, 2,000 detectors), the 256-event forward + adjoint captured graph, and
 the rows around the core path (hit merge, sphotonlite output, Cerenkov
@@ -201,7 +201,7 @@ def bench_adjoint(torch, dev, steps, regroup=0, batch=1, tape=32):
 
 
 def batched_workload():
-    """64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons,
+    """The batched workload: 64 events x Poisson(200) scintillation gensteps x Poisson(5000) photons,
     20,000 detectors at 19.4 m around a 17.7 m LS sphere (opticks_amd.configs.config2)"""
     from opticks_amd import configs
     w = configs.config2()
@@ -458,7 +458,7 @@ def cpu_baseline_events(w, seconds):
 
 
 def bench_config1(torch, dev, steps, regroup, batch, cpu):
-    """the 10M-photon single event (1,000 torch gensteps x 10,000 photons,
+    """The 10M-photon single event (1,000 torch gensteps x 10,000 photons,
     2,000 detectors), one simulate call per step with gensteps resident: seeding,
     propagation, stable hit compaction into preallocated buffers."""
     from opticks_amd import configs

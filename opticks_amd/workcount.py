@@ -74,8 +74,8 @@ def run(which):
         read_counters(lib, reset=True)
         sim.simulate_batch(gs, offs, ev, photon)
         torch.cuda.synchronize()
-        desc = ("batched-64-events (configs[2] layout, 20000 detectors)" if which == "batched64" else
-                "256 events (configs[4] layout: 20000 detectors, binned LS/water lengths)") + ", library launch defaults"
+        desc = ("batched-64-events (20000 detectors at 19.4 m around a 17.7 m LS sphere)" if which == "batched64" else
+                "256 events (20000 detectors, binned LS/water lengths)") + ", library launch defaults"
     elif which == "csg10M":
         w = configs.config1()
         N = w.num_photon
@@ -86,7 +86,7 @@ def run(which):
         read_counters(lib, reset=True)
         sim.simulate_device(gs, photon, n_photon=N)
         torch.cuda.synchronize()
-        desc = "csg-boolean-10M (configs[1], 2000 detectors), trace_regroup_pct 25, refill_batch 32"
+        desc = "csg-boolean-10M (1000 torch gensteps x 10000 photons, 2000 detectors), trace_regroup_pct 25, refill_batch 32"
     else:
         raise ValueError(which)
     c = read_counters(lib)

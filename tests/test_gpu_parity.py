@@ -2,9 +2,9 @@
 
 Bar: integer/bit outputs (seqhis, seqbnd, flags, indices) identical; photon
 floats identical bit for bit (both sides share operand order, no FMA
-contraction, correctly rounded sqrt/div and the same log/sin/cos), with the
-north-star floor of >= 99.9% seqhis agreement and 1e-3 mm / 1e-3 ns kept as
-the asserted tolerance for the rare photons that could differ.
+contraction, correctly rounded sqrt/div and the same log/sin/cos), with a
+floor of >= 99.9% seqhis agreement and 1e-3 mm / 1e-3 ns kept as the
+asserted tolerance for the rare photons that could differ.
 """
 import numpy as np
 import pytest
