@@ -227,6 +227,22 @@ def test_free_ball_absent_without_tlas_instances(exe):
     assert front.all() and r2 <= 0
 
 
+def test_moved_detectors_take_the_world_out_of_the_front(exe):
+    """detectors moved to three times their radius (outside the world box): the union of
+    the instance boxes grows so that the world's box is under a quarter of its area, and a
+    TLAS build makes no instance front (the premise of the GPU test that rebuilds the TLAS
+    after such a refit); the unmoved layout keeps the world front"""
+    w = configs.config1(num_gs=1, per_gs=1)
+    g = dict(w.geometry)
+    front, _, _ = front_and_ball(exe, g, "config1_unmoved")
+    assert front[0] and front.sum() == 1
+    moved = g["inst"].copy()
+    moved[1:, 3, :3] *= np.float32(3.0)
+    g["inst"] = moved
+    front, _, r2 = front_and_ball(exe, g, "config1_moved")
+    assert front.sum() == 0 and r2 <= 0
+
+
 def test_free_ball_with_scaled_detectors(exe):
     """detectors scaled by 1.2 about their own positions: the placed local box no longer
     bounds their distance (the transform is not rigid), so the ball falls back to the world

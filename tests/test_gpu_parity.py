@@ -276,6 +276,43 @@ def test_refit_after_moving_instances(scale):
     sim.close()
 
 
+def test_tlas_rebuild_after_refit_changes_the_front_set():
+    """Detectors moved to three times their radius, outside the world box: a TLAS rebuild
+    (a new leaf size) after the refit no longer keeps the world out of the TLAS, and its
+    instance record keeps the one-leaf link over its three prims that it got as a front
+    instance, now reached through a TLAS leaf: trace and simulate match the checker."""
+    w = configs.config1(num_gs=10, per_gs=1000)
+    sim = QSim(w.geometry, w.physics, SEventConfig(seed=4))
+    moved = w.geometry["inst"].copy()
+    moved[1:, 3, :3] *= np.float32(3.0)
+    sim.refit_instances(moved)
+    sim.set_config(SEventConfig(seed=4, bvh_leaf_size=2))
+    g2 = dict(w.geometry); g2["inst"] = moved
+    W = po.World(g2, w.physics)
+    rng = np.random.default_rng(5)
+    n = 40000
+    rays = np.zeros((n, 2, 4), np.float32)
+    rays[:, 0, :3] = rng.uniform(-20000, 20000, (n, 3))
+    rays[: n // 2, 0, :3] = rng.uniform(-6000, 6000, (n // 2, 3))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 1, :3] = d
+    rays[:, 1, 3] = 1e6
+    rt = torch.from_numpy(rays).to(DEV)
+    it = torch.zeros((n, 2, 4), dtype=torch.float32, device=DEV)
+    sim.trace_device(rt, it)
+    g = it.cpu().numpy()
+    r = W.trace(rays, traverse=0)
+    assert np.array_equal(g[:, 0].view(np.uint32), r[:, 0].view(np.uint32))
+    assert np.array_equal(g[:, 1, :3].view(np.uint32), r[:, 1, :3].view(np.uint32))
+    inst = g[:, 1, 0].view(np.uint32) >> 16
+    hit = g[:, 1, 1].view(np.uint32) != 0xFFFF
+    assert (hit & (inst == 0)).sum() > 1000 and (hit & (inst > 0)).sum() > 100
+    sim.simulate(w.gensteps, 0)
+    rp, rs = W.simulate(w.gensteps, seed=4)
+    compare(sim.photon, sim.seq, rp, rs)
+    sim.close()
+
+
 def test_leaf_zoo_trace_and_simulate_match_checker():
     """cone, hyperboloid, convex polyhedra, phicut/halfspace trees and a transformed cone:
     nearest hits against the checker's brute force, then full propagation"""
