@@ -927,15 +927,17 @@ OA_D int intersect_tree_small(Isect& out, const float4* root, const float4* itra
     return out.w > 0.f ? 1 : 0;
 }
 
-/* deep trees (height 4..OA_CSG_MAX_HEIGHT): out of line, result returned by value */
-template <bool EXT>
+/* the general algorithm out of line (stack depth D: 4 up to height 3, 8 deeper), result
+   returned by value: it runs only for trees the straight-line evaluator hands over, so
+   its code stays out of the propagation loop */
+template <int D, bool EXT>
 __device__ static __attribute__((noinline)) float4 intersect_tree_deep(const float4* root, const float4* itra,
                                                                       const float4* plan,
                                                                       float t_min, float ox, float oy, float oz,
                                                                       float dx, float dy, float dz)
 {
     Isect r;
-    intersect_tree_t<8, EXT>(r, root, itra, plan, t_min, mk(ox, oy, oz), mk(dx, dy, dz));
+    intersect_tree_t<D, EXT>(r, root, itra, plan, t_min, mk(ox, oy, oz), mk(dx, dy, dz));
     return make_float4(r.x, r.y, r.z, r.w);
 }
 
@@ -957,13 +959,11 @@ OA_D bool intersect_prim(Isect& is, const float4* root, const float4* itra, cons
         int h = numNode <= 7u ? intersect_tree_small<EXT>(is, root, itra, plan, t_min, ro, rd) : -1;
         if (h < 0) {
             OA_COUNT(OA_ST_TREE_GENERAL);
-            if (numNode <= 15u) {
-                h = intersect_tree_t<4, EXT>(is, root, itra, plan, t_min, ro, rd) ? 1 : 0;
-            } else {
-                const float4 r = intersect_tree_deep<EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
-                is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
-                h = is.w > 0.f ? 1 : 0;
-            }
+            const float4 r = numNode <= 15u
+                ? intersect_tree_deep<4, EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z)
+                : intersect_tree_deep<8, EXT>(root, itra, plan, t_min, ro.x, ro.y, ro.z, rd.x, rd.y, rd.z);
+            is.x = r.x; is.y = r.y; is.z = r.z; is.w = r.w;
+            h = is.w > 0.f ? 1 : 0;
         }
         OA_CYC_ADD(OA_ST_C_TREE, c_tree);
         return h != 0;
