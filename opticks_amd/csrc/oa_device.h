@@ -114,7 +114,30 @@ struct PrecookedRng {
     int32_t cursor, nk;
 };
 
-OA_D float uni(PhiloxRng& r) { return oa_rng_uniform(&r.ph); }
+/* the Philox block out of line: one copy of its ten rounds instead of one per draw site */
+__device__ static __attribute__((noinline)) uint4 philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                                uint32_t k0, uint32_t k1)
+{
+    oa_u32x4 c;
+    c.v[0] = c0; c.v[1] = c1; c.v[2] = c2; c.v[3] = c3;
+    const oa_u32x4 o = oa_philox4x32_10(c, k0, k1);
+    return make_uint4(o.v[0], o.v[1], o.v[2], o.v[3]);
+}
+/* oa_rng_uniform with the block generated out of line (the same words) */
+OA_D float uni(PhiloxRng& r)
+{
+    oa_philox_stream* s = &r.ph;
+    const uint32_t x = s->b0;
+    s->b0 = s->b1; s->b1 = s->b2; s->b2 = s->b3;
+    s->pos += 1u;
+    if ((s->pos & 3u) == 0u) {
+        const uint64_t block = s->pos >> 2;
+        const uint4 o = philox_block((uint32_t)block, (uint32_t)(block >> 32), (uint32_t)s->photon,
+                                     (uint32_t)(s->photon >> 32), (uint32_t)s->key, (uint32_t)(s->key >> 32));
+        s->b0 = o.x; s->b1 = o.y; s->b2 = o.z; s->b3 = o.w;
+    }
+    return oa_u32_to_uniform(x);
+}
 OA_D float uni(XorwowRng& r) { return oa_xorwow_uniform(&r.xw); }
 OA_D float uni(PrecookedRng& r)
 {
