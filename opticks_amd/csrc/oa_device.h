@@ -105,6 +105,9 @@ enum : uint32_t { TC_ZERO = 0, TC_UNION = 1, TC_INTERSECTION = 2, TC_DIFFERENCE 
 enum : int { RNG_PHILOX = 0, RNG_PRECOOKED = 1, RNG_XORWOW = 2 };
 
 struct PhiloxRng { oa_philox_stream ph; };
+/* the same stream with its block generated inline at each draw site (the taped
+   forward's instantiation, whose register allocation measured better with it) */
+struct PhiloxRngInline { oa_philox_stream ph; };
 struct XorwowRng { oa_xorwow_state xw; };
 /* precooked uniforms (the reference's s_seq aligned-randoms mode): a row of nk
    values per photon; the cursor wraps modulo nk as s_seq::flat recycles
@@ -138,6 +141,7 @@ OA_D float uni(PhiloxRng& r)
     }
     return oa_u32_to_uniform(x);
 }
+OA_D float uni(PhiloxRngInline& r) { return oa_rng_uniform(&r.ph); }
 OA_D float uni(XorwowRng& r) { return oa_xorwow_uniform(&r.xw); }
 OA_D float uni(PrecookedRng& r)
 {

@@ -81,6 +81,11 @@ OA_D void rng_start(const SimParams& P, PhiloxRng& r, int e, uint64_t photon_idx
     oa_rng_init(&r.ph, P.seed, P.rng_offset, P.skipahead, (uint64_t)P.ev_id[e], photon_idx);
 }
 
+OA_D void rng_start(const SimParams& P, PhiloxRngInline& r, int e, uint64_t photon_idx)
+{
+    oa_rng_init(&r.ph, P.seed, P.rng_offset, P.skipahead, (uint64_t)P.ev_id[e], photon_idx);
+}
+
 OA_D void rng_start(const SimParams& P, XorwowRng& r, int e, uint64_t photon_idx)
 {
     oa_xorwow_seed(&r.xw, P.seed, photon_idx);

@@ -14,10 +14,13 @@
 
 #if OA_SIM_RNG == 0
 #define OA_SIM_R oa::PhiloxRng
+#define OA_SIM_RT oa::PhiloxRngInline     /* the taped forward: the same stream, block inlined */
 #elif OA_SIM_RNG == 1
 #define OA_SIM_R oa::PrecookedRng
+#define OA_SIM_RT OA_SIM_R
 #else
 #define OA_SIM_R oa::XorwowRng
+#define OA_SIM_RT OA_SIM_R
 #endif
 
 namespace oa_launch {
@@ -36,8 +39,8 @@ hipError_t OA_SIM_NAME(sim_launch, OA_SIM_BLOCK, OA_SIM_RNG)(const SimParams& P,
         }
     } else if (P.tape) {
         /* a forward for a following adjoint: segment records on the tape (no lite output) */
-        if (f & 2) hipLaunchKernelGGL((k_sim_wave<B, 2, OA_SIM_R, oa::RecordTape>), dim3((unsigned)grid), dim3(B), 0, s, P);
-        else hipLaunchKernelGGL((k_sim_wave<B, 0, OA_SIM_R, oa::RecordTape>), dim3((unsigned)grid), dim3(B), 0, s, P);
+        if (f & 2) hipLaunchKernelGGL((k_sim_wave<B, 2, OA_SIM_RT, oa::RecordTape>), dim3((unsigned)grid), dim3(B), 0, s, P);
+        else hipLaunchKernelGGL((k_sim_wave<B, 0, OA_SIM_RT, oa::RecordTape>), dim3((unsigned)grid), dim3(B), 0, s, P);
     } else {
         switch (f) {
             case 0: hipLaunchKernelGGL((k_sim_wave<B, 0, OA_SIM_R>), dim3((unsigned)grid), dim3(B), 0, s, P); break;
@@ -86,3 +89,4 @@ int OA_SIM_NAME(adjoint_occupancy, OA_ADJ_BLOCK, OA_SIM_RNG)()
 } // namespace oa_launch
 
 #undef OA_SIM_R
+#undef OA_SIM_RT
