@@ -108,6 +108,10 @@ struct PhiloxRng { oa_philox_stream ph; };
 /* the same stream with its block generated inline at each draw site (the taped
    forward's instantiation, whose register allocation measured better with it) */
 struct PhiloxRngInline { oa_philox_stream ph; };
+/* whether the physics of a kernel drawing from R calls the table-index wrap's integer
+   divisions out of line (a smaller kernel), or inline (the taped forward) */
+template <class R> struct wrap_ool { static constexpr bool value = true; };
+template <> struct wrap_ool<PhiloxRngInline> { static constexpr bool value = false; };
 struct XorwowRng { oa_xorwow_state xw; };
 /* precooked uniforms (the reference's s_seq aligned-randoms mode): a row of nk
    values per photon; the cursor wraps modulo nk as s_seq::flat recycles

@@ -22,15 +22,21 @@ namespace oa {
 /* Wrap addressing of a table of n texels: i0 = i mod n in [0, n) and its
    neighbour (i0 + 1) mod n. The integer divisions run only for an index
    outside the table (wavelengths beyond the domain), the results are the same. */
+__device__ static __attribute__((noinline)) int wrap_index(int i, int n) { return ((i % n) + n) % n; }
+
+/* OOL: the integer divisions out of line (a smaller kernel; the taped forward's register
+   allocation does better with them inline: wrap_ool) */
+template <bool OOL = true>
 OA_D void wrap_pair(int i, int n, int& i0, int& i1)
 {
     i0 = i;
-    if (i0 < 0 || i0 >= n) i0 = ((i0 % n) + n) % n;
+    if (i0 < 0 || i0 >= n) i0 = OOL ? wrap_index(i0, n) : ((i0 % n) + n) % n;
     i1 = i0 + 1 == n ? 0 : i0 + 1;
 }
 
 /* One lerped float4 of bnd row (2*line+k) at wavelength nm: the FP32 form of
    the linearly filtered, wrap-addressed texture fetch of qbnd::boundary_lookup. */
+template <bool OOL = true>
 OA_D float4 bnd_lookup(const SimParams& P, float nm, int line, int k)
 {
     const int nwl = P.n_wl;
@@ -38,7 +44,7 @@ OA_D float4 bnd_lookup(const SimParams& P, float nm, int line, int k)
     float fl = floorf(fx);
     float a = fx - fl;
     int i0, i1;
-    wrap_pair((int)fl, nwl, i0, i1);
+    wrap_pair<OOL>((int)fl, nwl, i0, i1);
     const float4* row = P.bnd + (int64_t)(2 * line + k) * nwl;
     float4 v0 = row[i0];
     float4 v1 = row[i1];
@@ -106,6 +112,7 @@ OA_D const uint32_t* tape_record(const SimParams& P, int64_t g, int j)
     return P.tape + 3 * ((int64_t)j * P.tape_stride + g);
 }
 
+template <bool OOL = true>
 OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float cosTheta)
 {
     const int line = (int)boundary * 4;
@@ -113,13 +120,14 @@ OA_D void fill(const SimParams& P, State& s, uint32_t boundary, float wl, float 
     const int m2_line = cosTheta > 0.f ? line + 0 : line + 3;
     const int su_line = cosTheta > 0.f ? line + 2 : line + 1;
     s.m1_line = m1_line;
-    s.material1 = bnd_lookup(P, wl, m1_line, 0);
-    s.group_velocity = bnd_lookup(P, wl, m1_line, 1).x;
-    s.n2 = bnd_lookup(P, wl, m2_line, 0).x;
+    s.material1 = bnd_lookup<OOL>(P, wl, m1_line, 0);
+    s.group_velocity = bnd_lookup<OOL>(P, wl, m1_line, 1).x;
+    s.n2 = bnd_lookup<OOL>(P, wl, m2_line, 0).x;
     s.ems = P.optical[4 * su_line + 1];
-    s.surface = (s.ems != EMS_NOSURFACE) ? bnd_lookup(P, wl, su_line, 0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    s.surface = (s.ems != EMS_NOSURFACE) ? bnd_lookup<OOL>(P, wl, su_line, 0) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+template <bool OOL = true>
 OA_D float icdf_tex(const SimParams& P, float x, int layer)
 {
     const int n = P.n_icdf;
@@ -127,25 +135,26 @@ OA_D float icdf_tex(const SimParams& P, float x, int layer)
     float fl = floorf(xb);
     float a = xb - fl;
     int i0, i1;
-    wrap_pair((int)fl, n, i0, i1);
+    wrap_pair<OOL>((int)fl, n, i0, i1);
     int L = layer < P.n_icdf_layer ? layer : 0;
     const float* row = P.icdf + (int64_t)L * n;
     float v0 = row[i0], v1 = row[i1];
     return v0 + a * (v1 - v0);
 }
 
+template <bool OOL = true>
 OA_D float scint_wavelength(const SimParams& P, float u0)
 {
-    if (P.icdf_hd == 0) return icdf_tex(P, u0, 0);
+    if (P.icdf_hd == 0) return icdf_tex<OOL>(P, u0, 0);
     if (P.icdf_hd == 10) {
         bool lhs = u0 < 0.1f, rhs = u0 > 0.9f;
         float x = lhs ? (u0 * 10.f) : (rhs ? (u0 - 0.9f) * 10.f : u0);
-        return icdf_tex(P, x, lhs ? 1 : (rhs ? 2 : 0));
+        return icdf_tex<OOL>(P, x, lhs ? 1 : (rhs ? 2 : 0));
     }
     if (P.icdf_hd == 20) {
         bool lhs = u0 < 0.05f, rhs = u0 > 0.95f;
         float x = lhs ? (u0 * 20.f) : (rhs ? (u0 - 0.95f) * 20.f : u0);
-        return icdf_tex(P, x, lhs ? 1 : (rhs ? 2 : 0));
+        return icdf_tex<OOL>(P, x, lhs ? 1 : (rhs ? 2 : 0));
     }
     return 0.f;
 }
@@ -270,7 +279,7 @@ OA_D int to_boundary(const SimParams& P, uint32_t& flag, R& rng, Photon& p, cons
             float u_reemit = reemission_prob == 0.f ? 2.f : uni(rng);
             if (u_reemit < reemission_prob) {
                 float u_wl = uni(rng), u_mph = uni(rng), u_mct = uni(rng), u_pph = uni(rng), u_pct = uni(rng);
-                p.wavelength = scint_wavelength(P, u_wl);
+                p.wavelength = scint_wavelength<wrap_ool<R>::value>(P, u_wl);
                 p.mom = uniform_sphere(u_mph, u_mct);
                 p.pol = normalize(cross(uniform_sphere(u_pph, u_pct), p.mom));
                 flag = F_BULK_REEMIT;
@@ -373,7 +382,7 @@ OA_D int propagate(const SimParams& P, R& rng, Photon& p, const Prd& prd, Tape& 
     p.obf = (p.obf & 0x7fffffffu) | ((cosTheta < 0.f ? 0x1u : 0x0u) << 31);
     p.hitcount_iindex = 0x00010000u | (0x0000ffffu & iindex);
     State s;
-    fill(P, s, boundary, p.wavelength, cosTheta);
+    fill<wrap_ool<R>::value>(P, s, boundary, p.wavelength, cosTheta);
     uint32_t flag = 0u;
     int command = to_boundary(P, flag, rng, p, s, prd.t, tape);
     if (command == CMD_BOUNDARY) {
@@ -530,7 +539,7 @@ OA_D void gen_scint(const SimParams& P, Photon& p, R& rng, const float4* gq)
     phi = 2.f * OA_PI_F * u2;
     oa_sincosf(phi, &sinp, &cosp);
     p.pol = normalize(cosp * p.pol + sinp * cross(p.mom, p.pol));
-    p.wavelength = scint_wavelength(P, u3);
+    p.wavelength = scint_wavelength<wrap_ool<R>::value>(P, u3);
     const float4 q1 = gq[1], q2 = gq[2], q3 = gq[3], q5 = gq[5];
     const float charge = q3.y, meanVelocity = q3.w, step_length = q2.w, scint_time = q5.x;
     float fraction = charge == 0.f ? 1.f : uni(rng);
@@ -561,7 +570,7 @@ OA_D void gen_cerenkov(const SimParams& P, Photon& p, R& rng, const float4* gq)
         const float u0 = uni(rng);
         const float w = Wmin + u0 * (Wmax - Wmin);
         wavelength = Wmin * Wmax / w;
-        const float4 props = bnd_lookup(P, wavelength, matline, 0);
+        const float4 props = bnd_lookup<wrap_ool<R>::value>(P, wavelength, matline, 0);
         const float sampledRI = props.x;
         cosTheta = BetaInverse / sampledRI;
         sin2Theta = fmaxf(0.f, (1.f - cosTheta) * (1.f + cosTheta));
